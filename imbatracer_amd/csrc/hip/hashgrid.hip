@@ -1,0 +1,556 @@
+// Photon hash grid on the GPU: counting-sort build and k-capped range query.
+//
+// HashGrid (rangesearch/rangesearch.h:21-158) builds on the host with TBB: bounds reduce,
+// atomic count per bucket, serial exclusive scan, atomic scatter.  Here:
+//   bounds   k_bounds (per-block min/max) + k_params (combine, grow by 0.1%, derived values)
+//   count    k_count_buckets: bucket per photon, atomicAdd ticket per photon
+//   scan     k_scan_tiles / k_scan_top / k_scan_apply: bucket ends (inclusive scan)
+//   scatter  k_scatter: slot = bucket start + ticket
+//   order    k_order_small / k_order_large: each bucket's slots sorted by photon index, so the
+//            layout is deterministic and equals a serial run of the reference's scatter
+//   gather   positions copied to slot order as float4 (x, y, z, index bits)
+// The query visits the same 8 buckets in the same order as the reference (duplicates
+// included) and keeps a k-capped, distance-sorted list in registers with the reference's
+// insertion rule.  Squared distances are computed without fused multiply-add, as on the host.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+
+#include "common.h"
+
+namespace imba {
+
+struct GridParams {
+    float bmin[3], bmax[3];
+    float radius, radius_sqr, cell_size, inv_cell_size;
+    int table_size;
+    int count;
+};
+
+}  // namespace imba
+
+struct ImbaHashGrid {
+    int device = 0;
+    int table_size = 0;  // grows only
+    int count = 0;
+    int cap = 0;         // photon capacity of the per-photon arrays
+    int* cell_ends = nullptr;
+    int* counts = nullptr;
+    int* order = nullptr;
+    int* bucket = nullptr;
+    int* ticket = nullptr;
+    float4* pos4 = nullptr;
+    imba::GridParams* d_params = nullptr;
+    float* partial = nullptr;  // 6 per bounds block
+    int* scan_tmp = nullptr;   // tile sums
+    int scan_cap = 0;
+    int* large_list = nullptr; // buckets too large for one thread, then their count at [table]
+    int large_cap = 0;
+};
+
+namespace imba {
+namespace {
+
+constexpr int kBoundsBlocks = 1024;
+constexpr int kBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+constexpr int kSmallBucket = 32;
+constexpr int kLdsSort = 4096;
+
+__device__ __forceinline__ const float* photon_pos(const unsigned char* base, int stride, int off, int i) {
+    return reinterpret_cast<const float*>(base + (size_t)i * stride + off);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bounds(const unsigned char* ph, int stride, int off, int n, float* partial) {
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const float* p = photon_pos(ph, stride, off, i);
+        for (int k = 0; k < 3; k++) {
+            const float v = p[k];
+            lo[k] = lo[k] < v ? lo[k] : v;
+            hi[k] = hi[k] > v ? hi[k] : v;
+        }
+    }
+    __shared__ float red[6][kBlock];
+    for (int k = 0; k < 3; k++) { red[k][threadIdx.x] = lo[k]; red[3 + k][threadIdx.x] = hi[k]; }
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            for (int k = 0; k < 3; k++) {
+                const float a = red[k][threadIdx.x], b = red[k][threadIdx.x + s];
+                red[k][threadIdx.x] = a < b ? a : b;
+                const float c = red[3 + k][threadIdx.x], d = red[3 + k][threadIdx.x + s];
+                red[3 + k][threadIdx.x] = c > d ? c : d;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) partial[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_params(const float* partial, int nparts, GridParams* gp, float radius, int table_size, int count) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int b = 0; b < nparts; b++)
+        for (int k = 0; k < 3; k++) {
+            const float a = partial[6 * b + k], c = partial[6 * b + 3 + k];
+            lo[k] = lo[k] < a ? lo[k] : a;
+            hi[k] = hi[k] > c ? hi[k] : c;
+        }
+    for (int k = 0; k < 3; k++) {
+        const float e = hi[k] - lo[k];
+        gp->bmax[k] = hi[k] + e * 0.001f;
+        gp->bmin[k] = lo[k] - e * 0.001f;
+    }
+    gp->radius = radius;
+    gp->radius_sqr = radius * radius;
+    gp->cell_size = radius * 2.0f;
+    gp->inv_cell_size = 1.0f / gp->cell_size;
+    gp->table_size = table_size;
+    gp->count = count;
+}
+
+// Zero fill (a kernel rather than a memset node, so the build can be captured in a graph).
+__global__ __launch_bounds__(kBlock) void k_zero(int* p, int n) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) p[i] = 0;
+}
+
+__device__ __forceinline__ int bucket_of(uint32_t x, uint32_t y, uint32_t z, int table) {
+    return (int)(((x * 73856093u) ^ (y * 19349663u) ^ (z * 83492791u)) % (uint32_t)table);
+}
+
+__global__ __launch_bounds__(kBlock) void k_count_buckets(const unsigned char* ph, int stride, int off, int n,
+                                                          const GridParams* gp, int* bucket, int* ticket, int* counts) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float* p = photon_pos(ph, stride, off, i);
+    const float inv = gp->inv_cell_size;
+    const int cx = (int)floorf(inv * (p[0] - gp->bmin[0]));
+    const int cy = (int)floorf(inv * (p[1] - gp->bmin[1]));
+    const int cz = (int)floorf(inv * (p[2] - gp->bmin[2]));
+    const int b = bucket_of((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, gp->table_size);
+    bucket[i] = b;
+    ticket[i] = atomicAdd(&counts[b], 1);
+}
+
+// Inclusive scan of `n` ints: per-tile sums, a one-block scan of the sums, then each tile is
+// scanned again with its offset.
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(const int* in, int n, int* tile_sums) {
+    __shared__ int ws[kBlock / 64];
+    const int base = blockIdx.x * kScanTile;
+    int s = 0;
+    for (int it = 0; it < kScanItems; it++) {
+        const int i = base + it * kBlock + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kBlock / 64; w++) t += ws[w];
+        tile_sums[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(int* sums, int nt) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (nt + 1023) / 1024;
+    const int b0 = t * per, b1 = min(nt, b0 + per);
+    int s = 0;
+    for (int b = b0; b < b1; b++) s += sums[b];
+    part[t] = s;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int v = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - s;
+    for (int b = b0; b < b1; b++) {
+        const int c = sums[b];
+        sums[b] = run;
+        run += c;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_apply(const int* in, int n, const int* tile_offsets, int* out) {
+    __shared__ int wsum[kBlock / 64];
+    const int base = blockIdx.x * kScanTile;
+    int run = tile_offsets[blockIdx.x];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int it = 0; it < kScanItems; it++) {
+        const int i = base + it * kBlock + threadIdx.x;
+        const int v = i < n ? in[i] : 0;
+        // inclusive wave scan
+        int x = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        int woff = 0, total = 0;
+        for (int w = 0; w < kBlock / 64; w++) {
+            woff += w < wave ? wsum[w] : 0;
+            total += wsum[w];
+        }
+        if (i < n) out[i] = run + woff + x;
+        run += total;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(int n, const int* bucket, const int* ticket, const int* counts,
+                                                    const int* ends, int* order) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int b = bucket[i];
+    order[ends[b] - counts[b] + ticket[i]] = i;
+}
+
+// Buckets of up to kSmallBucket photons are ordered by one thread (insertion sort in place);
+// larger ones are listed for the block-wide pass.
+__global__ __launch_bounds__(kBlock) void k_order_small(int table, const int* counts, const int* ends, int* order,
+                                                        int* large_list, int* large_count) {
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= table) return;
+    const int c = counts[b];
+    if (c <= 1) return;
+    if (c > kSmallBucket) {
+        large_list[atomicAdd(large_count, 1)] = b;
+        return;
+    }
+    int* seg = order + (ends[b] - c);
+    for (int i = 1; i < c; i++) {
+        const int x = seg[i];
+        int j = i - 1;
+        while (j >= 0 && seg[j] > x) {
+            seg[j + 1] = seg[j];
+            j--;
+        }
+        seg[j + 1] = x;
+    }
+}
+
+// One block per large bucket: bitonic sort in LDS when it fits, otherwise each element's
+// final position is its rank (number of smaller indices; indices are distinct).
+__global__ __launch_bounds__(1024) void k_order_large(const int* large_list, const int* large_count, const int* counts,
+                                                      const int* ends, int* order, int* tmp) {
+    __shared__ int buf[kLdsSort];
+    const int nl = *large_count;
+    for (int li = blockIdx.x; li < nl; li += gridDim.x) {
+        const int b = large_list[li];
+        const int c = counts[b];
+        int* seg = order + (ends[b] - c);
+        if (c <= kLdsSort) {
+            int p2 = 1;
+            while (p2 < c) p2 <<= 1;
+            for (int i = threadIdx.x; i < p2; i += blockDim.x) buf[i] = i < c ? seg[i] : INT32_MAX;
+            __syncthreads();
+            for (int k = 2; k <= p2; k <<= 1) {
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    for (int i = threadIdx.x; i < p2; i += blockDim.x) {
+                        const int l = i ^ j;
+                        if (l > i) {
+                            const int x = buf[i], y = buf[l];
+                            const bool up = (i & k) == 0;
+                            if ((x > y) == up) { buf[i] = y; buf[l] = x; }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            for (int i = threadIdx.x; i < c; i += blockDim.x) seg[i] = buf[i];
+            __syncthreads();
+        } else {
+            int* dst = tmp + (ends[b] - c);
+            for (int i = threadIdx.x; i < c; i += blockDim.x) {
+                const int x = seg[i];
+                int rank = 0;
+                for (int j = 0; j < c; j++) rank += seg[j] < x;
+                dst[rank] = x;
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < c; i += blockDim.x) seg[i] = dst[i];
+            __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_pos(const unsigned char* ph, int stride, int off, int n, const int* order,
+                                                       float4* pos4) {
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const int i = order[s];
+    const float* p = photon_pos(ph, stride, off, i);
+    pos4[s] = make_float4(p[0], p[1], p[2], __int_as_float(i));
+}
+
+template <int KMAX>
+__global__ __launch_bounds__(kBlock) void k_query(const GridParams* gp, const int* ends, const float4* pos4,
+                                                  const unsigned char* qs, int qstride, int qoff, int nq, int k,
+                                                  int* out_idx, float* out_dist, int* out_count) {
+    const int q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= nq) return;
+    const float* qp = photon_pos(qs, qstride, qoff, q);
+    const float qx = qp[0], qy = qp[1], qz = qp[2];
+    float dist[KMAX];
+    int idx[KMAX];
+#pragma unroll
+    for (int l = 0; l < KMAX; l++) { dist[l] = 0.0f; idx[l] = -1; }
+    int count = 0;
+    const GridParams g = *gp;
+    const bool inside = qx >= g.bmin[0] && qy >= g.bmin[1] && qz >= g.bmin[2] &&
+                        qx <= g.bmax[0] && qy <= g.bmax[1] && qz <= g.bmax[2];
+    if (inside && g.count > 0) {
+        const float cx = g.inv_cell_size * (qx - g.bmin[0]);
+        const float cy = g.inv_cell_size * (qy - g.bmin[1]);
+        const float cz = g.inv_cell_size * (qz - g.bmin[2]);
+        const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
+        const int px = (int)fx, py = (int)fy, pz = (int)fz;
+        const int pxo = px + ((cx - fx) < 0.5f ? -1 : 1);
+        const int pyo = py + ((cy - fy) < 0.5f ? -1 : 1);
+        const int pzo = pz + ((cz - fz) < 0.5f ? -1 : 1);
+        for (int j = 0; j < 8; j++) {
+            const int x = (j & 4) ? pxo : px;
+            const int y = (j & 2) ? pyo : py;
+            const int z = (j & 1) ? pzo : pz;
+            const int b = bucket_of((uint32_t)x, (uint32_t)y, (uint32_t)z, g.table_size);
+            const int begin = b == 0 ? 0 : ends[b - 1];
+            const int end = ends[b];
+            for (int s = begin; s < end; s++) {
+                const float4 p = pos4[s];
+                const float dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                if (!(d2 <= g.radius_sqr)) continue;
+                float last = 0.0f;
+#pragma unroll
+                for (int l = 0; l < KMAX; l++)
+                    if (l == k - 1) last = dist[l];
+                if (count == k) {
+                    if (last < d2) continue;
+                } else {
+                    count++;
+                }
+                // keep the first count-1 entries, insert d2 after every entry <= d2
+                const int keep = count - 1;
+                int pos = 0;
+#pragma unroll
+                for (int l = 0; l < KMAX; l++) pos += (l < keep && dist[l] <= d2) ? 1 : 0;
+#pragma unroll
+                for (int l = KMAX - 1; l >= 1; l--) {
+                    if (l <= keep && l > pos) { dist[l] = dist[l - 1]; idx[l] = idx[l - 1]; }
+                }
+                const int pid = __float_as_int(p.w);
+#pragma unroll
+                for (int l = 0; l < KMAX; l++)
+                    if (l == pos) { dist[l] = d2; idx[l] = pid; }
+            }
+        }
+    }
+#pragma unroll
+    for (int l = 0; l < KMAX; l++) {
+        if (l < count) {
+            out_idx[(size_t)q * k + l] = idx[l];
+            if (out_dist) out_dist[(size_t)q * k + l] = dist[l];
+        }
+    }
+    out_count[q] = count;
+}
+
+template <typename T>
+int ensure(T*& p, int& cap_now, int want) {
+    if (p && cap_now >= want) return IMBA_OK;
+    if (p) IMBA_HIP_TRY(hipFree(p));
+    p = nullptr;
+    IMBA_HIP_TRY(hipMalloc((void**)&p, sizeof(T) * (size_t)std::max(want, 1)));
+    cap_now = want;
+    return IMBA_OK;
+}
+
+}  // namespace
+}  // namespace imba
+
+using namespace imba;
+
+extern "C" {
+
+int imba_hashgrid_create(ImbaHashGrid** out) {
+    IMBA_CHECK_ARG(out, "out is NULL");
+    ImbaHashGrid* g = new ImbaHashGrid();
+    if (hipGetDevice(&g->device) != hipSuccess) g->device = 0;
+    if (hipMalloc((void**)&g->d_params, sizeof(GridParams)) != hipSuccess ||
+        hipMalloc((void**)&g->partial, sizeof(float) * 6 * kBoundsBlocks) != hipSuccess) {
+        imba_hashgrid_release(g);
+        return fail(IMBA_ERR_OUT_OF_MEMORY, "hash grid parameters");
+    }
+    GridParams zero{};
+    if (hipMemcpy(g->d_params, &zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess) {
+        imba_hashgrid_release(g);
+        return fail(IMBA_ERR_HIP, "hash grid parameters");
+    }
+    *out = g;
+    return IMBA_OK;
+}
+
+int imba_hashgrid_release(ImbaHashGrid* g) {
+    if (!g) return IMBA_OK;
+    void* ptrs[] = {g->cell_ends, g->counts, g->order, g->bucket, g->ticket, g->pos4, g->d_params, g->partial,
+                    g->scan_tmp, g->large_list};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete g;
+    return IMBA_OK;
+}
+
+int imba_hashgrid_build(ImbaHashGrid* g, const void* photons, int stride, int pos_offset, int count, float radius,
+                        void* stream) {
+    IMBA_CHECK_ARG(g, "grid is NULL");
+    IMBA_CHECK_ARG(count >= 0 && count <= (1 << 29), "count out of range");
+    IMBA_CHECK_ARG(count == 0 || photons, "photons is NULL");
+    IMBA_CHECK_ARG(stride >= 12 && stride % 4 == 0 && pos_offset >= 0 && pos_offset % 4 == 0 && pos_offset + 12 <= stride,
+                   "stride/pos_offset must be multiples of 4 with the position inside the record");
+    IMBA_CHECK_ARG(radius > 0.0f, "radius must be positive");
+    bool sync;
+    hipStream_t s = resolve_stream(stream, &sync);
+    // Bucket table: max(previous size, 2 * count) (rangesearch.h:33-34).
+    const int table = std::max(g->table_size, 2 * count);
+    int tcap = g->table_size;
+    if (table > g->table_size) {
+        int c1 = 0, c2 = 0;
+        int r;
+        if ((r = ensure(g->cell_ends, c1, table))) return r;
+        if ((r = ensure(g->counts, c2, table))) return r;
+        g->table_size = table;
+        tcap = table;
+    }
+    (void)tcap;
+    if (count > g->cap) {
+        int c = 0, r;
+        if ((r = ensure(g->order, c, count))) return r;
+        c = 0; if ((r = ensure(g->bucket, c, count))) return r;
+        c = 0; if ((r = ensure(g->ticket, c, count))) return r;
+        c = 0; if ((r = ensure(g->pos4, c, count))) return r;
+        g->cap = count;
+    }
+    const int ntiles = ceil_div(std::max(table, 1), kScanTile);
+    if (ntiles > g->scan_cap) {
+        int r = ensure(g->scan_tmp, g->scan_cap, ntiles);
+        if (r) return r;
+    }
+    if (table + 1 > g->large_cap) {
+        int r = ensure(g->large_list, g->large_cap, table + 1);
+        if (r) return r;
+    }
+    g->count = count;
+    const unsigned char* ph = (const unsigned char*)photons;
+    const int nparts = count > 0 ? std::min(kBoundsBlocks, ceil_div(count, kBlock)) : 0;
+    if (count > 0) hipLaunchKernelGGL(k_bounds, dim3(nparts), dim3(kBlock), 0, s, ph, stride, pos_offset, count, g->partial);
+    hipLaunchKernelGGL(k_params, dim3(1), dim3(1), 0, s, g->partial, nparts, g->d_params, radius, table, count);
+    if (table > 0) {
+        hipLaunchKernelGGL(k_zero, dim3(std::min(ceil_div(table, kBlock), 4096)), dim3(kBlock), 0, s, g->counts, table);
+    }
+    if (count > 0) {
+        hipLaunchKernelGGL(k_count_buckets, dim3(ceil_div(count, kBlock)), dim3(kBlock), 0, s, ph, stride, pos_offset,
+                           count, g->d_params, g->bucket, g->ticket, g->counts);
+    }
+    if (table > 0) {
+        hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, s, g->counts, table, g->scan_tmp);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, g->scan_tmp, ntiles);
+        hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(kBlock), 0, s, g->counts, table, g->scan_tmp, g->cell_ends);
+    }
+    if (count > 0) {
+        hipLaunchKernelGGL(k_scatter, dim3(ceil_div(count, kBlock)), dim3(kBlock), 0, s, count, g->bucket, g->ticket,
+                           g->counts, g->cell_ends, g->order);
+        int* large_count = g->large_list + table;
+        hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlock), 0, s, large_count, 1);
+        hipLaunchKernelGGL(k_order_small, dim3(ceil_div(table, kBlock)), dim3(kBlock), 0, s, table, g->counts,
+                           g->cell_ends, g->order, g->large_list, large_count);
+        // the rank path writes through `bucket` as a slot-ordered temporary (no longer needed)
+        hipLaunchKernelGGL(k_order_large, dim3(256), dim3(1024), 0, s, g->large_list, large_count, g->counts,
+                           g->cell_ends, g->order, g->bucket);
+        hipLaunchKernelGGL(k_gather_pos, dim3(ceil_div(count, kBlock)), dim3(kBlock), 0, s, ph, stride, pos_offset,
+                           count, g->order, g->pos4);
+    }
+    IMBA_HIP_TRY(hipGetLastError());
+    if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
+    return IMBA_OK;
+}
+
+int imba_hashgrid_query(const ImbaHashGrid* g, const void* queries, int query_stride, int query_offset, int nq, int k,
+                        int* out_idx, float* out_dist, int* out_count, void* stream) {
+    IMBA_CHECK_ARG(g, "grid is NULL");
+    IMBA_CHECK_ARG(k >= 1 && k <= 64, "k must be in [1, 64]");
+    IMBA_CHECK_ARG(nq >= 0, "nq < 0");
+    IMBA_CHECK_ARG(query_stride >= 12 && query_stride % 4 == 0 && query_offset >= 0 && query_offset % 4 == 0 &&
+                       query_offset + 12 <= query_stride,
+                   "query stride/offset must be multiples of 4 with the position inside the record");
+    if (nq == 0) return IMBA_OK;
+    IMBA_CHECK_ARG(queries && out_idx && out_count, "NULL buffer");
+    if (g->table_size == 0) return fail(IMBA_ERR_INVALID_ARG, "grid has not been built");
+    bool sync;
+    hipStream_t s = resolve_stream(stream, &sync);
+    const unsigned char* qs = (const unsigned char*)queries;
+    const dim3 grid(ceil_div(nq, kBlock));
+    if (k <= 16)
+        hipLaunchKernelGGL(k_query<16>, grid, dim3(kBlock), 0, s, g->d_params, g->cell_ends, g->pos4, qs, query_stride,
+                           query_offset, nq, k, out_idx, out_dist, out_count);
+    else if (k <= 32)
+        hipLaunchKernelGGL(k_query<32>, grid, dim3(kBlock), 0, s, g->d_params, g->cell_ends, g->pos4, qs, query_stride,
+                           query_offset, nq, k, out_idx, out_dist, out_count);
+    else
+        hipLaunchKernelGGL(k_query<64>, grid, dim3(kBlock), 0, s, g->d_params, g->cell_ends, g->pos4, qs, query_stride,
+                           query_offset, nq, k, out_idx, out_dist, out_count);
+    IMBA_HIP_TRY(hipGetLastError());
+    if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
+    return IMBA_OK;
+}
+
+int imba_hashgrid_info(const ImbaHashGrid* g, ImbaHashGridInfo* info) {
+    IMBA_CHECK_ARG(g && info, "NULL argument");
+    GridParams p{};
+    IMBA_HIP_TRY(hipMemcpy(&p, g->d_params, sizeof(p), hipMemcpyDeviceToHost));
+    info->photon_count = g->count;
+    info->table_size = g->table_size;
+    for (int k = 0; k < 3; k++) {
+        info->bbox_min[k] = p.bmin[k];
+        info->bbox_max[k] = p.bmax[k];
+    }
+    info->radius = p.radius;
+    info->radius_sqr = p.radius_sqr;
+    info->cell_size = p.cell_size;
+    info->inv_cell_size = p.inv_cell_size;
+    info->cell_ends = g->cell_ends;
+    info->order = g->order;
+    info->positions = (const float*)g->pos4;
+    return IMBA_OK;
+}
+
+__global__ void k_gather_records(const unsigned char* src, int stride, int n, const int* order, unsigned char* dst) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= n) return;
+    const int i = order[s];
+    const int* a = reinterpret_cast<const int*>(src + (size_t)i * stride);
+    int* b = reinterpret_cast<int*>(dst + (size_t)s * stride);
+    for (int k = 0; k < stride / 4; k++) b[k] = a[k];
+}
+
+int imba_hashgrid_gather(const ImbaHashGrid* g, const void* src, int stride, void* dst, void* stream) {
+    IMBA_CHECK_ARG(g && src && dst, "NULL argument");
+    IMBA_CHECK_ARG(stride > 0 && stride % 4 == 0, "stride must be a positive multiple of 4");
+    if (g->count == 0) return IMBA_OK;
+    bool sync;
+    hipStream_t s = resolve_stream(stream, &sync);
+    hipLaunchKernelGGL(k_gather_records, dim3(ceil_div(g->count, 256)), dim3(256), 0, s, (const unsigned char*)src,
+                       stride, g->count, g->order, (unsigned char*)dst);
+    IMBA_HIP_TRY(hipGetLastError());
+    if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
+    return IMBA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,252 @@
+// Host side of the traversal kernels: launch configuration, scratch, C ABI entry points and
+// the reference FFI (traversal_gpu.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "common.h"
+#include "scene.h"
+#include "trace_kernels.h"
+
+namespace imba {
+
+namespace {
+
+std::mutex g_scratch_mutex;
+Scratch g_spill[64];
+
+struct LaunchShape {
+    int block;
+    int stack_lds;
+};
+
+template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS>
+int launch_one(const TraceArgs& args_in, int max_stack, int blocks_per_cu, hipStream_t s) {
+    TraceArgs a = args_in;
+    const DeviceProps* p = device_props();
+    if (!p) return fail(IMBA_ERR_NO_DEVICE, "no HIP device");
+    auto kern = k_trace<BLOCK, STACK_LDS, ANY, MASKED, STATS>;
+    int resident = 0;
+    IMBA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern, BLOCK, 0));
+    if (resident <= 0) resident = 1;
+    if (blocks_per_cu > 0) resident = std::min(resident, blocks_per_cu);
+    const int needed = ceil_div(a.count, BLOCK);
+    const int grid = std::max(1, std::min(needed, resident * p->cu_count));
+    a.spill = nullptr;
+    a.spill_depth = 0;
+    std::unique_lock<std::mutex> lock(g_scratch_mutex, std::defer_lock);
+    if (max_stack > STACK_LDS) {
+        a.spill_depth = max_stack - STACK_LDS;
+        lock.lock();
+        Scratch& sc = g_spill[p->device & 63];
+        int r = grow_buffer(sc, (size_t)a.spill_depth * grid * BLOCK * sizeof(int));
+        if (r) return r;
+        a.spill = (int*)sc.ptr;
+    }
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, s, a);
+    IMBA_HIP_TRY(hipGetLastError());
+    if (lock.owns_lock()) {
+        // The spill buffer is shared by the device's launches: keep it until this one ends.
+        IMBA_HIP_TRY(hipStreamSynchronize(s));
+    }
+    return IMBA_OK;
+}
+
+template <bool ANY, bool MASKED, bool STATS>
+int dispatch_shape(const TraceArgs& a, int max_stack, const ImbaTraceConfig* cfg, hipStream_t s) {
+    const int block = cfg && cfg->block_size ? cfg->block_size : 256;
+    const int stack = cfg && cfg->stack_lds ? cfg->stack_lds : 32;
+    const int bpc = cfg ? cfg->blocks_per_cu : 0;
+#define IMBA_SHAPE(B, S) \
+    if (block == B && stack == S) return launch_one<B, S, ANY, MASKED, STATS>(a, max_stack, bpc, s);
+    IMBA_SHAPE(256, 32)
+    IMBA_SHAPE(256, 16)
+    IMBA_SHAPE(128, 32)
+    IMBA_SHAPE(128, 48)
+    IMBA_SHAPE(64, 64)
+#undef IMBA_SHAPE
+    return fail(IMBA_ERR_INVALID_ARG, "unsupported trace shape block=%d stack_lds=%d "
+                "(supported: 256/32, 256/16, 128/32, 128/48, 64/64)", block, stack);
+}
+
+int launch_trace(const ImbaSceneDesc& d, const SceneInfo& info, int* d_err, const ImbaRay* rays, ImbaHit* hits,
+                 uint32_t* counts, int count, bool any, bool masked, const ImbaTraceConfig* cfg, void* stream) {
+    if (count < 0) return fail(IMBA_ERR_INVALID_ARG, "count < 0");
+    if (count == 0) return IMBA_OK;
+    if (!rays || !hits) return fail(IMBA_ERR_INVALID_ARG, "rays or hits is NULL");
+    TraceArgs a{};
+    a.nodes = (const float4*)d.nodes;       a.n_nodes = d.node_count;
+    a.inst = (const float4*)d.instances;    a.n_inst = d.instance_count;
+    a.tris = (const float4*)d.tris;         a.n_tris4 = d.tri_vec4_count;
+    a.indices = (const int4*)d.indices;     a.n_tri_ids = d.index_count / 4;
+    a.texcoords = (const float2*)d.texcoords; a.n_texcoords = d.texcoord_count;
+    a.masks = (const int*)d.masks;          a.n_masks = d.mask_count;
+    a.mask_buf = d.mask_buffer;             a.mask_buf_size = d.mask_buffer_size;
+    a.rays = (const float4*)rays;
+    a.hits = (int4*)hits;
+    a.count = count;
+    a.root = d.root;
+    a.counts = counts;
+    a.err = d_err;
+    bool sync;
+    hipStream_t s = resolve_stream(stream, &sync);
+    int r;
+    const bool stats = counts != nullptr;
+    if (stats) {
+        r = any ? (masked ? dispatch_shape<true, true, true>(a, info.max_stack, nullptr, s)
+                          : dispatch_shape<true, false, true>(a, info.max_stack, nullptr, s))
+                : (masked ? dispatch_shape<false, true, true>(a, info.max_stack, nullptr, s)
+                          : dispatch_shape<false, false, true>(a, info.max_stack, nullptr, s));
+    } else {
+        r = any ? (masked ? dispatch_shape<true, true, false>(a, info.max_stack, cfg, s)
+                          : dispatch_shape<true, false, false>(a, info.max_stack, cfg, s))
+                : (masked ? dispatch_shape<false, true, false>(a, info.max_stack, cfg, s)
+                          : dispatch_shape<false, false, false>(a, info.max_stack, cfg, s));
+    }
+    if (r) return r;
+    if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
+    return IMBA_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// Reference FFI: the caller passes only device pointers and the ray count.  Array sizes are
+// taken from the allocations holding them, and the structure is validated once per set of
+// arrays (the host copy made for that is dropped afterwards).
+struct FfiKey {
+    int root;
+    const void *nodes, *instances, *tris, *indices, *texcoords, *masks, *mask_buffer;
+    bool operator<(const FfiKey& o) const {
+        return std::tie(root, nodes, instances, tris, indices, texcoords, masks, mask_buffer) <
+               std::tie(o.root, o.nodes, o.instances, o.tris, o.indices, o.texcoords, o.masks, o.mask_buffer);
+    }
+};
+struct FfiScene {
+    ImbaSceneDesc dev;
+    SceneInfo info;
+    int* d_err = nullptr;
+};
+std::mutex g_ffi_mutex;
+std::map<FfiKey, FfiScene> g_ffi_scenes;
+
+size_t bytes_after(const void* p) {
+    if (!p) return 0;
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess || !base) return 0;
+    return (size_t)((const char*)base + size - (const char*)p);
+}
+
+template <typename T>
+std::vector<T> fetch(const void* p, size_t bytes) {
+    std::vector<T> v(bytes / sizeof(T));
+    if (!v.empty() && hipMemcpy(v.data(), p, v.size() * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) v.clear();
+    return v;
+}
+
+int ffi_scene(int root, const void* nodes, const void* instances, const void* tris, const void* indices,
+              const void* texcoords, const void* masks, const void* mask_buffer, FfiScene** out) {
+    FfiKey key{root, nodes, instances, tris, indices, texcoords, masks, mask_buffer};
+    std::lock_guard<std::mutex> lock(g_ffi_mutex);
+    auto it = g_ffi_scenes.find(key);
+    if (it != g_ffi_scenes.end()) { *out = &it->second; return IMBA_OK; }
+
+    ImbaSceneDesc d{};
+    d.root = root;
+    d.node_count = (int)std::min<size_t>(bytes_after(nodes) / sizeof(ImbaNode), INT32_MAX);
+    d.instance_count = (int)std::min<size_t>(bytes_after(instances) / sizeof(ImbaInstanceNode), INT32_MAX);
+    d.tri_vec4_count = (int)std::min<size_t>(bytes_after(tris) / sizeof(ImbaVec4), INT32_MAX);
+    d.index_count = (int)std::min<size_t>(bytes_after(indices) / sizeof(int) / 4 * 4, INT32_MAX);
+    d.texcoord_count = (int)std::min<size_t>(bytes_after(texcoords) / sizeof(ImbaVec2), INT32_MAX);
+    d.mask_count = (int)std::min<size_t>(bytes_after(masks) / sizeof(ImbaTransparencyMask), INT32_MAX);
+    d.mask_buffer_size = (int)std::min<size_t>(bytes_after(mask_buffer), INT32_MAX);
+    if (d.node_count == 0 || d.instance_count == 0 || d.tri_vec4_count == 0)
+        return fail(IMBA_ERR_INVALID_ARG, "scene arrays are not device allocations");
+
+    auto h_nodes = fetch<ImbaNode>(nodes, (size_t)d.node_count * sizeof(ImbaNode));
+    auto h_inst = fetch<ImbaInstanceNode>(instances, (size_t)d.instance_count * sizeof(ImbaInstanceNode));
+    auto h_tris = fetch<ImbaVec4>(tris, (size_t)d.tri_vec4_count * sizeof(ImbaVec4));
+    auto h_masks = fetch<ImbaTransparencyMask>(masks, (size_t)d.mask_count * sizeof(ImbaTransparencyMask));
+    auto h_mbuf = fetch<uint8_t>(mask_buffer, (size_t)d.mask_buffer_size);
+    ImbaSceneDesc h = d;
+    h.nodes = h_nodes.data();
+    h.instances = h_inst.data();
+    h.tris = h_tris.data();
+    h.masks = h_masks.data();
+    h.mask_buffer = h_mbuf.data();
+    FfiScene fs;
+    int r = validate_scene(h, fs.info);
+    if (r) return r;
+    fs.dev = d;
+    fs.dev.nodes = (const ImbaNode*)nodes;
+    fs.dev.instances = (const ImbaInstanceNode*)instances;
+    fs.dev.tris = (const ImbaVec4*)tris;
+    fs.dev.indices = (const int*)indices;
+    fs.dev.texcoords = (const ImbaVec2*)texcoords;
+    fs.dev.masks = (const ImbaTransparencyMask*)masks;
+    fs.dev.mask_buffer = (const uint8_t*)mask_buffer;
+    IMBA_HIP_TRY(hipMalloc((void**)&fs.d_err, sizeof(int)));
+    IMBA_HIP_TRY(hipMemset(fs.d_err, 0, sizeof(int)));
+    auto ins = g_ffi_scenes.emplace(key, fs);
+    *out = &ins.first->second;
+    return IMBA_OK;
+}
+
+void ffi_trace(bool any, int root, void* nodes, void* instances, void* tris, void* rays, void* hits, int* indices,
+               void* texcoords, void* masks, char* mask_buffer, int count) {
+    FfiScene* fs = nullptr;
+    int r = ffi_scene(root, nodes, instances, tris, indices, texcoords, masks, mask_buffer, &fs);
+    if (!r) r = launch_trace(fs->dev, fs->info, fs->d_err, (const ImbaRay*)rays, (ImbaHit*)hits, nullptr, count, any,
+                             !fs->info.all_opaque, nullptr, nullptr);
+    if (r) fprintf(stderr, "imbatracer_amd: %s_gpu_masked_instanced failed: %s\n", any ? "occluded" : "intersect",
+                   imba_last_error());
+}
+
+}  // namespace
+}  // namespace imba
+
+using namespace imba;
+
+extern "C" {
+
+int imba_intersect(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int count, const ImbaTraceConfig* cfg,
+                   void* stream) {
+    IMBA_CHECK_ARG(s, "scene is NULL");
+    return launch_trace(s->dev, s->info, s->d_error, rays, hits, nullptr, count, false, !s->info.all_opaque, cfg, stream);
+}
+
+int imba_occluded(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int count, const ImbaTraceConfig* cfg,
+                  void* stream) {
+    IMBA_CHECK_ARG(s, "scene is NULL");
+    return launch_trace(s->dev, s->info, s->d_error, rays, hits, nullptr, count, true, !s->info.all_opaque, cfg, stream);
+}
+
+int imba_intersect_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts, int count,
+                         void* stream) {
+    IMBA_CHECK_ARG(s && counts, "NULL argument");
+    return launch_trace(s->dev, s->info, s->d_error, rays, hits, counts, count, false, !s->info.all_opaque, nullptr,
+                        stream);
+}
+
+int imba_occluded_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts, int count,
+                        void* stream) {
+    IMBA_CHECK_ARG(s && counts, "NULL argument");
+    return launch_trace(s->dev, s->info, s->d_error, rays, hits, counts, count, true, !s->info.all_opaque, nullptr,
+                        stream);
+}
+
+// Reference FFI (render/scheduling/ray_queue.h:297-308, 347-358): synchronous, device pointers.
+void intersect_gpu_masked_instanced(int root, void* nodes, void* instances, void* tris, void* rays, void* hits,
+                                    int* indices, void* texcoords, void* masks, char* mask_buffer, int count) {
+    ffi_trace(false, root, nodes, instances, tris, rays, hits, indices, texcoords, masks, mask_buffer, count);
+}
+
+void occluded_gpu_masked_instanced(int root, void* nodes, void* instances, void* tris, void* rays, void* hits,
+                                   int* indices, void* texcoords, void* masks, char* mask_buffer, int count) {
+    ffi_trace(true, root, nodes, instances, tris, rays, hits, indices, texcoords, masks, mask_buffer, count);
+}
+
+}  // extern "C"

@@ -1,0 +1,269 @@
+"""Python side of the library: scene assembly, resident scenes, traversal, compaction and the
+photon hash grid, all through the C ABI (imbatracer_amd.native).
+
+Arrays handed to the device functions are DeviceBuffer objects (device memory); host arrays
+are numpy arrays.  Every call raises ImbaError on failure.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import native as N
+from .native import (BUILDER_BINNED_SAH, BUILDER_SBVH, HIT_DTYPE, INSTANCE_DTYPE, MASK_DTYPE, NODE_DTYPE, RAY_DTYPE,
+                     DeviceBuffer, check)
+
+IDENTITY = np.eye(4, dtype=np.float32)
+
+
+# ---------------------------------------------------------------------------------------------
+# Synthetic inputs
+def random_triangles(n: int, seed: int = 0x1234):
+    """Random triangles in the unit cube (vertices (3n,4) float32, indices (n,4) uint32)."""
+    v = np.empty((3 * n, 4), np.float32)
+    i = np.empty((n, 4), np.uint32)
+    check(N.lib.imba_gen_random_triangles(seed, n, N.ptr(v), N.ptr(i)), "imba_gen_random_triangles")
+    return v, i
+
+
+def icosphere(level: int, radius: float, centre, material: int = 0):
+    nv, nt = C.c_int(), C.c_int()
+    check(N.lib.imba_gen_icosphere_counts(level, C.byref(nv), C.byref(nt)), "icosphere counts")
+    v = np.empty((nv.value, 4), np.float32)
+    i = np.empty((nt.value, 4), np.uint32)
+    check(N.lib.imba_gen_icosphere(level, radius, centre[0], centre[1], centre[2], material, N.ptr(v), N.ptr(i)),
+          "imba_gen_icosphere")
+    return v, i
+
+
+def camera_rays(pos, direction, up, fov_deg: float, width: int, height: int) -> np.ndarray:
+    rays = np.empty(width * height, RAY_DTYPE)
+    p = np.asarray(pos, np.float32)
+    d = np.asarray(direction, np.float32)
+    u = np.asarray(up, np.float32)
+    check(N.lib.imba_gen_camera_rays(N.ptr(p), N.ptr(d), N.ptr(u), fov_deg, width, height, N.ptr(rays)),
+          "imba_gen_camera_rays")
+    return rays
+
+
+def make_rays(org: np.ndarray, dirs: np.ndarray, tmin, tmax) -> np.ndarray:
+    n = len(org)
+    r = np.empty(n, RAY_DTYPE)
+    r["org"][:, :3] = org
+    r["org"][:, 3] = tmin
+    r["dir"][:, :3] = dirs
+    r["dir"][:, 3] = tmax
+    return r
+
+
+# ---------------------------------------------------------------------------------------------
+class SceneBuild:
+    """Meshes + instances -> BVH2 traversal arrays (host)."""
+
+    def __init__(self):
+        self.h = C.c_void_p()
+        check(N.lib.imba_build_create(C.byref(self.h)), "imba_build_create")
+        self._keep = []
+
+    def add_mesh(self, vertices: np.ndarray, indices: np.ndarray, texcoords: np.ndarray | None = None) -> int:
+        v = np.ascontiguousarray(vertices, np.float32)
+        if v.shape[1] == 3:
+            v = np.ascontiguousarray(np.concatenate([v, np.ones((len(v), 1), np.float32)], 1))
+        i = np.ascontiguousarray(indices, np.uint32)
+        if i.ndim == 2 and i.shape[1] == 3:
+            i = np.ascontiguousarray(np.concatenate([i, np.zeros((len(i), 1), np.uint32)], 1))
+        tc = None if texcoords is None else np.ascontiguousarray(texcoords, np.float32)
+        d = N.MeshDesc(v.ctypes.data, len(v), i.ctypes.data, len(i), None if tc is None else tc.ctypes.data)
+        mid = C.c_int()
+        check(N.lib.imba_build_add_mesh(self.h, C.byref(d), C.byref(mid)), "imba_build_add_mesh")
+        return mid.value
+
+    def add_instance(self, mesh_id: int, mat4=IDENTITY) -> int:
+        m = np.ascontiguousarray(mat4, np.float32).reshape(16)
+        iid = C.c_int()
+        check(N.lib.imba_build_add_instance(self.h, mesh_id, N.ptr(m), C.byref(iid)), "imba_build_add_instance")
+        return iid.value
+
+    def set_mask(self, material: int, texels: np.ndarray):
+        t = np.ascontiguousarray(texels, np.uint8)
+        check(N.lib.imba_build_set_mask(self.h, material, t.shape[1], t.shape[0], N.ptr(t)), "imba_build_set_mask")
+
+    def run(self, builder: int = BUILDER_SBVH, threads: int = 1):
+        check(N.lib.imba_build_run(self.h, builder, threads), "imba_build_run")
+        return self
+
+    def desc(self) -> N.SceneDesc:
+        d = N.SceneDesc()
+        check(N.lib.imba_build_get_desc(self.h, C.byref(d)), "imba_build_get_desc")
+        return d
+
+    def arrays(self) -> dict:
+        """Copies of the host arrays."""
+        d = self.desc()
+
+        def arr(p, n, dt):
+            if n == 0 or not p:
+                return np.zeros(0, dt)
+            buf = (C.c_char * (n * np.dtype(dt).itemsize)).from_address(p)
+            return np.frombuffer(buf, dtype=dt).copy()
+
+        return {
+            "root": d.root,
+            "nodes": arr(d.nodes, d.node_count, NODE_DTYPE),
+            "instances": arr(d.instances, d.instance_count, INSTANCE_DTYPE),
+            "tris": arr(d.tris, d.tri_vec4_count, np.dtype(("<f4", 4))).reshape(-1, 4),
+            "indices": arr(d.indices, d.index_count, np.int32),
+            "texcoords": arr(d.texcoords, d.texcoord_count, np.dtype(("<f4", 2))).reshape(-1, 2),
+            "masks": arr(d.masks, d.mask_count, MASK_DTYPE),
+            "mask_buffer": arr(d.mask_buffer, d.mask_buffer_size, np.uint8),
+        }
+
+    def tri_attrs(self):
+        n = C.c_void_p()
+        m = C.c_void_p()
+        cnt = C.c_int()
+        check(N.lib.imba_build_get_tri_attrs(self.h, C.byref(n), C.byref(m), C.byref(cnt)), "tri attrs")
+        normals = np.frombuffer((C.c_float * (3 * cnt.value)).from_address(n.value), np.float32).reshape(-1, 3).copy()
+        mats = np.frombuffer((C.c_int * cnt.value).from_address(m.value), np.int32).copy()
+        return normals, mats
+
+    def __del__(self):
+        try:
+            N.lib.imba_build_release(self.h)
+        except Exception:
+            pass
+
+
+def desc_from_arrays(a: dict) -> tuple[N.SceneDesc, list]:
+    """SceneDesc over host numpy arrays (the list keeps them alive)."""
+    keep = []
+
+    def p(x):
+        x = np.ascontiguousarray(x)
+        keep.append(x)
+        return x.ctypes.data if x.size else None
+
+    d = N.SceneDesc(a["root"], p(a["nodes"]), len(a["nodes"]), p(a["instances"]), len(a["instances"]),
+                    p(a["tris"]), len(a["tris"]), p(a["indices"]), len(a["indices"]),
+                    p(a["texcoords"]), len(a["texcoords"]), p(a["masks"]), len(a["masks"]),
+                    p(a["mask_buffer"]), len(a["mask_buffer"]))
+    return d, keep
+
+
+def emit_mesh(vertices, indices, builder: int, width: int, cost: int) -> np.ndarray:
+    v = np.ascontiguousarray(vertices, np.float32)
+    i = np.ascontiguousarray(indices, np.uint32)
+    d = N.MeshDesc(v.ctypes.data, len(v), i.ctypes.data, len(i), None)
+    need = C.c_size_t()
+    N.lib.imba_bvh_emit_mesh(C.byref(d), builder, width, cost, None, 0, C.byref(need))
+    out = np.empty(need.value, np.int32)
+    check(N.lib.imba_bvh_emit_mesh(C.byref(d), builder, width, cost, N.ptr(out), out.size, C.byref(need)), "emit")
+    return out
+
+
+def emit_boxes(boxes, centres, width: int, cost: int) -> np.ndarray:
+    b = np.ascontiguousarray(boxes, np.float32)
+    c = np.ascontiguousarray(centres, np.float32)
+    need = C.c_size_t()
+    N.lib.imba_bvh_emit_boxes(N.ptr(b), N.ptr(c), len(b), width, cost, None, 0, C.byref(need))
+    out = np.empty(need.value, np.int32)
+    check(N.lib.imba_bvh_emit_boxes(N.ptr(b), N.ptr(c), len(b), width, cost, N.ptr(out), out.size, C.byref(need)),
+          "emit boxes")
+    return out
+
+
+# ---------------------------------------------------------------------------------------------
+class Scene:
+    """A device-resident, validated scene."""
+
+    def __init__(self, desc: N.SceneDesc):
+        self.h = C.c_void_p()
+        check(N.lib.imba_scene_upload(C.byref(desc), C.byref(self.h)), "imba_scene_upload")
+
+    def device_desc(self) -> N.SceneDesc:
+        d = N.SceneDesc()
+        check(N.lib.imba_scene_device_desc(self.h, C.byref(d)), "device desc")
+        return d
+
+    def footprint(self) -> int:
+        s = C.c_size_t()
+        check(N.lib.imba_scene_footprint(self.h, C.byref(s)), "footprint")
+        return s.value
+
+    def error_count(self) -> int:
+        c = C.c_int()
+        check(N.lib.imba_trace_error_count(self.h, C.byref(c)), "error count")
+        return c.value
+
+    def intersect(self, rays: DeviceBuffer, hits: DeviceBuffer, count: int, cfg: N.TraceConfig | None = None,
+                  stream=None):
+        check(N.lib.imba_intersect(self.h, rays.ptr, hits.ptr, count, C.byref(cfg) if cfg else None, N._s(stream)),
+              "imba_intersect")
+
+    def occluded(self, rays: DeviceBuffer, hits: DeviceBuffer, count: int, cfg: N.TraceConfig | None = None,
+                 stream=None):
+        check(N.lib.imba_occluded(self.h, rays.ptr, hits.ptr, count, C.byref(cfg) if cfg else None, N._s(stream)),
+              "imba_occluded")
+
+    def intersect_stats(self, rays: DeviceBuffer, hits: DeviceBuffer, counts: DeviceBuffer, count: int, any_hit=False,
+                        stream=None):
+        f = N.lib.imba_occluded_stats if any_hit else N.lib.imba_intersect_stats
+        check(f(self.h, rays.ptr, hits.ptr, counts.ptr, count, N._s(stream)), "stats")
+
+    def trace_host(self, rays: np.ndarray, any_hit: bool = False, cfg: N.TraceConfig | None = None) -> np.ndarray:
+        """Convenience: upload rays, trace, download hits."""
+        rb = DeviceBuffer.from_array(rays)
+        hb = DeviceBuffer(len(rays) * HIT_DTYPE.itemsize)
+        (self.occluded if any_hit else self.intersect)(rb, hb, len(rays), cfg)
+        return hb.download(HIT_DTYPE, len(rays))
+
+    def __del__(self):
+        try:
+            N.lib.imba_scene_release(self.h)
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------------------------
+def compact_hits(rays: DeviceBuffer, hits: DeviceBuffer, states: DeviceBuffer, state_stride: int, count: int,
+                 rays_out: DeviceBuffer, hits_out: DeviceBuffer, states_out: DeviceBuffer, stream=None) -> int:
+    n = C.c_int()
+    check(N.lib.imba_compact_hits(rays.ptr, hits.ptr, states.ptr, state_stride, count, rays_out.ptr, hits_out.ptr,
+                                  states_out.ptr, C.byref(n), N._s(stream)), "imba_compact_hits")
+    return n.value
+
+
+def compact_rays(rays: DeviceBuffer, states: DeviceBuffer, state_stride: int, count: int, rays_out: DeviceBuffer,
+                 states_out: DeviceBuffer, stream=None) -> int:
+    n = C.c_int()
+    check(N.lib.imba_compact_rays(rays.ptr, states.ptr, state_stride, count, rays_out.ptr, states_out.ptr, C.byref(n),
+                                  N._s(stream)), "imba_compact_rays")
+    return n.value
+
+
+class HashGrid:
+    def __init__(self):
+        self.h = C.c_void_p()
+        check(N.lib.imba_hashgrid_create(C.byref(self.h)), "imba_hashgrid_create")
+
+    def build(self, photons: DeviceBuffer, stride: int, pos_offset: int, count: int, radius: float, stream=None):
+        check(N.lib.imba_hashgrid_build(self.h, photons.ptr, stride, pos_offset, count, radius, N._s(stream)),
+              "imba_hashgrid_build")
+
+    def query(self, queries: DeviceBuffer, stride: int, offset: int, nq: int, k: int, out_idx: DeviceBuffer,
+              out_count: DeviceBuffer, out_dist: DeviceBuffer | None = None, stream=None):
+        check(N.lib.imba_hashgrid_query(self.h, queries.ptr, stride, offset, nq, k, out_idx.ptr,
+                                        out_dist.ptr if out_dist is not None else None, out_count.ptr, N._s(stream)),
+              "imba_hashgrid_query")
+
+    def info(self) -> N.HashGridInfo:
+        i = N.HashGridInfo()
+        check(N.lib.imba_hashgrid_info(self.h, C.byref(i)), "imba_hashgrid_info")
+        return i
+
+    def __del__(self):
+        try:
+            N.lib.imba_hashgrid_release(self.h)
+        except Exception:
+            pass

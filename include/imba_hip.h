@@ -1,0 +1,273 @@
+/*
+ * imba_hip.h — C ABI of the MI355X (gfx950) traversal / compaction / hash-grid library.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes and returns an int status
+ * (IMBA_OK == 0).  On failure imba_last_error() returns a message for the calling thread.
+ * Pointers documented as "device" must be device memory of the current HIP device
+ * (imba_malloc or any hipMalloc'd memory); "host" pointers are ordinary host memory.
+ * A `stream` argument is a hipStream_t passed as void*; NULL means the library's stream for
+ * the calling thread, and such calls return only after the work has finished.
+ *
+ * The reference's own FFI for traversal (the AnyDSL-generated functions
+ * {intersect,occluded}_gpu_masked_instanced) is declared with the reference's type names in
+ * traversal_gpu.h, which is implemented by the same library.
+ */
+#ifndef IMBA_HIP_H
+#define IMBA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------------------------
+ * Traversal ABI types.  Field names follow the reference's traversal ABI (re-exported at
+ * src/imbatracer/core/traversal_interface.h:12-17, fields used at
+ * render/scheduling/ray_queue.h:42-54, core/gpu_adapter.cpp:14-31,62-141,276-295,
+ * render/integrators/integrator.h:72-90, core/mask.h:9-20).  Byte layout is this library's
+ * own and is fixed here (sizes are checked with static_asserts in the library).
+ * ---------------------------------------------------------------------------------------- */
+typedef struct ImbaVec4 { float x, y, z, w; } ImbaVec4;                 /* 16 B */
+typedef struct ImbaVec2 { float x, y; } ImbaVec2;                       /*  8 B */
+
+/* org.w = tmin, dir.w = tmax (camera.h:61-64, pt.cpp:38-41). 32 B. */
+typedef struct ImbaRay { ImbaVec4 org; ImbaVec4 dir; } ImbaRay;
+
+/* tri_id < 0: no hit / unoccluded.  tri_id is the global triangle id written in the
+ * triangle records, inst_id the instance index, tmax the hit distance in the ray's own
+ * parameterisation, u the barycentric weight of the triangle's second vertex. 16 B. */
+typedef struct ImbaHit { int tri_id; int inst_id; float tmax; float u; } ImbaHit;
+
+typedef struct ImbaBBox { float lo_x, hi_x, lo_y, hi_y, lo_z, hi_z; } ImbaBBox;   /* 24 B */
+
+/* BVH2 node.  left/right >= 0: node index; < 0: ~(index of the first record of a leaf)
+ * (first Vec4 of the leaf's triangles in a mesh BVH, first InstanceNode in the top level).
+ * right == IMBA_DUMMY_CHILD marks the dummy parent written above a single-leaf tree. 64 B,
+ * so that one node is four 16-byte loads. */
+typedef struct ImbaNode {
+    ImbaBBox left_bb;
+    ImbaBBox right_bb;
+    int left;
+    int right;
+    int pad[2];
+} ImbaNode;
+
+/* Instance record: transf is the world->object 3x4 matrix (rows), id the instance index,
+ * next the root node of the instanced mesh BVH; pad = {-1,-1} ends a top-level leaf. 64 B. */
+typedef struct ImbaInstanceNode {
+    ImbaVec4 transf[3];
+    int id;
+    int next;
+    int pad[2];
+} ImbaInstanceNode;
+
+/* Opacity mask of one material: width x height bytes starting at mask_buffer[offset]. */
+typedef struct ImbaTransparencyMask { int width; int height; int offset; } ImbaTransparencyMask;
+
+#define IMBA_DUMMY_CHILD 0x76543210
+#define IMBA_LEAF_END_BITS 0x80000000u   /* bit pattern of v2.w on the last triangle of a leaf */
+
+/* ------------------------------------------------------------------------------------------ */
+enum {
+    IMBA_OK = 0,
+    IMBA_ERR_INVALID_ARG = 1,
+    IMBA_ERR_HIP = 2,
+    IMBA_ERR_OUT_OF_MEMORY = 3,
+    IMBA_ERR_BAD_BVH = 4,
+    IMBA_ERR_CAPACITY = 5,
+    IMBA_ERR_NO_DEVICE = 6,
+    IMBA_ERR_KERNEL = 7
+};
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* imba_last_error(void);
+/* Library version string. */
+const char* imba_version(void);
+
+/* ---------------------------- device memory, streams, events -----------------------------
+ * Replacement for the anydsl::Array / anydsl::copy runtime used by the reference
+ * (ray_queue.h:77-80,277,292; scene.cpp:14-27,121-151). */
+int imba_device_count(int* count);
+int imba_set_device(int device);
+int imba_device_info(int* cu_count, int* lds_bytes_per_block, int* l2_bytes, char* name, int name_len);
+int imba_malloc(void** ptr, size_t bytes);
+int imba_free(void* ptr);
+int imba_host_alloc(void** ptr, size_t bytes);     /* pinned host memory */
+int imba_host_free(void* ptr);
+int imba_memcpy_h2d(void* dst_device, const void* src_host, size_t bytes, void* stream);
+int imba_memcpy_d2h(void* dst_host, const void* src_device, size_t bytes, void* stream);
+int imba_memcpy_d2d(void* dst_device, const void* src_device, size_t bytes, void* stream);
+int imba_stream_create(void** stream);
+int imba_stream_destroy(void* stream);
+int imba_stream_sync(void* stream);
+int imba_event_create(void** event);
+int imba_event_destroy(void* event);
+int imba_event_record(void* event, void* stream);
+int imba_event_elapsed_ms(float* ms, void* start, void* end);
+
+/* ------------------------------ host BVH construction ------------------------------------
+ * Restatement of the reference builders and GPU layout writers:
+ *   SplitBvhBuilder (core/sbvh_builder.h:28-165), FastBvhBuilder (core/fast_bvh_builder.h:43-160),
+ *   GpuMeshAdapter / GpuTopLevelAdapter (core/gpu_adapter.cpp:33-308),
+ *   Scene::build_mesh_accels / build_top_level_accel (render/scene.cpp:37-117). */
+enum { IMBA_BUILDER_SBVH = 0, IMBA_BUILDER_BINNED_SAH = 1 };
+
+typedef struct ImbaMeshDesc {
+    const float* vertices;      /* host, 4 floats per vertex (x, y, z, w) */
+    int vertex_count;
+    const uint32_t* indices;    /* host, 4 per triangle: i0, i1, i2, material */
+    int tri_count;
+    const float* texcoords;     /* host, 2 floats per vertex, or NULL (all zero) */
+} ImbaMeshDesc;
+
+/* Host-side traversal arrays (the TraversalData of ray_queue.h:42-54 before upload). */
+typedef struct ImbaSceneDesc {
+    int root;
+    const ImbaNode* nodes;                 int node_count;
+    const ImbaInstanceNode* instances;     int instance_count;
+    const ImbaVec4* tris;                  int tri_vec4_count;
+    const int* indices;                    int index_count;      /* 4 per triangle */
+    const ImbaVec2* texcoords;             int texcoord_count;
+    const ImbaTransparencyMask* masks;     int mask_count;
+    const uint8_t* mask_buffer;            int mask_buffer_size;
+} ImbaSceneDesc;
+
+typedef struct ImbaSceneBuild ImbaSceneBuild;
+
+int imba_build_create(ImbaSceneBuild** out);
+int imba_build_release(ImbaSceneBuild* b);
+/* Copies the mesh. mesh_id is its index (order of calls). */
+int imba_build_add_mesh(ImbaSceneBuild* b, const ImbaMeshDesc* mesh, int* mesh_id);
+/* mat: row-major 4x4 object->world; the inverse is computed as core/float4x4.h:144-191. */
+int imba_build_add_instance(ImbaSceneBuild* b, int mesh_id, const float* mat16, int* inst_id);
+/* Optional opacity mask for a material (bytes != 0 are opaque). Materials without one
+ * are opaque (MaskBuffer default, core/mask.h:22-26,43-45). */
+int imba_build_set_mask(ImbaSceneBuild* b, int material, int width, int height, const uint8_t* texels);
+/* Builds every mesh BVH with `builder`, then the top level with the binned-SAH builder.
+ * threads > 1 builds meshes in parallel; the output is identical for any thread count. */
+int imba_build_run(ImbaSceneBuild* b, int builder, int threads);
+/* Host pointers stay valid until imba_build_release. */
+int imba_build_get_desc(const ImbaSceneBuild* b, ImbaSceneDesc* out);
+/* Per-global-triangle geometric normal (3 floats) and material, for shading kernels. */
+int imba_build_get_tri_attrs(const ImbaSceneBuild* b, const float** geom_normals, const int** materials, int* tri_count);
+
+/* Runs one builder and writes the sequence of nodes and leaves it emits (for checking the
+ * builders and for custom layout writers).  Records, as int32 (floats as their bits):
+ *   node: 0, count, parent box, count child boxes      leaf: 1, count, box, count primitive ids
+ * A box is min.x, min.y, min.z, max.x, max.y, max.z.  width is 2 or 4 children per node;
+ * cost selects the cost functions of the reference adapters.  *len receives the number of
+ * int32 written, or needed when cap is too small (then IMBA_ERR_CAPACITY is returned). */
+enum { IMBA_COST_GPU_MESH = 0, IMBA_COST_GPU_TOP = 1, IMBA_COST_CPU = 2 };
+int imba_bvh_emit_mesh(const ImbaMeshDesc* mesh, int builder, int width, int cost,
+                       int32_t* out, size_t cap, size_t* len);
+/* Binned SAH over given primitive boxes (6 floats) and centres (3 floats). */
+int imba_bvh_emit_boxes(const float* boxes, const float* centres, int n, int width, int cost,
+                        int32_t* out, size_t cap, size_t* len);
+
+/* ------------------------------ device-resident scenes -----------------------------------
+ * Scene::upload_* (render/scene.cpp:119-188).  The scene is validated on upload: child
+ * indices in range and increasing from parent to child, leaf records in range, instance
+ * roots in range, leaf sentinels present. */
+typedef struct ImbaScene ImbaScene;
+int imba_scene_upload(const ImbaSceneDesc* host, ImbaScene** out);
+int imba_scene_release(ImbaScene* s);
+/* Device pointers of the resident arrays (counts as uploaded). */
+int imba_scene_device_desc(const ImbaScene* s, ImbaSceneDesc* out);
+/* Bytes of device memory held by the scene. */
+int imba_scene_footprint(const ImbaScene* s, size_t* bytes);
+
+/* ------------------------------ traversal -------------------------------------------------
+ * Closest hit (K1) and any hit (K2) of `count` rays (device) against a resident scene,
+ * hits written to device memory.  Semantics of one ray:
+ *   - a triangle is hit at t when the ray meets it inside or on its edges, det != 0,
+ *     tmin <= t and t <= tmax, and its material mask is opaque at the hit's texcoords;
+ *   - closest hit: smallest t; equal t -> smallest tri_id, then smallest inst_id;
+ *   - occluded: the first hit found in traversal order (near child first, ties left);
+ *   - no hit: tri_id = inst_id = -1, tmax = ray tmax, u = 0. */
+typedef struct ImbaTraceConfig {
+    int block_size;        /* threads per block: 64..1024, multiple of 64 (0 = default)    */
+    int stack_lds;         /* per-lane stack entries kept in LDS (0 = default)             */
+    int persistent;        /* 1: persistent waves fetching rays with a global counter      */
+    int blocks_per_cu;     /* persistent grid size per CU (0 = default)                    */
+} ImbaTraceConfig;
+
+int imba_intersect(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int count,
+                   const ImbaTraceConfig* cfg, void* stream);
+int imba_occluded(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int count,
+                  const ImbaTraceConfig* cfg, void* stream);
+/* Same traversal, also writes per-ray visit counts (device, 3 uint32 per ray:
+ * inner nodes whose children were box-tested, triangles tested, instances entered). */
+int imba_intersect_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts,
+                         int count, void* stream);
+int imba_occluded_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts,
+                        int count, void* stream);
+/* Number of rays the last launches found a malformed BVH for (0 for a valid scene). */
+int imba_trace_error_count(const ImbaScene* s, int* count);
+
+/* ------------------------------ ray-queue compaction --------------------------------------
+ * RayQueue::compact_hits (ray_queue.h:161-185): moves rays, hits and states of rays with
+ * hits[i].tri_id >= 0 to the front, keeping their order; the misses follow, also in order.
+ * RayQueue::compact_rays (ray_queue.h:188-207): moves rays and states whose state starts
+ * with an int pixel_id >= 0 to the front, keeping their order; hits are not moved.
+ * Out-of-place: inputs and outputs are distinct device buffers.  *new_count (host) receives
+ * the number of hits / survivors; for compact_rays it is `count` when no ray was removed.
+ * state_stride is the size in bytes of one state record (multiple of 4). */
+int imba_compact_hits(const ImbaRay* rays_in, const ImbaHit* hits_in, const void* states_in,
+                      int state_stride, int count,
+                      ImbaRay* rays_out, ImbaHit* hits_out, void* states_out,
+                      int* new_count, void* stream);
+int imba_compact_rays(const ImbaRay* rays_in, const void* states_in, int state_stride, int count,
+                      ImbaRay* rays_out, void* states_out, int* new_count, void* stream);
+
+/* ------------------------------ photon hash grid ------------------------------------------
+ * HashGrid::build / query (rangesearch/rangesearch.h:25-148).  Photons are records of
+ * `stride` bytes whose position (3 floats) is at byte `pos_offset`.  The grid keeps the
+ * photons' indices ordered by cell, and within a cell by index.  The bucket table only
+ * grows (max(previous size, 2 * count)), as in the reference. */
+typedef struct ImbaHashGrid ImbaHashGrid;
+typedef struct ImbaHashGridInfo {
+    int photon_count;
+    int table_size;                 /* number of buckets                                   */
+    float bbox_min[3], bbox_max[3]; /* photon bounds expanded by 0.1% of the extents       */
+    float radius, radius_sqr, cell_size, inv_cell_size;
+    const int* cell_ends;           /* device, table_size entries: end of each bucket      */
+    const int* order;               /* device, photon_count entries: photon index per slot */
+    const float* positions;         /* device, 4 floats per slot: x, y, z, index bits      */
+} ImbaHashGridInfo;
+
+int imba_hashgrid_create(ImbaHashGrid** out);
+int imba_hashgrid_release(ImbaHashGrid* g);
+int imba_hashgrid_build(ImbaHashGrid* g, const void* photons, int stride, int pos_offset,
+                        int count, float radius, void* stream);
+/* For each query position (records of query_stride bytes, position at query_offset): up to
+ * k photon indices (out_idx[q*k + j]) sorted by squared distance (out_dist, may be NULL),
+ * with the reference's rules for duplicates and ties; out_count[q] = number written.
+ * 1 <= k <= 64. */
+int imba_hashgrid_query(const ImbaHashGrid* g, const void* queries, int query_stride, int query_offset,
+                        int nq, int k, int* out_idx, float* out_dist, int* out_count, void* stream);
+int imba_hashgrid_info(const ImbaHashGrid* g, ImbaHashGridInfo* info);
+/* Copies photon records into grid order (the reference's photons_ array): dst[slot] = src[order[slot]]. */
+int imba_hashgrid_gather(const ImbaHashGrid* g, const void* src, int stride, void* dst, void* stream);
+
+/* ------------------------------ synthetic workloads ---------------------------------------
+ * Seeded generators for the measured configurations (host memory outputs). */
+/* Random triangles in the unit cube: per triangle a centre ~ U[0,1)^3, then three vertices
+ * centre + U[-0.01,0.01]^3, drawn in that order from the MWC64X generator of
+ * render/random.h:15-47 seeded with `seed`.  vertices: 12*n floats, indices: 4*n. */
+int imba_gen_random_triangles(uint64_t seed, int n, float* vertices, uint32_t* indices);
+/* Icosphere: 5-level midpoint subdivision of an icosahedron projected to the sphere. */
+int imba_gen_icosphere_counts(int level, int* vertex_count, int* tri_count);
+int imba_gen_icosphere(int level, float radius, float cx, float cy, float cz, uint32_t material,
+                       float* vertices, uint32_t* indices);
+/* Pinhole camera rays (render/ray_gen/camera.h:19-79) for pixel centres, row-major,
+ * tmin = 0, tmax = FLT_MAX. rays: host, w*h entries. */
+int imba_gen_camera_rays(const float* pos3, const float* dir3, const float* up3, float fov_deg,
+                         int width, int height, ImbaRay* rays);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IMBA_HIP_H */
