@@ -1,0 +1,243 @@
+// Path-tracing ray producers for the measured workloads: camera rays and Lambertian bounces
+// written straight into device ray queues, so a frame's bounce loop never leaves the GPU.
+//
+// Camera rays follow PathTracer's sample function (render/integrators/pt.cpp:168-177) with the
+// per-sample seeding of PixelRayGen::fill_queue (render/ray_gen/ray_gen.h:51-72) and the
+// pinhole camera of render/ray_gen/camera.h:57-79.  A bounce follows PathTracer::bounce
+// (pt.cpp:46-80) for a Lambertian material: terminate past the maximum path length, Russian
+// roulette (render/random.h:183-188) from bounce `rr_start` on, cosine-weighted direction
+// (random.h:80-83) in the frame of random.h:57-72 around the geometric normal, throughput
+// times the albedo.  Rays that hit an emissive material add throughput * emission to the
+// image and end.  Terminated rays get pixel_id = -1 (integrator.h:125-128) and are removed by
+// imba_compact_rays.
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+
+namespace imba {
+namespace {
+
+struct Rng {
+    uint64_t s;
+    __device__ uint32_t next() {
+        const uint32_t c = (uint32_t)(s >> 32);
+        const uint32_t x = (uint32_t)(s & 0xFFFFFFFFull);
+        s = (uint64_t)x * 4294883355ull + c;
+        return x ^ c;
+    }
+    __device__ float uniform() { return (float)next() / 4294967296.0f; }
+};
+
+__device__ __forceinline__ Rng seeded(uint32_t seed_base, int index, int pixel) {
+    int seed = (int)seed_base;
+    for (int k = 0; k < 4; k++) seed = (int)((uint32_t)(33u * (uint32_t)seed) ^ (uint32_t)index);
+    Rng r;
+    r.s = (uint64_t)(int64_t)seed;
+    const int n = (seed % 5) + 16 + pixel % 5;
+    for (int k = 0; k < n; k++) r.next();
+    return r;
+}
+
+struct PtState {  // 32 bytes; pixel_id first so compact_rays can test it
+    int pixel_id;
+    int sample_id;
+    uint64_t rng;
+    float tp[3];
+    int bounces;
+};
+static_assert(sizeof(PtState) == 32, "PtState layout");
+
+struct CamArgs {
+    float m[16];  // raster -> world (row major), applied to (raster y, raster x, 0, 1)
+    float pos[3];
+    int width, height, spp;
+    uint32_t seed_base;
+    int first;  // first sample index of this launch
+    int count;
+};
+
+__global__ __launch_bounds__(256) void k_camera(CamArgs c, float4* rays, PtState* states) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= c.count) return;
+    const int i = c.first + j;
+    const int pixel = i / c.spp;
+    const int sample = i % c.spp;
+    const int y = pixel / c.width, x = pixel % c.width;
+    Rng rng = seeded(c.seed_base, i, pixel);
+    const float sx = (float)x + rng.uniform();
+    const float sy = (float)y + rng.uniform();
+    float t[4];
+    for (int r = 0; r < 4; r++) t[r] = c.m[4 * r] * sy + c.m[4 * r + 1] * sx + c.m[4 * r + 2] * 0.0f + c.m[4 * r + 3];
+    float dx = t[0] / t[3] - c.pos[0], dy = t[1] / t[3] - c.pos[1], dz = t[2] / t[3] - c.pos[2];
+    const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+    dx *= inv; dy *= inv; dz *= inv;
+    rays[2 * (size_t)j] = make_float4(c.pos[0], c.pos[1], c.pos[2], 0.0f);
+    rays[2 * (size_t)j + 1] = make_float4(dx, dy, dz, 3.402823466e38f);
+    PtState s;
+    s.pixel_id = pixel;
+    s.sample_id = sample;
+    s.rng = rng.s;
+    s.tp[0] = s.tp[1] = s.tp[2] = 1.0f;
+    s.bounces = 0;
+    states[j] = s;
+}
+
+struct BounceArgs {
+    const float4* rays;
+    const int4* hits;
+    PtState* states;
+    float4* rays_out;  // may alias rays
+    int count;
+    const float* tri_normals;  // 3 per global triangle
+    const int* tri_material;
+    int n_tris;
+    const float* albedo;    // 3 per material
+    const float* emission;  // 3 per material
+    int n_materials;
+    int max_path_len;
+    int rr_start;
+    float offset;
+    float* image;  // 3 per pixel or null
+};
+
+__global__ __launch_bounds__(256) void k_bounce(BounceArgs a) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.count) return;
+    PtState s = a.states[i];
+    const int4 h = a.hits[i];
+    if (h.x < 0 || h.x >= a.n_tris) {
+        s.pixel_id = -1;
+        a.states[i] = s;
+        return;
+    }
+    const int mat = a.tri_material[h.x];
+    const bool mat_ok = mat >= 0 && mat < a.n_materials;
+    if (mat_ok) {
+        const float er = a.emission[3 * mat], eg = a.emission[3 * mat + 1], eb = a.emission[3 * mat + 2];
+        if (er > 0.0f || eg > 0.0f || eb > 0.0f) {
+            if (a.image) {
+                atomicAdd(&a.image[3 * (size_t)s.pixel_id + 0], s.tp[0] * er);
+                atomicAdd(&a.image[3 * (size_t)s.pixel_id + 1], s.tp[1] * eg);
+                atomicAdd(&a.image[3 * (size_t)s.pixel_id + 2], s.tp[2] * eb);
+            }
+            s.pixel_id = -1;
+            a.states[i] = s;
+            return;
+        }
+    }
+    if (s.bounces + 1 >= a.max_path_len || !mat_ok) {
+        s.pixel_id = -1;
+        a.states[i] = s;
+        return;
+    }
+    Rng rng;
+    rng.s = s.rng;
+    if (s.bounces >= a.rr_start) {
+        const float lum = 0.2126f * s.tp[0] + 0.7152f * s.tp[1] + 0.0722f * s.tp[2];
+        const float p = fminf(1.0f, lum * 2.0f);
+        if (!(rng.uniform() < p)) {
+            s.pixel_id = -1;
+            a.states[i] = s;
+            return;
+        }
+        const float ip = 1.0f / p;
+        s.tp[0] *= ip; s.tp[1] *= ip; s.tp[2] *= ip;
+    }
+    const float4 o = a.rays[2 * (size_t)i];
+    const float4 d = a.rays[2 * (size_t)i + 1];
+    const float t = __int_as_float(h.z);
+    const float px = o.x + t * d.x, py = o.y + t * d.y, pz = o.z + t * d.z;
+    float nx = a.tri_normals[3 * (size_t)h.x], ny = a.tri_normals[3 * (size_t)h.x + 1], nz = a.tri_normals[3 * (size_t)h.x + 2];
+    if (nx * d.x + ny * d.y + nz * d.z > 0.0f) { nx = -nx; ny = -ny; nz = -nz; }
+    // orthonormal frame around n
+    const bool xbig = fabsf(nx) > fabsf(ny);
+    const float n0 = xbig ? nx : ny;
+    const float sig = xbig ? -1.0f : 1.0f;
+    const float il = 1.0f / sqrtf(n0 * n0 + nz * nz);
+    float tx, ty, tz;
+    if (xbig) { tx = nz * sig * il; ty = 0.0f; } else { ty = nz * sig * il; tx = 0.0f; }
+    tz = n0 * -1.0f * sig * il;
+    float bx = ny * tz - nz * ty, by = nz * tx - nx * tz, bz = nx * ty - ny * tx;
+    const float bl = 1.0f / sqrtf(bx * bx + by * by + bz * bz);
+    bx *= bl; by *= bl; bz *= bl;
+    const float u1 = rng.uniform(), u2 = rng.uniform();
+    const float sin_t = sqrtf(1.0f - u2), cos_t = sqrtf(u2);
+    float sp, cp;
+    sincosf(2.0f * 3.14159265359f * u1, &sp, &cp);
+    const float lx = sin_t * cp, ly = sin_t * sp, lz = cos_t;
+    const float wx = tx * lx + bx * ly + nx * lz;
+    const float wy = ty * lx + by * ly + ny * lz;
+    const float wz = tz * lx + bz * ly + nz * lz;
+    s.tp[0] *= a.albedo[3 * mat];
+    s.tp[1] *= a.albedo[3 * mat + 1];
+    s.tp[2] *= a.albedo[3 * mat + 2];
+    s.bounces++;
+    s.rng = rng.s;
+    a.states[i] = s;
+    a.rays_out[2 * (size_t)i] = make_float4(px, py, pz, a.offset);
+    a.rays_out[2 * (size_t)i + 1] = make_float4(wx, wy, wz, 3.402823466e38f);
+}
+
+}  // namespace
+}  // namespace imba
+
+using namespace imba;
+
+extern "C" {
+
+int imba_pt_camera(const float* raster_to_world16, const float* cam_pos3, int width, int height, int spp,
+                   uint32_t seed_base, int first, int count, ImbaRay* rays, void* states, void* stream) {
+    IMBA_CHECK_ARG(raster_to_world16 && cam_pos3 && rays && states, "NULL argument");
+    IMBA_CHECK_ARG(width > 0 && height > 0 && spp > 0 && first >= 0 && count >= 0, "bad sizes");
+    IMBA_CHECK_ARG((int64_t)first + count <= (int64_t)width * height * spp, "samples out of range");
+    if (count == 0) return IMBA_OK;
+    CamArgs c;
+    for (int k = 0; k < 16; k++) c.m[k] = raster_to_world16[k];
+    for (int k = 0; k < 3; k++) c.pos[k] = cam_pos3[k];
+    c.width = width;
+    c.height = height;
+    c.spp = spp;
+    c.seed_base = seed_base;
+    c.first = first;
+    c.count = count;
+    bool sync;
+    hipStream_t s = resolve_stream(stream, &sync);
+    hipLaunchKernelGGL(k_camera, dim3(ceil_div(count, 256)), dim3(256), 0, s, c, (float4*)rays, (PtState*)states);
+    IMBA_HIP_TRY(hipGetLastError());
+    if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
+    return IMBA_OK;
+}
+
+int imba_pt_bounce(const ImbaRay* rays, const ImbaHit* hits, void* states, int count, const float* tri_normals,
+                   const int* tri_materials, int tri_count, const float* albedo, const float* emission,
+                   int material_count, int max_path_len, int rr_start, float offset, float* image, ImbaRay* rays_out,
+                   void* stream) {
+    IMBA_CHECK_ARG(count >= 0, "count < 0");
+    if (count == 0) return IMBA_OK;
+    IMBA_CHECK_ARG(rays && hits && states && tri_normals && tri_materials && albedo && emission && rays_out,
+                   "NULL argument");
+    BounceArgs a;
+    a.rays = (const float4*)rays;
+    a.hits = (const int4*)hits;
+    a.states = (PtState*)states;
+    a.rays_out = (float4*)rays_out;
+    a.count = count;
+    a.tri_normals = tri_normals;
+    a.tri_material = tri_materials;
+    a.n_tris = tri_count;
+    a.albedo = albedo;
+    a.emission = emission;
+    a.n_materials = material_count;
+    a.max_path_len = max_path_len;
+    a.rr_start = rr_start;
+    a.offset = offset;
+    a.image = image;
+    bool sync;
+    hipStream_t s = resolve_stream(stream, &sync);
+    hipLaunchKernelGGL(k_bounce, dim3(ceil_div(count, 256)), dim3(256), 0, s, a);
+    IMBA_HIP_TRY(hipGetLastError());
+    if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
+    return IMBA_OK;
+}
+
+}  // extern "C"

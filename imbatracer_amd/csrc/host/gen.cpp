@@ -146,9 +146,9 @@ int imba_gen_icosphere(int level, float radius, float cx, float cy, float cz, ui
 
 // PerspectiveCamera(w, h, fov) + move(pos, dir, up) + generate_ray(x + 0.5, y + 0.5)
 // (render/ray_gen/camera.h:19-79), aspect 1, near 0.1, far 10000.
-int imba_gen_camera_rays(const float* pos3, const float* dir3, const float* up3, float fov_deg,
-                         int width, int height, ImbaRay* rays) {
-    IMBA_CHECK_ARG(pos3 && dir3 && up3 && rays && width > 0 && height > 0, "bad arguments");
+int imba_gen_camera_matrix(const float* pos3, const float* dir3, const float* up3, float fov_deg,
+                           int width, int height, float* out16) {
+    IMBA_CHECK_ARG(pos3 && dir3 && up3 && out16 && width > 0 && height > 0, "bad arguments");
     const float3 pos(pos3[0], pos3[1], pos3[2]);
     const float3 dir = normalize(float3(dir3[0], dir3[1], dir3[2]));
     const float3 right = normalize(cross(dir, float3(up3[0], up3[1], up3[2])));
@@ -166,6 +166,18 @@ int imba_gen_camera_rays(const float* pos3, const float* dir3, const float* up3,
     const Mat4 s2w = invert(w2s);
     const Mat4 r2w = mul(mul(s2w, translation(-1.0f, -1.0f, 0.0f)),
                          diag(2.0f / (float)height, 2.0f / (float)width, 0.0f, 1.0f));
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) out16[4 * r + c] = r2w.m[r][c];
+    return IMBA_OK;
+}
+
+int imba_gen_camera_rays(const float* pos3, const float* dir3, const float* up3, float fov_deg,
+                         int width, int height, ImbaRay* rays) {
+    IMBA_CHECK_ARG(pos3 && dir3 && up3 && rays && width > 0 && height > 0, "bad arguments");
+    Mat4 r2w;
+    int rc = imba_gen_camera_matrix(pos3, dir3, up3, fov_deg, width, height, &r2w.m[0][0]);
+    if (rc) return rc;
+    const float3 pos(pos3[0], pos3[1], pos3[2]);
     for (int y = 0; y < height; y++) {
         for (int x = 0; x < width; x++) {
             const float px = (float)x + 0.5f, py = (float)y + 0.5f;

@@ -144,6 +144,9 @@ _sig("imba_gen_random_triangles", i32, C.c_uint64, i32, vp, vp)
 _sig("imba_gen_icosphere_counts", i32, i32, ip, ip)
 _sig("imba_gen_icosphere", i32, i32, f32, f32, f32, f32, C.c_uint32, vp, vp)
 _sig("imba_gen_camera_rays", i32, vp, vp, vp, f32, i32, i32, vp)
+_sig("imba_gen_camera_matrix", i32, vp, vp, vp, f32, i32, i32, vp)
+_sig("imba_pt_camera", i32, vp, vp, i32, i32, i32, C.c_uint32, i32, i32, vp, vp, vp)
+_sig("imba_pt_bounce", i32, vp, vp, vp, i32, vp, vp, i32, vp, vp, i32, i32, i32, f32, vp, vp, vp)
 _sig("intersect_gpu_masked_instanced", None, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32)
 _sig("occluded_gpu_masked_instanced", None, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32)
 

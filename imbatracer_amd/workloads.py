@@ -1,0 +1,172 @@
+"""Seeded synthetic workloads of the measured configurations.
+
+random_triangle_workload: 65,536 random triangles in the unit cube, binned-SAH BVH, 1024x1024
+    pinhole primary rays from (0.5, 0.5, -1.5) looking +z (60 deg), and one occlusion ray per
+    pixel towards the point light (0.5, 2, 0.5).
+sphere_grid_scene: an open box of 12 triangles, a 32 x 32 grid of level-5 icospheres
+    (20,480 triangles each, radius 0.4, spacing 1.0; 20,971,520 triangles in all), one
+    triangular area light; every sphere is its own mesh with its own Lambertian albedo drawn
+    uniformly from [0.2, 0.8]^3; SBVH per mesh; 1,026 identity instances.
+PathTracer: a device-side frame loop over that scene: camera rays, closest-hit traversal,
+    Lambertian bounce, queue compaction, up to max_path_len - 1 bounces, Russian roulette
+    from bounce `rr_start` on.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import native as N
+from . import scene as S
+
+STATE_BYTES = 32
+
+
+def mwc_uniform(seed: int, n: int) -> np.ndarray:
+    """n draws of the MWC64X generator's random_float() (render/random.h:19-46)."""
+    out = np.empty(n, np.float32)
+    s = seed & 0xFFFFFFFFFFFFFFFF
+    for k in range(n):
+        c = s >> 32
+        x = s & 0xFFFFFFFF
+        s = (x * 4294883355 + c) & 0xFFFFFFFFFFFFFFFF
+        out[k] = np.float32(np.float32(x ^ c) / np.float32(4294967296.0))
+    return out
+
+
+def open_box(lo, hi):
+    """12 triangles: floor (as 2 quads), back, left, right walls and ceiling; the front
+    (camera) side is open."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    xm = 0.5 * (x0 + x1)
+    quads = [
+        [(x0, y0, z0), (xm, y0, z0), (xm, y0, z1), (x0, y0, z1)],  # floor, left half
+        [(xm, y0, z0), (x1, y0, z0), (x1, y0, z1), (xm, y0, z1)],  # floor, right half
+        [(x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1)],  # back
+        [(x0, y0, z0), (x0, y0, z1), (x0, y1, z1), (x0, y1, z0)],  # left
+        [(x1, y0, z0), (x1, y1, z0), (x1, y1, z1), (x1, y0, z1)],  # right
+        [(x0, y1, z0), (x0, y1, z1), (x1, y1, z1), (x1, y1, z0)],  # ceiling
+    ]
+    v, idx = [], []
+    for q in quads:
+        b = len(v)
+        v.extend(q)
+        idx.extend([(b, b + 1, b + 2), (b, b + 2, b + 3)])
+    v = np.array([(*p, 1.0) for p in v], np.float32)
+    i = np.array([(*t, 0) for t in idx], np.uint32)
+    return v, i
+
+
+class SphereGridScene:
+    def __init__(self, side: int = 32, level: int = 5, radius: float = 0.4, spacing: float = 1.0,
+                 builder: int = N.BUILDER_SBVH, threads: int = 8, seed: int = 0x5EED):
+        self.side = side
+        ext = side * spacing
+        self.bounds = ((-1.0, 0.0, -12.0), (ext + 1.0, 8.0, ext + 1.0))
+        self.build = S.SceneBuild()
+        n_mat = side * side + 2
+        self.albedo = np.zeros((n_mat, 3), np.float32)
+        self.emission = np.zeros((n_mat, 3), np.float32)
+        # material 0: box, 1..side^2: spheres, last: light
+        bv, bi = open_box(*self.bounds)
+        self.build.add_mesh(bv, bi)
+        self.albedo[0] = 0.6
+        draws = mwc_uniform(seed, 3 * side * side)
+        self.albedo[1:1 + side * side] = (0.2 + 0.6 * draws).reshape(-1, 3)
+        for k in range(side * side):
+            gx, gz = k % side, k // side
+            centre = ((gx + 0.5) * spacing, radius, (gz + 0.5) * spacing)
+            v, i = S.icosphere(level, radius, centre, 1 + k)
+            self.build.add_mesh(v, i)
+        light = n_mat - 1
+        lv = np.array([[ext * 0.35, 7.9, ext * 0.35, 1], [ext * 0.65, 7.9, ext * 0.35, 1],
+                       [ext * 0.5, 7.9, ext * 0.65, 1]], np.float32)
+        self.build.add_mesh(lv, np.array([[0, 2, 1, light]], np.uint32))
+        self.emission[light] = 40.0
+        self.build.run(builder, threads)
+        self.desc = self.build.desc()
+        self.tri_normals, self.tri_materials = self.build.tri_attrs()
+        self.tri_count = len(self.tri_materials)
+        self.camera = ((ext * 0.5, 6.0, -10.0), (0.0, -6.0, 26.0), (0.0, 1.0, 0.0), 60.0)
+        self.offset = 1e-4 * float(np.linalg.norm(np.subtract(self.bounds[1], self.bounds[0])))
+
+
+class PathTracer:
+    """Device-resident PT frame loop over a SphereGridScene (or any scene with per-triangle
+    normals and materials)."""
+
+    def __init__(self, scene: SphereGridScene, width: int, height: int, spp: int = 1, max_path_len: int = 8,
+                 rr_start: int = 3, seed_base: int = 0x1234567, trace_cfg: N.TraceConfig | None = None):
+        self.s = scene
+        self.gpu_scene = S.Scene(scene.desc)
+        self.w, self.h, self.spp = width, height, spp
+        self.n = width * height * spp
+        self.max_path_len, self.rr_start, self.seed_base = max_path_len, rr_start, seed_base
+        self.cfg = trace_cfg
+        pos, d, up, fov = scene.camera
+        self.cam_pos = np.asarray(pos, np.float32)
+        self.r2w = np.zeros(16, np.float32)
+        dd, uu = np.asarray(d, np.float32), np.asarray(up, np.float32)
+        N.check(N.lib.imba_gen_camera_matrix(N.ptr(self.cam_pos), N.ptr(dd), N.ptr(uu), fov, width, height,
+                                             N.ptr(self.r2w)), "camera matrix")
+        self.rays = [N.DeviceBuffer(self.n * 32), N.DeviceBuffer(self.n * 32)]
+        self.states = [N.DeviceBuffer(self.n * STATE_BYTES), N.DeviceBuffer(self.n * STATE_BYTES)]
+        self.hits = N.DeviceBuffer(self.n * 16)
+        self.counts = None
+        self.tri_n = N.DeviceBuffer.from_array(scene.tri_normals)
+        self.tri_m = N.DeviceBuffer.from_array(scene.tri_materials)
+        self.albedo = N.DeviceBuffer.from_array(scene.albedo)
+        self.emission = N.DeviceBuffer.from_array(scene.emission)
+        self.image = N.DeviceBuffer(width * height * 12)
+        self.stream = N.Stream()
+        self.events = []
+
+    def frame(self, stats: bool = False, keep_queues: int = 0, timed: bool = False):
+        """Runs one frame. Returns per-bounce ray counts, and with stats=True the per-bounce
+        visit-count sums (nodes, triangles, instances); keep_queues > 0 also returns host
+        copies of the first keep_queues ray queues."""
+        st = self.stream.handle
+        n = self.n
+        N.check(N.lib.imba_pt_camera(N.ptr(self.r2w), N.ptr(self.cam_pos), self.w, self.h, self.spp, self.seed_base, 0,
+                                     n, self.rays[0].ptr, self.states[0].ptr, st), "camera")
+        cur = 0
+        counts, visit, queues, ev = [], [], [], []
+        if stats and self.counts is None:
+            self.counts = N.DeviceBuffer(self.n * 12)
+        for bounce in range(self.max_path_len):
+            if n == 0:
+                break
+            counts.append(n)
+            if len(queues) < keep_queues:
+                queues.append(self.rays[cur].download(N.RAY_DTYPE, n, st))
+            if stats:
+                N.check(N.lib.imba_intersect_stats(self.gpu_scene.h, self.rays[cur].ptr, self.hits.ptr,
+                                                   self.counts.ptr, n, st), "intersect_stats")
+                c = self.counts.download(np.uint32, 3 * n, st).reshape(-1, 3)
+                visit.append(c.astype(np.int64).sum(0))
+            else:
+                if timed:
+                    e0, e1 = N.Event(), N.Event()
+                    e0.record(self.stream)
+                N.check(N.lib.imba_intersect(self.gpu_scene.h, self.rays[cur].ptr, self.hits.ptr, n,
+                                             C.byref(self.cfg) if self.cfg else None, st), "intersect")
+                if timed:
+                    e1.record(self.stream)
+                    ev.append((e0, e1, n))
+            N.check(N.lib.imba_pt_bounce(self.rays[cur].ptr, self.hits.ptr, self.states[cur].ptr, n, self.tri_n.ptr,
+                                         self.tri_m.ptr, self.s.tri_count, self.albedo.ptr, self.emission.ptr,
+                                         len(self.s.albedo), self.max_path_len, self.rr_start, self.s.offset,
+                                         self.image.ptr, self.rays[cur].ptr, st), "bounce")
+            nn = C.c_int()
+            N.check(N.lib.imba_compact_rays(self.rays[cur].ptr, self.states[cur].ptr, STATE_BYTES, n,
+                                            self.rays[1 - cur].ptr, self.states[1 - cur].ptr, C.byref(nn), st),
+                    "compact_rays")
+            n = nn.value
+            cur = 1 - cur
+        self.events = ev
+        return counts, visit, queues
+
+    def trace_times_ms(self):
+        return [(e0.elapsed_ms(e1), n) for e0, e1, n in self.events]

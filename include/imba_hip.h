@@ -251,6 +251,24 @@ int imba_hashgrid_info(const ImbaHashGrid* g, ImbaHashGridInfo* info);
 /* Copies photon records into grid order (the reference's photons_ array): dst[slot] = src[order[slot]]. */
 int imba_hashgrid_gather(const ImbaHashGrid* g, const void* src, int stride, void* dst, void* stream);
 
+/* ------------------------------ path-tracing ray producers --------------------------------
+ * Device-side producers of the queues a path tracer traverses (the shading that the
+ * reference runs on the host around RayQueue, render/integrators/pt.cpp).  States are
+ * 32-byte records: int pixel_id, int sample_id, uint64 rng, float throughput[3], int bounces.
+ * imba_pt_camera: samples first..first+count-1 of a width x height x spp image; sample i is
+ *   pixel i / spp, seeded as render/ray_gen/ray_gen.h:51-72 with seed_base, jittered as
+ *   pt.cpp:168-177 and shot through raster_to_world (imba_gen_camera_matrix).
+ * imba_pt_bounce: for every ray of the queue and its hit: emission (added to image, may be
+ *   NULL) ends the path, otherwise a Lambertian bounce (pt.cpp:46-80) with Russian roulette
+ *   from bounce rr_start on; the new ray starts at the hit with tmin = offset.  Ended paths get
+ *   pixel_id = -1.  rays_out may equal rays. */
+int imba_pt_camera(const float* raster_to_world16, const float* cam_pos3, int width, int height, int spp,
+                   uint32_t seed_base, int first, int count, ImbaRay* rays, void* states, void* stream);
+int imba_pt_bounce(const ImbaRay* rays, const ImbaHit* hits, void* states, int count, const float* tri_normals,
+                   const int* tri_materials, int tri_count, const float* albedo, const float* emission,
+                   int material_count, int max_path_len, int rr_start, float offset, float* image,
+                   ImbaRay* rays_out, void* stream);
+
 /* ------------------------------ synthetic workloads ---------------------------------------
  * Seeded generators for the measured configurations (host memory outputs). */
 /* Random triangles in the unit cube: per triangle a centre ~ U[0,1)^3, then three vertices
@@ -265,6 +283,9 @@ int imba_gen_icosphere(int level, float radius, float cx, float cy, float cz, ui
  * tmin = 0, tmax = FLT_MAX. rays: host, w*h entries. */
 int imba_gen_camera_rays(const float* pos3, const float* dir3, const float* up3, float fov_deg,
                          int width, int height, ImbaRay* rays);
+/* The camera's raster -> world matrix (row major, applied to (raster y, raster x, 0, 1)). */
+int imba_gen_camera_matrix(const float* pos3, const float* dir3, const float* up3, float fov_deg,
+                           int width, int height, float* raster_to_world16);
 
 #ifdef __cplusplus
 }
