@@ -24,7 +24,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 
-HOST_SOURCES = ["host/vec.cpp", "host/status.cpp", "host/scene_build.cpp", "host/gen.cpp", "host/bvh_emit.cpp"]
+HOST_SOURCES = ["host/vec.cpp", "host/status.cpp", "host/scene_build.cpp", "host/gen.cpp", "host/bvh_emit.cpp",
+                "host/abi_check.cpp"]
 HIP_SOURCES = ["hip/runtime.hip", "hip/traverse.hip", "hip/compact.hip", "hip/hashgrid.hip", "hip/pt.hip"]
 
 HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-I" + INCLUDE, "-pthread"]

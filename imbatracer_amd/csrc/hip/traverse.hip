@@ -119,9 +119,12 @@ int launch_trace(const ImbaSceneDesc& d, const SceneInfo& info, int* d_err, cons
 struct FfiKey {
     int root;
     const void *nodes, *instances, *tris, *indices, *texcoords, *masks, *mask_buffer;
+    int counts[7];
     bool operator<(const FfiKey& o) const {
-        return std::tie(root, nodes, instances, tris, indices, texcoords, masks, mask_buffer) <
-               std::tie(o.root, o.nodes, o.instances, o.tris, o.indices, o.texcoords, o.masks, o.mask_buffer);
+        return std::tie(root, nodes, instances, tris, indices, texcoords, masks, mask_buffer, counts[0], counts[1],
+                        counts[2], counts[3], counts[4], counts[5], counts[6]) <
+               std::tie(o.root, o.nodes, o.instances, o.tris, o.indices, o.texcoords, o.masks, o.mask_buffer,
+                        o.counts[0], o.counts[1], o.counts[2], o.counts[3], o.counts[4], o.counts[5], o.counts[6]);
     }
 };
 struct FfiScene {
@@ -147,22 +150,32 @@ std::vector<T> fetch(const void* p, size_t bytes) {
     return v;
 }
 
-int ffi_scene(int root, const void* nodes, const void* instances, const void* tris, const void* indices,
-              const void* texcoords, const void* masks, const void* mask_buffer, FfiScene** out) {
-    FfiKey key{root, nodes, instances, tris, indices, texcoords, masks, mask_buffer};
+// `given` holds device pointers and, where known, element counts (0 = take the count from the
+// allocation that holds the pointer).
+int ffi_scene(const ImbaSceneDesc& given, FfiScene** out) {
+    const int root = given.root;
+    const void *nodes = given.nodes, *instances = given.instances, *tris = given.tris, *indices = given.indices,
+               *texcoords = given.texcoords, *masks = given.masks, *mask_buffer = given.mask_buffer;
+    FfiKey key{root, nodes, instances, tris, indices, texcoords, masks, mask_buffer,
+               {given.node_count, given.instance_count, given.tri_vec4_count, given.index_count,
+                given.texcoord_count, given.mask_count, given.mask_buffer_size}};
     std::lock_guard<std::mutex> lock(g_ffi_mutex);
     auto it = g_ffi_scenes.find(key);
     if (it != g_ffi_scenes.end()) { *out = &it->second; return IMBA_OK; }
 
+    auto count_of = [](int given_count, const void* p, size_t rec) {
+        if (given_count > 0) return given_count;
+        return (int)std::min<size_t>(bytes_after(p) / rec, INT32_MAX);
+    };
     ImbaSceneDesc d{};
     d.root = root;
-    d.node_count = (int)std::min<size_t>(bytes_after(nodes) / sizeof(ImbaNode), INT32_MAX);
-    d.instance_count = (int)std::min<size_t>(bytes_after(instances) / sizeof(ImbaInstanceNode), INT32_MAX);
-    d.tri_vec4_count = (int)std::min<size_t>(bytes_after(tris) / sizeof(ImbaVec4), INT32_MAX);
-    d.index_count = (int)std::min<size_t>(bytes_after(indices) / sizeof(int) / 4 * 4, INT32_MAX);
-    d.texcoord_count = (int)std::min<size_t>(bytes_after(texcoords) / sizeof(ImbaVec2), INT32_MAX);
-    d.mask_count = (int)std::min<size_t>(bytes_after(masks) / sizeof(ImbaTransparencyMask), INT32_MAX);
-    d.mask_buffer_size = (int)std::min<size_t>(bytes_after(mask_buffer), INT32_MAX);
+    d.node_count = count_of(given.node_count, nodes, sizeof(ImbaNode));
+    d.instance_count = count_of(given.instance_count, instances, sizeof(ImbaInstanceNode));
+    d.tri_vec4_count = count_of(given.tri_vec4_count, tris, sizeof(ImbaVec4));
+    d.index_count = count_of(given.index_count, indices, sizeof(int)) / 4 * 4;
+    d.texcoord_count = count_of(given.texcoord_count, texcoords, sizeof(ImbaVec2));
+    d.mask_count = count_of(given.mask_count, masks, sizeof(ImbaTransparencyMask));
+    d.mask_buffer_size = count_of(given.mask_buffer_size, mask_buffer, 1);
     if (d.node_count == 0 || d.instance_count == 0 || d.tri_vec4_count == 0)
         return fail(IMBA_ERR_INVALID_ARG, "scene arrays are not device allocations");
 
@@ -198,7 +211,16 @@ int ffi_scene(int root, const void* nodes, const void* instances, const void* tr
 void ffi_trace(bool any, int root, void* nodes, void* instances, void* tris, void* rays, void* hits, int* indices,
                void* texcoords, void* masks, char* mask_buffer, int count) {
     FfiScene* fs = nullptr;
-    int r = ffi_scene(root, nodes, instances, tris, indices, texcoords, masks, mask_buffer, &fs);
+    ImbaSceneDesc given{};
+    given.root = root;
+    given.nodes = (const ImbaNode*)nodes;
+    given.instances = (const ImbaInstanceNode*)instances;
+    given.tris = (const ImbaVec4*)tris;
+    given.indices = indices;
+    given.texcoords = (const ImbaVec2*)texcoords;
+    given.masks = (const ImbaTransparencyMask*)masks;
+    given.mask_buffer = (const uint8_t*)mask_buffer;
+    int r = ffi_scene(given, &fs);
     if (!r) r = launch_trace(fs->dev, fs->info, fs->d_err, (const ImbaRay*)rays, (ImbaHit*)hits, nullptr, count, any,
                              !fs->info.all_opaque, nullptr, nullptr);
     if (r) fprintf(stderr, "imbatracer_amd: %s_gpu_masked_instanced failed: %s\n", any ? "occluded" : "intersect",
@@ -236,6 +258,16 @@ int imba_occluded_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, 
     IMBA_CHECK_ARG(s && counts, "NULL argument");
     return launch_trace(s->dev, s->info, s->d_error, rays, hits, counts, count, true, !s->info.all_opaque, nullptr,
                         stream);
+}
+
+int imba_trace_desc(const ImbaSceneDesc* dev, const ImbaRay* rays, ImbaHit* hits, int count, int any_hit,
+                    void* stream) {
+    IMBA_CHECK_ARG(dev, "desc is NULL");
+    FfiScene* fs = nullptr;
+    int r = ffi_scene(*dev, &fs);
+    if (r) return r;
+    return launch_trace(fs->dev, fs->info, fs->d_err, rays, hits, nullptr, count, any_hit != 0, !fs->info.all_opaque,
+                        nullptr, stream);
 }
 
 // Reference FFI (render/scheduling/ray_queue.h:297-308, 347-358): synchronous, device pointers.

@@ -203,6 +203,12 @@ int imba_intersect_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits,
                          int count, void* stream);
 int imba_occluded_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts,
                         int count, void* stream);
+/* Traversal over caller-owned device arrays described by `dev` (device pointers and
+ * element counts), for callers that manage their own TraversalData.  The arrays are copied
+ * to the host and validated once per distinct set of pointers and counts.  any_hit: 0 closest
+ * hit, 1 occluded. */
+int imba_trace_desc(const ImbaSceneDesc* dev, const ImbaRay* rays, ImbaHit* hits, int count, int any_hit,
+                    void* stream);
 /* Number of rays the last launches found a malformed BVH for (0 for a valid scene). */
 int imba_trace_error_count(const ImbaScene* s, int* count);
 

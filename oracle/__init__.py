@@ -140,16 +140,20 @@ def brute_closest(tri_verts: np.ndarray, rays: np.ndarray):
     return ids, ts
 
 
+def _record_bytes(states):
+    return states.dtype.itemsize * (1 if states.ndim == 1 else int(np.prod(states.shape[1:])))
+
+
 def compact_hits(rays, hits, states):
     rays, hits, states = rays.copy(), hits.copy(), np.ascontiguousarray(states).copy()
-    stride = states.dtype.itemsize if states.ndim == 1 else states.strides[0]
+    stride = _record_bytes(states)
     n = lib().oracle_compact_hits(_p(rays), _p(hits), _p(states), stride, len(rays))
     return n, rays, hits, states
 
 
 def compact_rays(rays, states):
     rays, states = rays.copy(), np.ascontiguousarray(states).copy()
-    stride = states.dtype.itemsize if states.ndim == 1 else states.strides[0]
+    stride = _record_bytes(states)
     n = lib().oracle_compact_rays(_p(rays), _p(states), stride, len(rays))
     return n, rays, states
 

@@ -1,0 +1,80 @@
+"""GPU queue compaction against the oracle restatement of RayQueue::compact_hits / compact_rays
+(render/scheduling/ray_queue.h:161-207).
+
+compact_hits: the hit prefix (rays, hits, states) and the returned count must equal the
+reference loop's exactly; the miss suffix must be the same multiset (the reference's swap loop
+permutes it) and, for this library, the misses in their original order.
+compact_rays: survivors, their order and the new size must equal the reference loop's.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from imbatracer_amd import native as N
+from imbatracer_amd import scene as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _queue(n, hit_frac, seed, state_words=4):
+    rng = np.random.default_rng(seed)
+    rays = np.zeros(n, N.RAY_DTYPE)
+    rays["org"] = rng.random((n, 4), dtype=np.float32)
+    rays["dir"] = rng.random((n, 4), dtype=np.float32)
+    hits = np.zeros(n, N.HIT_DTYPE)
+    is_hit = rng.random(n) < hit_frac
+    hits["tri_id"] = np.where(is_hit, rng.integers(0, 1 << 30, n), -1)
+    hits["inst_id"] = rng.integers(-1, 100, n)
+    hits["tmax"] = rng.random(n, dtype=np.float32)
+    hits["u"] = rng.random(n, dtype=np.float32)
+    states = rng.integers(-5, 1 << 30, (n, state_words)).astype(np.int32)
+    states[:, 0] = np.arange(n)  # pixel id
+    return rays, hits, states
+
+
+def _bytes(a):
+    return np.ascontiguousarray(a).view(np.uint8).reshape(len(a), a.dtype.itemsize)
+
+
+def _gpu_compact_hits(rays, hits, states):
+    n = len(rays)
+    rb, hb, sb = (N.DeviceBuffer.from_array(x) for x in (rays, hits, states))
+    ro, ho, so = N.DeviceBuffer(max(rb.nbytes, 1)), N.DeviceBuffer(max(hb.nbytes, 1)), N.DeviceBuffer(max(sb.nbytes, 1))
+    k = S.compact_hits(rb, hb, sb, 4 * states.shape[1], n, ro, ho, so)
+    return (k, ro.download(N.RAY_DTYPE, n), ho.download(N.HIT_DTYPE, n),
+            so.download(np.int32, states.size).reshape(states.shape))
+
+
+@pytest.mark.parametrize("n,frac", [(0, 0.5), (1, 1.0), (1, 0.0), (63, 0.5), (64, 0.3), (2048, 0.5), (2049, 0.9),
+                                    (100_000, 0.37), (1_048_576, 0.5), (3_000_000, 0.99)])
+def test_compact_hits(gpu, n, frac):
+    rays, hits, states = _queue(n, frac, seed=n)
+    k, r, h, s = _gpu_compact_hits(rays, hits, states)
+    k_ref, r_ref, h_ref, s_ref = oracle.compact_hits(rays, hits, states)
+    assert k == k_ref == int((hits["tri_id"] >= 0).sum())
+    assert np.array_equal(_bytes(h[:k]), _bytes(h_ref[:k]))
+    assert np.array_equal(_bytes(r[:k]), _bytes(r_ref[:k]))
+    assert np.array_equal(s[:k], s_ref[:k])
+    # misses: same multiset as the reference loop, and in input order here
+    miss = hits["tri_id"] < 0
+    assert np.array_equal(np.sort(s[k:, 0]), np.sort(s_ref[k:, 0]))
+    assert np.array_equal(s[k:, 0], states[miss, 0])
+    assert np.array_equal(_bytes(r[k:]), _bytes(rays[miss]))
+
+
+@pytest.mark.parametrize("n,frac,words", [(0, 0.5, 4), (1, 0.0, 4), (777, 0.5, 8), (65536, 0.75, 4),
+                                          (1_048_576, 0.2, 10), (921_600, 1.0, 8)])
+def test_compact_rays(gpu, n, frac, words):
+    rays, hits, states = _queue(n, 0.5, seed=7 + n, state_words=words)
+    rng = np.random.default_rng(n)
+    alive = rng.random(n) < frac
+    states[~alive, 0] = -1
+    rb, sb = N.DeviceBuffer.from_array(rays), N.DeviceBuffer.from_array(states)
+    ro, so = N.DeviceBuffer(max(rb.nbytes, 1)), N.DeviceBuffer(max(sb.nbytes, 1))
+    k = S.compact_rays(rb, sb, 4 * states.shape[1], n, ro, so)
+    k_ref, r_ref, s_ref = oracle.compact_rays(rays, states)
+    assert k == k_ref
+    r = ro.download(N.RAY_DTYPE, n)
+    s = so.download(np.int32, states.size).reshape(states.shape)
+    assert np.array_equal(_bytes(r[:k]), _bytes(r_ref[:k]))
+    assert np.array_equal(s[:k], s_ref[:k])

@@ -45,6 +45,33 @@ def test_closest_and_occluded_c1_shape(gpu, builder):
     assert scene.error_count() == 0
 
 
+def test_c1_full_size(gpu):
+    """The CPU-runnable configuration at full size: 65,536 random triangles (binned SAH),
+    1024x1024 primary rays and 1,048,576 occlusion rays towards (0.5, 2, 0.5)."""
+    b, v, i = T.random_tri_scene(65536, N.BUILDER_BINNED_SAH)
+    arrays = b.arrays()
+    scene = S.Scene(b.desc())
+    prim = T.primary_rays(1024, 1024)
+    rb = N.DeviceBuffer.from_array(prim)
+    hb = N.DeviceBuffer(len(prim) * 16)
+    cb = N.DeviceBuffer(len(prim) * 12)
+    scene.intersect_stats(rb, hb, cb, len(prim))
+    got = hb.download(N.HIT_DTYPE, len(prim))
+    cnt = cb.download(np.uint32, 3 * len(prim)).reshape(-1, 3)
+    want, wcnt, bad = oracle.trace(arrays, prim, counts=True)
+    assert bad == 0
+    assert _same(got, want), _report(got, want)
+    assert np.array_equal(cnt, wcnt)
+    scene.intersect(rb, hb, len(prim))
+    assert _same(hb.download(N.HIT_DTYPE, len(prim)), want)
+    shadow = T.shadow_rays(prim, want)
+    got_o = scene.trace_host(shadow, any_hit=True)
+    want_o, bad = oracle.trace(arrays, shadow, any_hit=True, threads=16)
+    assert _same(got_o, want_o), _report(got_o, want_o)
+    # the primary hits form a plausible image of the triangle cloud
+    assert 0.2 < (want["tri_id"] >= 0).mean() < 0.99
+
+
 def test_incoherent_rays_and_stats(gpu):
     b, v, i = T.random_tri_scene(65536, N.BUILDER_BINNED_SAH)
     arrays = b.arrays()
