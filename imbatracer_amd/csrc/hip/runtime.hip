@@ -291,6 +291,16 @@ int imba_event_elapsed_ms(float* ms, void* start, void* end) {
 }
 
 // ---------------------------------------------------------------------------------------
+int imba_scene_validate(const ImbaSceneDesc* h, int* max_stack, int* all_opaque) {
+    IMBA_CHECK_ARG(h, "desc is NULL");
+    SceneInfo info;
+    int r = validate_scene(*h, info);
+    if (r) return r;
+    if (max_stack) *max_stack = info.max_stack;
+    if (all_opaque) *all_opaque = info.all_opaque ? 1 : 0;
+    return IMBA_OK;
+}
+
 int imba_scene_upload(const ImbaSceneDesc* h, ImbaScene** out) {
     IMBA_CHECK_ARG(h && out, "NULL argument");
     *out = nullptr;

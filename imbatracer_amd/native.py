@@ -123,7 +123,9 @@ _sig("imba_build_get_desc", i32, vp, C.POINTER(SceneDesc))
 _sig("imba_build_get_tri_attrs", i32, vp, C.POINTER(vp), C.POINTER(vp), ip)
 _sig("imba_bvh_emit_mesh", i32, C.POINTER(MeshDesc), i32, i32, i32, vp, sz, C.POINTER(sz))
 _sig("imba_bvh_emit_boxes", i32, vp, vp, i32, i32, i32, vp, sz, C.POINTER(sz))
+_sig("imba_scene_validate", i32, C.POINTER(SceneDesc), ip, ip)
 _sig("imba_scene_upload", i32, C.POINTER(SceneDesc), C.POINTER(vp))
+_sig("imba_trace_desc", i32, C.POINTER(SceneDesc), vp, vp, i32, i32, vp)
 _sig("imba_scene_release", i32, vp)
 _sig("imba_scene_device_desc", i32, vp, C.POINTER(SceneDesc))
 _sig("imba_scene_footprint", i32, vp, C.POINTER(sz))

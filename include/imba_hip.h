@@ -171,6 +171,10 @@ int imba_bvh_emit_boxes(const float* boxes, const float* centres, int n, int wid
  * indices in range and increasing from parent to child, leaf records in range, instance
  * roots in range, leaf sentinels present. */
 typedef struct ImbaScene ImbaScene;
+/* The upload's checks alone, on host arrays (no device needed): IMBA_ERR_BAD_BVH with a
+ * message for a malformed scene; otherwise the deepest traversal stack a ray can need and
+ * whether every mask texel is opaque. */
+int imba_scene_validate(const ImbaSceneDesc* host, int* max_stack, int* all_opaque);
 int imba_scene_upload(const ImbaSceneDesc* host, ImbaScene** out);
 int imba_scene_release(ImbaScene* s);
 /* Device pointers of the resident arrays (counts as uploaded). */

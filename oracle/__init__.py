@@ -92,6 +92,10 @@ def ref_lib():
         R.ref_emit_boxes.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_size_t,
                                      C.POINTER(C.c_size_t)]
         R.ref_emit_boxes.restype = C.c_int
+        R.ref_rng.argtypes = [C.c_uint64, C.c_int, C.c_void_p, C.c_void_p]
+        R.ref_rng.restype = None
+        R.ref_random_triangles.argtypes = [C.c_uint64, C.c_int, C.c_void_p]
+        R.ref_random_triangles.restype = None
         _ref = R
     return _ref
 
@@ -198,6 +202,19 @@ def ref_emit_mesh(vertices, indices, builder: int, width: int, cost: int) -> np.
     out = np.empty(need.value, np.int32)
     R.ref_emit_mesh(_p(v), len(v), _p(i), len(i), builder, width, cost, _p(out), out.size, C.byref(need))
     return out
+
+
+def ref_rng(seed: int, n: int):
+    raw = np.empty(n, np.uint32)
+    uni = np.empty(n, np.float32)
+    ref_lib().ref_rng(seed, n, _p(raw), _p(uni))
+    return raw, uni
+
+
+def ref_random_triangles(seed: int, n: int) -> np.ndarray:
+    v = np.empty((3 * n, 4), np.float32)
+    ref_lib().ref_random_triangles(seed, n, _p(v))
+    return v
 
 
 def ref_emit_boxes(boxes, centres, width: int, cost: int) -> np.ndarray:

@@ -13,6 +13,7 @@
 #include "imbatracer/core/fast_bvh_builder.h"
 #include "imbatracer/core/mesh.h"
 #include "imbatracer/core/sbvh_builder.h"
+#include "imbatracer/render/random.h"
 
 namespace {
 
@@ -117,6 +118,31 @@ extern "C" int ref_emit_mesh(const float* v4, int nv, const uint32_t* idx4, int 
         if (width == 2) run_mesh<2, CpuCost>(m, builder, 1e-4f, v); else run_mesh<4, CpuCost>(m, builder, 1e-4f, v);
     }
     return finish(v, out, cap, len);
+}
+
+// The reference generator (render/random.h:15-47): n draws of random_int(0, 0x7FFFFFFF)
+// (= MWC64X() mod 0x7FFFFFFF; the raw draw is private), and n random_float() draws of a
+// second generator with the same seed.
+extern "C" void ref_rng(uint64_t seed, int n, uint32_t* raw, float* uniform) {
+    imba::RNG a(seed), b(seed);
+    for (int i = 0; i < n; i++) raw[i] = (uint32_t)a.random_int(0, 0x7FFFFFFF);
+    for (int i = 0; i < n; i++) uniform[i] = b.random_float();
+}
+
+// Random-triangle scene of the CPU-runnable configuration drawn with the reference generator:
+// per triangle a centre (3 x random_float()), then three vertices centre + random_float(-0.01, 0.01)
+// per coordinate.
+extern "C" void ref_random_triangles(uint64_t seed, int n, float* vertices) {
+    imba::RNG rng(seed);
+    for (int t = 0; t < n; t++) {
+        float c[3];
+        for (int k = 0; k < 3; k++) c[k] = rng.random_float();
+        for (int v = 0; v < 3; v++) {
+            float* out = vertices + 4 * (3 * (size_t)t + v);
+            for (int k = 0; k < 3; k++) out[k] = c[k] + rng.random_float(-0.01f, 0.01f);
+            out[3] = 1.0f;
+        }
+    }
 }
 
 extern "C" int ref_emit_boxes(const float* boxes, const float* centres, int n, int width, int cost, int32_t* out,
