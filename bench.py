@@ -79,7 +79,7 @@ def run_pt(args):
     threads = max(1, min(16, os.cpu_count() or 1))
     scene = W.SphereGridScene(side=args.side, level=5, builder=N.BUILDER_SBVH, threads=threads)
     log("scene: %d triangles, %d nodes, built in %.1f s" % (scene.tri_count, scene.desc.node_count, time.time() - t0))
-    cfg = N.TraceConfig(args.block, args.stack, 0, 0) if args.block else None
+    cfg = N.TraceConfig(args.block, args.stack, args.persistent, args.blocks_per_cu) if (args.block or args.persistent or args.blocks_per_cu) else None
     pt = W.PathTracer(scene, args.width, args.height, spp=1, max_path_len=8, rr_start=3, trace_cfg=cfg)
     log("uploaded %.2f GB" % (pt.gpu_scene.footprint() / 1e9))
 
@@ -147,7 +147,7 @@ def run_c1(args):
     rb = N.DeviceBuffer.from_array(prim)
     hb = N.DeviceBuffer(n * 16)
     cb = N.DeviceBuffer(n * 12)
-    cfg = N.TraceConfig(args.block, args.stack, 0, 0) if args.block else None
+    cfg = N.TraceConfig(args.block, args.stack, args.persistent, args.blocks_per_cu) if (args.block or args.persistent or args.blocks_per_cu) else None
     sc.intersect_stats(rb, hb, cb, n)
     hits = hb.download(N.HIT_DTYPE, n)
     vis_p = cb.download(np.uint32, 3 * n).reshape(-1, 3).astype(np.int64).sum(0)
@@ -202,6 +202,8 @@ def main():
     ap.add_argument("--side", type=int, default=32, help="spheres per side of the grid")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0)
+    ap.add_argument("--persistent", type=int, default=0)
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

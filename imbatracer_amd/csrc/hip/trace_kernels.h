@@ -107,16 +107,18 @@ __device__ __forceinline__ bool tri_hit(const RaySpace& r, const float4& a, cons
 }
 
 // Opacity of the material mask at the hit (texel: wrap the interpolated texcoords to [0,1),
-// scale by the mask size, truncate, clamp to the last texel).
-__device__ __forceinline__ bool texel_opaque(const TraceArgs& a, int tri, float u, float v, bool& bad) {
-    if ((unsigned)tri >= (unsigned)a.n_tri_ids) { bad = true; return true; }
+// scale by the mask size, truncate, clamp to the last texel).  Returns 1 when opaque, 0 when
+// transparent, and 3 (opaque, bit 1 set) when the scene arrays are inconsistent.
+__device__ __forceinline__ int texel_opaque(const TraceArgs& a, int tri, float u, float v) {
+    if ((unsigned)tri >= (unsigned)a.n_tri_ids) return 3;
     const int4 ind = a.indices[tri];
-    if ((unsigned)ind.w >= (unsigned)a.n_masks) { bad = true; return true; }
+    if ((unsigned)ind.w >= (unsigned)a.n_masks) return 3;
     const int w = a.masks[3 * ind.w + 0], h = a.masks[3 * ind.w + 1], off = a.masks[3 * ind.w + 2];
     int texel = 0;
     if (!(w == 1 && h == 1)) {
         if ((unsigned)ind.x >= (unsigned)a.n_texcoords || (unsigned)ind.y >= (unsigned)a.n_texcoords ||
-            (unsigned)ind.z >= (unsigned)a.n_texcoords) { bad = true; return true; }
+            (unsigned)ind.z >= (unsigned)a.n_texcoords)
+            return 3;
         const float2 t0 = a.texcoords[ind.x], t1 = a.texcoords[ind.y], t2 = a.texcoords[ind.z];
         const float w0 = (1.0f - u) - v;
         const float s = fmaf(t0.x, w0, fmaf(t1.x, u, t2.x * v));
@@ -128,8 +130,8 @@ __device__ __forceinline__ bool texel_opaque(const TraceArgs& a, int tri, float 
         texel = yi * w + xi;
     }
     const long long at = (long long)off + texel;
-    if (at < 0 || at >= a.mask_buf_size) { bad = true; return true; }
-    return a.mask_buf[at] != 0;
+    if (at < 0 || at >= a.mask_buf_size) return 3;
+    return a.mask_buf[at] != 0 ? 1 : 0;
 }
 
 template <int BLOCK, int STACK_LDS>
@@ -160,140 +162,201 @@ struct LaneStack {
     }
 };
 
-struct HitRecord {
-    float t, u;
-    int tri, inst;
+enum : int { kTop = 1, kInstLast = 2, kBad = 4, kDone = 8 };
+
+// Traversal state of one ray (kept in registers across the steps of the persistent kernel).
+struct TraceState {
+    RaySpace world;  // world-space ray
+    RaySpace cs;     // ray in the space of the current level (world or an instance's object space)
+    float tmin, tmax;
+    float best_t, best_u;
+    int best_tri, best_inst;
+    int node;        // next node / leaf link to process, kNoNode if none
+    int tri;         // next triangle record of the current leaf, -1 if none
+    int inst, inst_id, inst_sp;  // current instance, its id, stack depth when it was entered
+    int flags;
+    uint32_t cn, ct, ci;  // visit counts (stats variant)
 };
 
-// Traverses one ray. Returns the hit record; visit counts are accumulated in c_*.
-template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS>
-__device__ __forceinline__ HitRecord trace_ray(const TraceArgs& a, float4 ro, float4 rd, LaneStack<BLOCK, STACK_LDS>& st,
-                                               uint32_t& c_nodes, uint32_t& c_tris, uint32_t& c_inst, bool& bad) {
-    RaySpace world;
-    world.ox = ro.x; world.oy = ro.y; world.oz = ro.z;
-    world.dx = rd.x; world.dy = rd.y; world.dz = rd.z;
-    setup_space(world);
-    RaySpace cs = world;
-    const float tmin = ro.w;
-    const float tmax = rd.w;
+__device__ __forceinline__ void state_init(TraceState& s, const float4& ro, const float4& rd, int root) {
+    s.world.ox = ro.x; s.world.oy = ro.y; s.world.oz = ro.z;
+    s.world.dx = rd.x; s.world.dy = rd.y; s.world.dz = rd.z;
+    setup_space(s.world);
+    s.cs = s.world;
+    s.tmin = ro.w;
+    s.tmax = rd.w;
+    s.best_t = rd.w;
+    s.best_u = 0.0f;
+    s.best_tri = -1;
+    s.best_inst = -1;
+    s.node = root;
+    s.tri = -1;
+    s.inst = -1;
+    s.inst_id = -1;
+    s.inst_sp = -1;
+    s.flags = kTop | kInstLast;
+    s.cn = s.ct = s.ci = 0;
+}
 
-    HitRecord best;
-    best.t = tmax;
-    best.u = 0.0f;
-    best.tri = -1;
-    best.inst = -1;
+// Enters instance i of the current top-level leaf: the ray goes to its object space.
+// Returns the instance's root node, or kNoNode for a malformed record.
+template <bool STATS>
+__device__ __forceinline__ int enter_instance(const TraceArgs& a, TraceState& s, int i, int sp) {
+    if ((unsigned)i >= (unsigned)a.n_inst) {
+        s.flags |= kBad | kInstLast;
+        return kNoNode;
+    }
+    const float4 m0 = a.inst[4 * i + 0], m1 = a.inst[4 * i + 1], m2 = a.inst[4 * i + 2];
+    const int4 meta = reinterpret_cast<const int4*>(a.inst)[4 * i + 3];
+    if (STATS) s.ci++;
+    const RaySpace& w = s.world;
+    s.cs.ox = fmaf(m0.x, w.ox, fmaf(m0.y, w.oy, fmaf(m0.z, w.oz, m0.w)));
+    s.cs.oy = fmaf(m1.x, w.ox, fmaf(m1.y, w.oy, fmaf(m1.z, w.oz, m1.w)));
+    s.cs.oz = fmaf(m2.x, w.ox, fmaf(m2.y, w.oy, fmaf(m2.z, w.oz, m2.w)));
+    s.cs.dx = fmaf(m0.x, w.dx, fmaf(m0.y, w.dy, m0.z * w.dz));
+    s.cs.dy = fmaf(m1.x, w.dx, fmaf(m1.y, w.dy, m1.z * w.dz));
+    s.cs.dz = fmaf(m2.x, w.dx, fmaf(m2.y, w.dy, m2.z * w.dz));
+    setup_space(s.cs);
+    s.inst = i;
+    s.inst_id = meta.x;
+    s.inst_sp = sp;
+    s.flags = (s.flags & ~(kTop | kInstLast)) | (meta.z == -1 ? kInstLast : 0);
+    if ((unsigned)meta.y >= (unsigned)a.n_nodes) {
+        s.flags |= kBad;
+        return kNoNode;
+    }
+    return meta.y;
+}
 
-    int node = a.root;
-    bool top = true;
-    int inst_sp = -1;
-    int inst = -1, inst_id = -1;
-    bool inst_last = true;
-    st.sp = 0;
-
-    auto enter_instance = [&](int i) -> int {
-        if ((unsigned)i >= (unsigned)a.n_inst) { bad = true; inst_last = true; return kNoNode; }
-        const float4 m0 = a.inst[4 * i + 0], m1 = a.inst[4 * i + 1], m2 = a.inst[4 * i + 2];
-        const int4 meta = reinterpret_cast<const int4*>(a.inst)[4 * i + 3];
-        if (STATS) c_inst++;
-        cs.ox = fmaf(m0.x, world.ox, fmaf(m0.y, world.oy, fmaf(m0.z, world.oz, m0.w)));
-        cs.oy = fmaf(m1.x, world.ox, fmaf(m1.y, world.oy, fmaf(m1.z, world.oz, m1.w)));
-        cs.oz = fmaf(m2.x, world.ox, fmaf(m2.y, world.oy, fmaf(m2.z, world.oz, m2.w)));
-        cs.dx = fmaf(m0.x, world.dx, fmaf(m0.y, world.dy, m0.z * world.dz));
-        cs.dy = fmaf(m1.x, world.dx, fmaf(m1.y, world.dy, m1.z * world.dz));
-        cs.dz = fmaf(m2.x, world.dx, fmaf(m2.y, world.dy, m2.z * world.dz));
-        setup_space(cs);
-        inst = i;
-        inst_id = meta.x;
-        inst_last = meta.z == -1;
-        inst_sp = st.sp;
-        top = false;
-        if ((unsigned)meta.y >= (unsigned)a.n_nodes) { bad = true; return kNoNode; }
-        return meta.y;
-    };
-
-    for (;;) {
-        if (node >= 0) {
-            if (node >= a.n_nodes) {
-                bad = true;
-                node = kNoNode;
-            } else {
-                const float4 n0 = a.nodes[4 * node + 0];
-                const float4 n1 = a.nodes[4 * node + 1];
-                const float4 n2 = a.nodes[4 * node + 2];
-                const int4 n3 = reinterpret_cast<const int4*>(a.nodes)[4 * node + 3];
-                if (STATS) c_nodes++;
-                float tl = 0.0f, tr = 0.0f;
-                bool hl = box_hit(cs, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, tmin, best.t, tl);
-                bool hr = n3.y != kDummyChild && box_hit(cs, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, tmin, best.t, tr);
-                if (hl && n3.x >= 0 && n3.x <= node) { hl = false; bad = true; }
-                if (hr && n3.y >= 0 && n3.y <= node) { hr = false; bad = true; }
-                if (hl && hr) {
-                    const bool left_first = tl <= tr;
-                    if (!st.push(left_first ? n3.y : n3.x)) bad = true;
-                    node = left_first ? n3.x : n3.y;
-                } else if (hl) {
-                    node = n3.x;
-                } else if (hr) {
-                    node = n3.y;
-                } else {
-                    node = kNoNode;
-                }
-            }
-        } else if (node != kNoNode) {
-            const int leaf = ~node;
-            if (top) {
-                node = enter_instance(leaf);
-                if (node != kNoNode) continue;
-            } else {
-                bool done = false;
-                for (int t = leaf;; t += 3) {
-                    if (t + 2 >= a.n_tris4) { bad = true; break; }
-                    const float4 v0 = a.tris[t], v1 = a.tris[t + 1], v2 = a.tris[t + 2];
-                    if (STATS) c_tris++;
-                    float th, uh, vh;
-                    if (tri_hit(cs, v0, v1, v2, th, uh, vh) && th >= tmin) {
-                        const int tri = __float_as_int(v1.w);
-                        if (ANY) {
-                            if (th <= tmax && (!MASKED || texel_opaque(a, tri, uh, vh, bad))) {
-                                best.t = th; best.u = uh; best.tri = tri; best.inst = inst_id;
-                                done = true;
-                                break;
-                            }
-                        } else {
-                            const bool better = th < best.t ||
-                                (th == best.t && ((unsigned)tri < (unsigned)best.tri ||
-                                                  (tri == best.tri && (unsigned)inst_id < (unsigned)best.inst)));
-                            if (better && (!MASKED || texel_opaque(a, tri, uh, vh, bad))) {
-                                best.t = th; best.u = uh; best.tri = tri; best.inst = inst_id;
-                            }
-                        }
-                    }
-                    if (__float_as_uint(v2.w) == kLeafEnd) break;
-                }
-                if (done) break;
-                node = kNoNode;
-            }
-        }
-
-        if (node == kNoNode) {
-            // The current instance's tree is exhausted: enter the next instance of the same
-            // top-level leaf, or return to the top level in world space.
-            while (!top && st.sp == inst_sp) {
-                if (inst_last) {
-                    top = true;
-                    cs = world;
-                    inst_sp = -1;
-                    break;
-                }
-                node = enter_instance(inst + 1);
-                if (node != kNoNode) break;
-            }
-            if (node != kNoNode) continue;
-            if (st.sp == 0) break;
-            node = st.pop();
+// A leaf link in s.node: top-level leaves enter their first instance, mesh leaves start
+// their triangle list.
+template <bool STATS>
+__device__ __forceinline__ void resolve_leaf(const TraceArgs& a, TraceState& s, int sp) {
+    if (s.node < 0 && s.node != kNoNode) {
+        if (s.flags & kTop) {
+            s.node = enter_instance<STATS>(a, s, ~s.node, sp);
+        } else {
+            s.tri = ~s.node;
+            s.node = kNoNode;
         }
     }
-    return best;
+}
+
+// Advances one ray by one step: bookkeeping (next instance, back to world space, pop) if
+// nothing is pending, then one triangle test or one inner node.  Returns true when the ray
+// is finished.
+template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS>
+__device__ __forceinline__ bool advance(const TraceArgs& a, TraceState& s, LaneStack<BLOCK, STACK_LDS>& st) {
+    if (s.tri < 0 && s.node == kNoNode) {
+        // the current instance's tree is exhausted: next instance of the leaf or world space
+        while (!(s.flags & kTop) && st.sp == s.inst_sp) {
+            if (s.flags & kInstLast) {
+                s.flags |= kTop;
+                s.cs = s.world;
+                s.inst_sp = -1;
+                break;
+            }
+            s.node = enter_instance<STATS>(a, s, s.inst + 1, st.sp);
+            if (s.node != kNoNode) break;
+        }
+        if (s.node == kNoNode) {
+            if (st.sp == 0) return true;
+            s.node = st.pop();
+            resolve_leaf<STATS>(a, s, st.sp);
+        }
+    }
+    if (s.tri >= 0) {
+        const int t = s.tri;
+        if (t + 2 >= a.n_tris4) {
+            s.flags |= kBad;
+            s.tri = -1;
+            return false;
+        }
+        const float4 v0 = a.tris[t], v1 = a.tris[t + 1], v2 = a.tris[t + 2];
+        if (STATS) s.ct++;
+        s.tri = (__float_as_uint(v2.w) == kLeafEnd) ? -1 : t + 3;
+        float th, uh, vh;
+        if (tri_hit(s.cs, v0, v1, v2, th, uh, vh) && th >= s.tmin) {
+            const int tid = __float_as_int(v1.w);
+            if (ANY) {
+                if (th <= s.tmax) {
+                    int op = 1;
+                    if (MASKED) {
+                        op = texel_opaque(a, tid, uh, vh);
+                        if (op & 2) s.flags |= kBad;
+                    }
+                    if (op & 1) {
+                        s.best_t = th; s.best_u = uh; s.best_tri = tid; s.best_inst = s.inst_id;
+                        return true;
+                    }
+                }
+            } else {
+                const bool better = th < s.best_t ||
+                    (th == s.best_t && ((unsigned)tid < (unsigned)s.best_tri ||
+                                        (tid == s.best_tri && (unsigned)s.inst_id < (unsigned)s.best_inst)));
+                if (better) {
+                    int op = 1;
+                    if (MASKED) {
+                        op = texel_opaque(a, tid, uh, vh);
+                        if (op & 2) s.flags |= kBad;
+                    }
+                    if (op & 1) {
+                        s.best_t = th; s.best_u = uh; s.best_tri = tid; s.best_inst = s.inst_id;
+                    }
+                }
+            }
+        }
+        return false;
+    }
+    if (s.node >= 0) {
+        const int n = s.node;
+        if (n >= a.n_nodes) {
+            s.flags |= kBad;
+            s.node = kNoNode;
+            return false;
+        }
+        const float4 n0 = a.nodes[4 * n + 0];
+        const float4 n1 = a.nodes[4 * n + 1];
+        const float4 n2 = a.nodes[4 * n + 2];
+        const int4 n3 = reinterpret_cast<const int4*>(a.nodes)[4 * n + 3];
+        if (STATS) s.cn++;
+        float tl = 0.0f, tr = 0.0f;
+        bool hl = box_hit(s.cs, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, s.tmin, s.best_t, tl);
+        bool hr = n3.y != kDummyChild && box_hit(s.cs, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, s.tmin, s.best_t, tr);
+        if (hl && n3.x >= 0 && n3.x <= n) { hl = false; s.flags |= kBad; }
+        if (hr && n3.y >= 0 && n3.y <= n) { hr = false; s.flags |= kBad; }
+        if (hl && hr) {
+            const bool left_first = tl <= tr;
+            if (!st.push(left_first ? n3.y : n3.x)) s.flags |= kBad;
+            s.node = left_first ? n3.x : n3.y;
+        } else {
+            s.node = hl ? n3.x : (hr ? n3.y : kNoNode);
+        }
+        resolve_leaf<STATS>(a, s, st.sp);
+    }
+    return false;
+}
+
+template <int BLOCK, int STACK_LDS>
+__device__ __forceinline__ void stack_init(LaneStack<BLOCK, STACK_LDS>& st, int* lds, const TraceArgs& a) {
+    st.lds = lds;
+    st.spill = a.spill;
+    st.spill_depth = a.spill_depth;
+    st.lane_slot = threadIdx.x;
+    st.spill_slot = blockIdx.x * BLOCK + threadIdx.x;
+    st.spill_stride = gridDim.x * BLOCK;
+    st.sp = 0;
+}
+
+template <bool STATS>
+__device__ __forceinline__ void write_result(const TraceArgs& a, const TraceState& s, int id) {
+    a.hits[id] = make_int4(s.best_tri, s.best_inst, __float_as_int(s.best_t), __float_as_int(s.best_u));
+    if (STATS) {
+        a.counts[3 * (size_t)id + 0] = s.cn;
+        a.counts[3 * (size_t)id + 1] = s.ct;
+        a.counts[3 * (size_t)id + 2] = s.ci;
+    }
+    if (s.flags & kBad) atomicAdd(a.err, 1);
 }
 
 // One ray per thread, rays strided over a grid sized to the resident waves.
@@ -301,27 +364,68 @@ template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS>
 __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     __shared__ int lds_stack[STACK_LDS * BLOCK];
     LaneStack<BLOCK, STACK_LDS> st;
-    st.lds = lds_stack;
-    st.spill = a.spill;
-    st.spill_depth = a.spill_depth;
-    st.lane_slot = threadIdx.x;
-    st.spill_slot = blockIdx.x * BLOCK + threadIdx.x;
-    st.spill_stride = gridDim.x * BLOCK;
-    st.sp = 0;
+    stack_init(st, lds_stack, a);
     const int stride = gridDim.x * BLOCK;
     for (int id = blockIdx.x * BLOCK + threadIdx.x; id < a.count; id += stride) {
-        const float4 ro = a.rays[2 * (size_t)id];
-        const float4 rd = a.rays[2 * (size_t)id + 1];
-        uint32_t cn = 0, ct = 0, ci = 0;
-        bool bad = false;
-        const HitRecord h = trace_ray<BLOCK, STACK_LDS, ANY, MASKED, STATS>(a, ro, rd, st, cn, ct, ci, bad);
-        a.hits[id] = make_int4(h.tri, h.inst, __float_as_int(h.t), __float_as_int(h.u));
-        if (STATS) {
-            a.counts[3 * (size_t)id + 0] = cn;
-            a.counts[3 * (size_t)id + 1] = ct;
-            a.counts[3 * (size_t)id + 2] = ci;
+        TraceState s;
+        state_init(s, a.rays[2 * (size_t)id], a.rays[2 * (size_t)id + 1], a.root);
+        st.sp = 0;
+        while (!advance<BLOCK, STACK_LDS, ANY, MASKED, STATS>(a, s, st)) {
         }
-        if (bad) atomicAdd(a.err, 1);
+        write_result<STATS>(a, s, id);
+    }
+}
+
+// Persistent variant: every wave keeps 64 rays in flight and, whenever at least `refill` of its
+// lanes have finished, takes the next consecutive block of ray ids for them with one atomic
+// on a global counter (ballot + popcount + mbcnt).  Each iteration advances every active lane
+// by one step of `advance`, so the per-ray results are those of k_trace.
+// counter[0] = next ray, counter[1] = finished waves; the last wave to finish resets both.
+template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS>
+__global__ __launch_bounds__(BLOCK) void k_trace_dyn(TraceArgs a, int refill) {
+    __shared__ int lds_stack[STACK_LDS * BLOCK];
+    LaneStack<BLOCK, STACK_LDS> st;
+    stack_init(st, lds_stack, a);
+    const int lane = __lane_id();
+    int* counter = a.work_counter;
+    int id = -1;             // ray of this lane, -1 = none
+    bool exhausted = false;  // wave-uniform: no rays left to fetch
+    TraceState s;
+    for (;;) {
+        const unsigned long long idle = __ballot(id < 0);
+        if (!exhausted && __popcll(idle) >= refill) {
+            const int n = __popcll(idle);
+            const int leader = __ffsll((long long)idle) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(&counter[0], n);
+            base = __shfl(base, leader, 64);
+            if (base + n >= a.count) exhausted = true;
+            if (id < 0) {
+                const int r = base + __builtin_amdgcn_mbcnt_hi((unsigned)(idle >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((unsigned)idle, 0u));
+                if (r < a.count) {
+                    id = r;
+                    state_init(s, a.rays[2 * (size_t)r], a.rays[2 * (size_t)r + 1], a.root);
+                    st.sp = 0;
+                }
+            }
+        }
+        if (__ballot(id >= 0) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (id >= 0 && advance<BLOCK, STACK_LDS, ANY, MASKED, STATS>(a, s, st)) {
+            write_result<STATS>(a, s, id);
+            id = -1;
+        }
+    }
+    // the last wave out resets the counters for the launch that reuses this slot
+    if (lane == 0) {
+        const int waves = gridDim.x * (BLOCK / 64);
+        if (atomicAdd(&counter[1], 1) == waves - 1) {
+            atomicExch(&counter[0], 0);
+            atomicExch(&counter[1], 0);
+        }
     }
 }
 

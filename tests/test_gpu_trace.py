@@ -92,7 +92,7 @@ def test_incoherent_rays_and_stats(gpu):
         assert _same(hb.download(N.HIT_DTYPE, len(rays)), want)
 
 
-@pytest.mark.parametrize("shape", [(256, 32), (256, 16), (128, 32), (128, 48), (64, 64)])
+@pytest.mark.parametrize("shape", [(256, 16), (256, 8), (256, 32), (128, 16), (128, 32), (64, 32), (64, 64)])
 def test_launch_shapes_agree(gpu, shape):
     b, v, i = T.random_tri_scene(20000, N.BUILDER_SBVH)
     scene = S.Scene(b.desc())
@@ -105,6 +105,24 @@ def test_launch_shapes_agree(gpu, shape):
             assert _same(got, want), _report(got, want)
         else:
             assert np.array_equal(got["tri_id"] >= 0, oracle.trace(b.arrays(), rays, any_hit=True)[0]["tri_id"] >= 0)
+
+
+@pytest.mark.parametrize("shape", [(256, 32, 1), (256, 16, 16), (256, 8, 64), (128, 32, 8), (64, 64, 32)])
+def test_persistent_fetch_agrees(gpu, shape):
+    """The persistent kernel (rays fetched per wave from a global counter) gives the
+    oracle's results, repeatedly (its counter must return to zero after each launch)."""
+    b = T.instanced_scene()
+    arrays = b.arrays()
+    scene = S.Scene(b.desc())
+    rays = T.random_rays(50000, seed=12, box=(-2.0, 2.0))
+    cfg = N.TraceConfig(shape[0], shape[1], shape[2], 0)
+    for any_hit in (False, True):
+        want, _ = oracle.trace(arrays, rays, any_hit=any_hit, threads=8)
+        for _ in range(3):
+            got = scene.trace_host(rays, any_hit=any_hit, cfg=cfg)
+            assert _same(got, want), _report(got, want)
+    small = rays[:37]
+    assert _same(scene.trace_host(small, cfg=cfg), oracle.trace(arrays, small)[0])
 
 
 def test_instanced_scene(gpu):
