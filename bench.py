@@ -14,15 +14,18 @@ and upload are outside the timed region).
 Also reported: the roofline of the traversal kernel (algorithmic bytes per ray
 32 + 16 + 64 N_node + 48 N_tri + 64 N_inst with visit counts measured by the stats variant of
 the same kernel on the same rays, over the kernel time measured with HIP events on its stream)
-and a CPU baseline (the C restatement of the traversal, oracle/trace.c, on host threads over a
-bounded sample of the same rays).
+and a CPU baseline: the reference's CPU path restated (oracle/cpu_bvh4.c over the reference's
+CPU layout, BVH4 nodes + 4-triangle packets built from the same scene) on host threads over a
+bounded sample of the same rays.
 """
 from __future__ import annotations
 
 import argparse
 import ctypes as C
+import glob
 import json
 import os
+import re
 import sys
 import time
 
@@ -45,30 +48,60 @@ def ray_bytes(visits, rays):
     return 32.0 * rays + 16.0 * rays + 64.0 * nodes + 48.0 * tris + 64.0 * inst
 
 
-def cpu_baseline(desc, ray_sets, any_flags, budget_s=12.0):
-    """Times the oracle traversal on host threads over a bounded sample of the given rays."""
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` on `workload` from the rocprofv3 PMC summaries under
+    profiles/ (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a device copy of known size),
+    as (bytes, file), or (None, None) when no summary covers this workload."""
+    pat = re.compile(r"traffic: workload=(\S+) kernel=(\S+) hbm_bytes_per_launch=(\d+)")
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(f) as fh:
+            for line in fh:
+                m = pat.search(line)
+                if m and m.group(1) == workload and m.group(2) == kernel:
+                    return int(m.group(3)), os.path.relpath(f, ROOT)
+    return None, None
+
+
+def cpu_baseline(build, builder, threads_build, ray_sets, any_flags, budget_s=12.0):
+    """Times the reference's CPU path restated (oracle/cpu_bvh4.c: BVH4 nodes + 4-triangle
+    packets of core/cpu_adapter.cpp, 4-wide SSE/FMA box and triangle tests, one ray per host
+    thread at a time) on the host cores over a bounded sample of the benchmarked rays: one
+    untimed pass, then whole passes until the time budget is used."""
     import oracle
 
+    t0 = time.perf_counter()
+    build.cpu_layout(builder, threads_build)
+    t_layout = time.perf_counter() - t0
+    desc = build.cpu_desc()
     threads = max(1, min(16, os.cpu_count() or 1))
     od = oracle.SceneDesc(*[getattr(desc, f) for f, _ in oracle.SceneDesc._fields_])
     L = oracle.lib()
-    done, elapsed, passes = 0, 0.0, 0
     sets = [(np.ascontiguousarray(r), a) for r, a in zip(ray_sets, any_flags)]
     hits = np.empty(max(len(r) for r, _ in sets), oracle.HIT_DTYPE)
-    # whole passes over the sample until the time budget is used
-    while elapsed < budget_s and passes < 100:
+
+    def one_pass():
+        bad = 0
+        t = time.perf_counter()
         for rays, any_hit in sets:
-            t0 = time.perf_counter()
-            L.oracle_trace_mt(C.byref(od), C.c_void_p(rays.ctypes.data), len(rays), int(any_hit),
-                              C.c_void_p(hits.ctypes.data), threads)
-            elapsed += time.perf_counter() - t0
-            done += len(rays)
-        passes += 1
-    return {"value": done / elapsed / 1e6 if elapsed > 0 else None, "unit": "Mrays/s", "cores": threads,
-            "kind": "port",
-            "sample": "%d passes over %s rays of the benchmarked queues (%d rays), traced by oracle/trace.c "
-                      "(scalar, BVH2) on %d host threads in %.1f s" % (
-                          passes, "+".join(str(len(r)) for r, _ in sets), done, threads, elapsed)}
+            bad += L.cpu4_trace_mt(C.byref(od), C.c_void_p(rays.ctypes.data), len(rays), int(any_hit),
+                                   C.c_void_p(hits.ctypes.data), threads)
+        if bad:
+            raise RuntimeError("CPU traversal met a malformed structure")
+        return time.perf_counter() - t
+
+    one_pass()
+    n_pass = sum(len(r) for r, _ in sets)
+    times = []
+    while sum(times) < budget_s and len(times) < 100:
+        times.append(one_pass())
+    elapsed = sum(times)
+    return {"value": n_pass * len(times) / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": "%d timed passes (after 1 warm-up) over %s rays of the benchmarked queues, %s; "
+                      "oracle/cpu_bvh4.c (the reference CPU layout: BVH4 + 4-triangle packets, 4-wide SSE/FMA) "
+                      "on %d host threads, %.1f s; median pass %.1f Mrays/s; CPU layout built in %.1f s" % (
+                          len(times), "+".join(str(len(r)) for r, _ in sets),
+                          "/".join("occlusion" if a else "closest" for _, a in sets), threads, elapsed,
+                          n_pass / float(np.median(times)) / 1e6, t_layout)}
 
 
 def run_pt(args):
@@ -88,58 +121,88 @@ def run_pt(args):
     stat_bytes = [ray_bytes(v, n) for v, n in zip(visits, counts)]
     log("rays per bounce %s" % counts)
 
+    # eager frames (host reads each bounce's queue size): traversal kernel times for the roofline
     for _ in range(args.warmup):
         pt.frame()
     pt.stream.sync()
-    frame_ms, trace_ms, rays_total, bytes_total = [], 0.0, 0, 0.0
-    t_start = time.perf_counter()
+    eager_ms, trace_ms, rays_eager, bytes_total = [], 0.0, 0, 0.0
+    t_eager = time.perf_counter()
     for _ in range(args.steps):
         f0 = time.perf_counter()
         c, _, _ = pt.frame(timed=True)
         pt.stream.sync()
-        frame_ms.append((time.perf_counter() - f0) * 1e3)
+        eager_ms.append((time.perf_counter() - f0) * 1e3)
         if c != counts:
             raise RuntimeError("frame ray counts changed between frames: %s vs %s" % (c, counts))
         for (ms, n), b in zip(pt.trace_times_ms(), stat_bytes):
             trace_ms += ms
             bytes_total += b
+        rays_eager += sum(c)
+    wall_eager = time.perf_counter() - t_eager
+
+    # the measured frames
+    if args.mode == "graph":
+        pt.build_graph(use_batch=args.trace_kernel == "batch")
+        for _ in range(args.warmup):
+            pt.frame_graph()
+        frame = pt.frame_graph
+    else:
+        def frame():
+            c, _, _ = pt.frame()
+            pt.stream.sync()
+            return c
+    frame_ms, rays_total = [], 0
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        f0 = time.perf_counter()
+        c = frame()
+        frame_ms.append((time.perf_counter() - f0) * 1e3)
+        if c != counts:
+            raise RuntimeError("frame ray counts changed between frames: %s vs %s" % (c, counts))
         rays_total += sum(c)
     wall = time.perf_counter() - t_start
     if pt.gpu_scene.error_count():
         raise RuntimeError("traversal reported a malformed scene")
     achieved = bytes_total / (trace_ms * 1e-3) / 1e9
+    workload = "pt-%dx%d-1spp-depth8-rr3-sphere-grid-%dtri-sbvh" % (args.width, args.height, scene.tri_count)
+    traffic, traffic_src = measured_traffic(workload, "k_trace_closest") if cfg is None else (None, None)
     res = {
         "metric": METRIC, "value": rays_total / wall / 1e6, "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded procedural scene and camera)",
-        "config": {"workload": "pt-1280x720-1spp-depth8-rr3-sphere-grid-%dtri-sbvh" % scene.tri_count,
+        "config": {"workload": workload,
                    "width": args.width, "height": args.height, "spp": 1, "max_path_len": 8,
                    "rays_per_frame": int(sum(counts)), "rays_per_bounce": counts,
                    "triangles": scene.tri_count, "nodes": scene.desc.node_count,
+                   "frame": "one graph launch per frame (queue sizes on the device)" if args.mode == "graph"
+                            else "eager launches, host reads each bounce's queue size",
+                   "trace_kernel": args.trace_kernel,
                    "trace_shape": [args.block, args.stack] if args.block else "default"},
-        "trace_mrays_s": rays_total / (trace_ms * 1e-3) / 1e6,
+        "eager_frames": {"value": rays_eager / wall_eager / 1e6, "ms_per_step": wall_eager * 1e3 / args.steps,
+                         "frame_latency_ms": {"p50": float(np.percentile(eager_ms, 50)),
+                                              "p99": float(np.percentile(eager_ms, 99))}},
+        "trace_mrays_s": rays_eager / (trace_ms * 1e-3) / 1e6,
         "frame_latency_ms": {"p50": float(np.percentile(frame_ms, 50)), "p99": float(np.percentile(frame_ms, 99))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_trace (closest hit)",
-                     "bytes_per_ray": bytes_total / max(rays_total, 1),
+                     "algorithmic_bytes_per_launch": sum(stat_bytes) / len(stat_bytes),
+                     "bytes_per_ray": bytes_total / max(rays_eager, 1),
                      "visits_per_ray": [float(x) / sum(counts) for x in np.sum(visits, 0)]},
     }
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(scene.desc, queues[:2], [False, False], budget_s=args.cpu_seconds)
+        res["cpu_baseline"] = cpu_baseline(scene.build, N.BUILDER_SBVH, threads, queues[:2], [False, False],
+                                           budget_s=args.cpu_seconds)
     return res
 
 
 def run_c1(args):
     from imbatracer_amd import native as N
     from imbatracer_amd import scene as S
-    from tests import scenes as T
+    from imbatracer_amd import workloads as T
 
-    v, i = S.random_triangles(65536, 0x1234)
-    b = S.SceneBuild()
-    b.add_mesh(v, i)
-    b.run(N.BUILDER_BINNED_SAH, 1)
+    b, _, _ = T.random_tri_scene(65536, N.BUILDER_BINNED_SAH)
     desc = b.desc()
     sc = S.Scene(desc)
     prim = T.primary_rays(1024, 1024)
@@ -175,19 +238,114 @@ def run_c1(args):
     wall = time.perf_counter() - t0
     t_closest = sum(a.elapsed_ms(b) for a, b, c in ev)
     achieved = ray_bytes(vis_p, n) * args.steps / (t_closest * 1e-3) / 1e9
+    workload = "c1-65536-random-tris-binned-sah-1M-primary-1M-occlusion"
+    traffic, traffic_src = measured_traffic(workload, "k_trace_closest") if cfg is None else (None, None)
     res = {
         "metric": METRIC, "value": 2 * n * args.steps / wall / 1e6, "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded random triangles, pinhole camera, point light)",
-        "config": {"workload": "c1-65536-random-tris-binned-sah-1M-primary-1M-occlusion"},
+        "config": {"workload": workload},
         "closest_mrays_s": n * args.steps / (t_closest * 1e-3) / 1e6,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_trace (closest hit)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": "k_trace (closest hit)", "algorithmic_bytes_per_launch": ray_bytes(vis_p, n),
                      "bytes_per_ray": ray_bytes(vis_p, n) / n},
     }
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(desc, [prim, shadow], [False, True], budget_s=args.cpu_seconds)
+        res["cpu_baseline"] = cpu_baseline(b, N.BUILDER_BINNED_SAH, 1, [prim, shadow], [False, True],
+                                           budget_s=args.cpu_seconds)
+    return res
+
+
+def run_c4(args):
+    """256 ray queues over 8 scenes (~40M incoherent closest-hit rays per step): one batched
+    launch per step against one launch per queue."""
+    from imbatracer_amd import native as N
+    from imbatracer_amd import scene as S
+    from imbatracer_amd import workloads as W
+
+    sizes = [65536, 131072, 262144, 524288, 65536, 131072, 262144, 524288]
+    builds, scenes = [], []
+    t0 = time.time()
+    for k, n in enumerate(sizes):
+        b, _, _ = W.random_tri_scene(n, N.BUILDER_SBVH if k % 2 else N.BUILDER_BINNED_SAH, seed=0x1234 + k)
+        builds.append(b)
+        scenes.append(S.Scene(b.desc()))
+    log("8 scenes built in %.1f s" % (time.time() - t0))
+    n_queues, per_queue = 256, args.c4_rays // 256
+    rng = np.random.default_rng(2024)
+    queues = []
+    for q in range(n_queues):
+        o = rng.uniform(-0.2, 1.2, (per_queue, 3)).astype(np.float32)
+        d = rng.normal(size=(per_queue, 3)).astype(np.float32)
+        d /= np.linalg.norm(d, axis=1, keepdims=True).astype(np.float32)
+        rays = S.make_rays(o, d, np.float32(0.0), np.float32(3.4028235e38))
+        queues.append((scenes[q % 8], N.DeviceBuffer.from_array(rays), N.DeviceBuffer(per_queue * 16), per_queue,
+                       None, False))
+        if q < 2:
+            queues[-1] += (rays,)
+    log("%d queues x %d rays uploaded" % (n_queues, per_queue))
+    # visit counts per queue (untimed)
+    cb = N.DeviceBuffer(per_queue * 12)
+    bytes_step = 0.0
+    for sc, rb, hb, n, _, _ in [q[:6] for q in queues]:
+        sc.intersect_stats(rb, hb, cb, n)
+        bytes_step += ray_bytes(cb.download(np.uint32, 3 * n).reshape(-1, 3).astype(np.int64).sum(0), n)
+    cfg = N.TraceConfig(args.block, args.stack, 0, args.blocks_per_cu) if (args.block or args.blocks_per_cu) else None
+    batch = S.TraceBatch([q[:6] for q in queues])
+    st = N.Stream()
+
+    def per_queue_launches():
+        for sc, rb, hb, n, _, _ in [q[:6] for q in queues]:
+            sc.intersect(rb, hb, n, cfg, st.handle)
+
+    # parity: the batch's hits equal the per-queue launches' hits
+    per_queue_launches()
+    st.sync()
+    ref = [q[2].download(N.HIT_DTYPE, q[3]) for q in queues]
+    batch.launch(cfg, st)
+    st.sync()
+    for q, r in zip(queues, ref):
+        if not np.array_equal(q[2].download(N.HIT_DTYPE, q[3]).view(np.int32), r.view(np.int32)):
+            raise RuntimeError("batched hits differ from per-queue hits")
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        st.sync()
+        ev = []
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            e0, e1 = N.Event(), N.Event()
+            e0.record(st)
+            fn()
+            e1.record(st)
+            ev.append((e0, e1))
+        st.sync()
+        return time.perf_counter() - t, sum(a.elapsed_ms(b) for a, b in ev)
+
+    wall_b, gpu_b = timed(lambda: batch.launch(cfg, st))
+    wall_q, gpu_q = timed(per_queue_launches)
+    rays_step = n_queues * per_queue
+    achieved = bytes_step * args.steps / (gpu_b * 1e-3) / 1e9
+    res = {
+        "metric": METRIC, "value": rays_step * args.steps / wall_b / 1e6, "unit": "Mrays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall_b * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded random triangles and incoherent rays)",
+        "config": {"workload": "c4-256-queues-%d-rays-8-scenes" % rays_step, "queues": n_queues,
+                   "rays_per_queue": per_queue, "scene_triangles": sizes,
+                   "trace": "one batched launch per step (imba_trace_batch_launch)"},
+        "per_queue_launches": {"value": rays_step * args.steps / wall_q / 1e6, "ms_per_step": wall_q * 1e3 / args.steps,
+                               "gpu_ms_per_step": gpu_q / args.steps},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_trace_batch (closest hit)",
+                     "algorithmic_bytes_per_launch": bytes_step, "bytes_per_ray": bytes_step / rays_step},
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(builds[0], N.BUILDER_BINNED_SAH, 1, [queues[0][6]], [False],
+                                           budget_s=args.cpu_seconds)
     return res
 
 
@@ -196,7 +354,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pt", choices=["pt", "c1"])
+    ap.add_argument("--workload", default="pt", choices=["pt", "c1", "c4"])
+    ap.add_argument("--c4-rays", type=int, default=40_000_000, help="c4: rays over all 256 queues")
     ap.add_argument("--width", type=int, default=1280)
     ap.add_argument("--height", type=int, default=720)
     ap.add_argument("--side", type=int, default=32, help="spheres per side of the grid")
@@ -204,6 +363,10 @@ def main():
     ap.add_argument("--stack", type=int, default=0)
     ap.add_argument("--persistent", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--mode", default="graph", choices=["graph", "eager"],
+                    help="pt: frames as one captured graph (default) or as eager launches")
+    ap.add_argument("--trace-kernel", default="plain", choices=["plain", "batch"],
+                    help="pt graph frames: k_trace with a device count, or the batched queue kernel")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -224,7 +387,7 @@ def main():
         N.check(N.lib.imba_memcpy_d2d(b.ptr, a.ptr, nbytes, None), "copy")
         print(json.dumps({"calibrate_copy_bytes": nbytes}))
         return
-    res = run_pt(args) if args.workload == "pt" else run_c1(args)
+    res = {"pt": run_pt, "c1": run_c1, "c4": run_c4}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

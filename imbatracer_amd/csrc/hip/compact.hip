@@ -13,6 +13,8 @@
 // Traffic per entry: key read twice (4 B), record read and written once (32 + 16 + S B).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "common.h"
 
 namespace imba {
@@ -30,12 +32,19 @@ struct CompactArgs {
     const int4* hits_in;
     const int* states_in;
     int stride_dw;  // state record size in dwords
-    int n;
+    int n;               // entries (or, with count_dev, the capacity)
+    const int* count_dev;  // device entry count read when the launch runs, or null
     float4* rays_out;
     int4* hits_out;
     int* states_out;
     int* block_offsets;  // nblocks exclusive offsets, then the total at [nblocks]
+    int* count_out;      // device copy of the total, or null
 };
+
+// Entries of this launch: n, or the device count clamped to [0, n].
+__device__ __forceinline__ int entries(const CompactArgs& a) {
+    return a.count_dev ? min(max(*a.count_dev, 0), a.n) : a.n;
+}
 
 template <int MODE>
 __device__ __forceinline__ bool kept(const CompactArgs& a, int i) {
@@ -51,11 +60,12 @@ template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_count(CompactArgs a) {
     __shared__ int wave_sum[kWaves];
     const int base = blockIdx.x * kTile;
+    const int n = entries(a);
     int cnt = 0;
 #pragma unroll
     for (int it = 0; it < kItems; it++) {
         const int i = base + it * kBlock + (int)threadIdx.x;
-        const bool f = i < a.n && kept<MODE>(a, i);
+        const bool f = i < n && kept<MODE>(a, i);
         cnt += __popcll(__ballot(f));
     }
     // every lane of a wave holds the same count
@@ -68,8 +78,9 @@ __global__ __launch_bounds__(kBlock) void k_count(CompactArgs a) {
     }
 }
 
-// Exclusive scan of nb block counts in place; the total goes to block_offsets[nb].
-__global__ __launch_bounds__(1024) void k_scan_offsets(int* offs, int nb) {
+// Exclusive scan of nb block counts in place; the total goes to block_offsets[nb] (and to
+// *count_out when given).
+__global__ __launch_bounds__(1024) void k_scan_offsets(int* offs, int nb, int* count_out) {
     __shared__ int part[1024];
     const int t = threadIdx.x;
     const int per = (nb + 1023) / 1024;
@@ -91,7 +102,10 @@ __global__ __launch_bounds__(1024) void k_scan_offsets(int* offs, int nb) {
         offs[b] = run;
         run += c;
     }
-    if (t == 1023) offs[nb] = part[1023];
+    if (t == 1023) {
+        offs[nb] = part[1023];
+        if (count_out) *count_out = part[1023];
+    }
 }
 
 template <int MODE>
@@ -112,6 +126,7 @@ template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_scatter(CompactArgs a, int nblocks) {
     __shared__ int wave_cnt[kWaves];
     const int base = blockIdx.x * kTile;
+    const int n = entries(a);
     const int total = a.block_offsets[nblocks];
     int kept_before = a.block_offsets[blockIdx.x];  // kept entries before this chunk
     int removed_before = base - kept_before;         // removed entries before this chunk
@@ -120,7 +135,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(CompactArgs a, int nblocks) 
     for (int it = 0; it < kItems; it++) {
         const int chunk = base + it * kBlock;
         const int i = chunk + (int)threadIdx.x;
-        const bool valid = i < a.n;
+        const bool valid = i < n;
         const bool f = valid && kept<MODE>(a, i);
         const unsigned long long m = __ballot(f);
         const int r = lane_rank(m);
@@ -140,7 +155,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(CompactArgs a, int nblocks) 
             const int removed_in_chunk_before = (int)threadIdx.x - kept_in_chunk_before;
             move_record<MODE>(a, i, total + removed_before + removed_in_chunk_before);
         }
-        const int chunk_valid = max(0, min(kBlock, a.n - chunk));
+        const int chunk_valid = max(0, min(kBlock, n - chunk));
         kept_before += chunk_kept;
         removed_before += chunk_valid - chunk_kept;
         __syncthreads();
@@ -169,7 +184,7 @@ int run_compact(CompactArgs a, int* new_count, void* stream) {
     bool sync;
     hipStream_t s = resolve_stream(stream, &sync);
     hipLaunchKernelGGL(k_count<MODE>, dim3(nb), dim3(kBlock), 0, s, a);
-    hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, a.block_offsets, nb);
+    hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, a.block_offsets, nb, (int*)nullptr);
     hipLaunchKernelGGL(k_scatter<MODE>, dim3(nb), dim3(kBlock), 0, s, a, nb);
     IMBA_HIP_TRY(hipGetLastError());
     if (new_count) {
@@ -191,6 +206,38 @@ int run_compact(CompactArgs a, int* new_count, void* stream) {
 using namespace imba;
 
 extern "C" {
+
+size_t imba_compact_workspace_size(int capacity) {
+    return sizeof(int) * ((size_t)std::max(1, ceil_div(capacity > 0 ? capacity : 0, kTile)) + 1);
+}
+
+int imba_compact_rays_dev(const ImbaRay* rays_in, const void* states_in, int state_stride, int capacity,
+                          const int* count_dev, ImbaRay* rays_out, void* states_out, int* new_count_dev,
+                          void* workspace, void* stream) {
+    IMBA_CHECK_ARG(state_stride > 0 && state_stride % 4 == 0, "state_stride must be a positive multiple of 4");
+    IMBA_CHECK_ARG(capacity >= 0, "capacity < 0");
+    IMBA_CHECK_ARG(count_dev && new_count_dev && workspace, "count_dev, new_count_dev and workspace are required");
+    IMBA_CHECK_ARG(capacity == 0 || (rays_in && states_in && rays_out && states_out), "NULL buffer");
+    CompactArgs a{};
+    a.rays_in = (const float4*)rays_in;
+    a.states_in = (const int*)states_in;
+    a.stride_dw = state_stride / 4;
+    a.n = capacity;
+    a.count_dev = count_dev;
+    a.rays_out = (float4*)rays_out;
+    a.states_out = (int*)states_out;
+    a.block_offsets = (int*)workspace;
+    a.count_out = new_count_dev;
+    const int nb = std::max(1, ceil_div(capacity, kTile));
+    bool sync;
+    hipStream_t s = resolve_stream(stream, &sync);
+    hipLaunchKernelGGL(k_count<kModeRays>, dim3(nb), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, a.block_offsets, nb, new_count_dev);
+    hipLaunchKernelGGL(k_scatter<kModeRays>, dim3(nb), dim3(kBlock), 0, s, a, nb);
+    IMBA_HIP_TRY(hipGetLastError());
+    if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
+    return IMBA_OK;
+}
 
 int imba_compact_hits(const ImbaRay* rays_in, const ImbaHit* hits_in, const void* states_in, int state_stride, int count,
                       ImbaRay* rays_out, ImbaHit* hits_out, void* states_out, int* new_count, void* stream) {

@@ -87,7 +87,8 @@ struct BounceArgs {
     const int4* hits;
     PtState* states;
     float4* rays_out;  // may alias rays
-    int count;
+    int count;         // entries (or, with count_dev, the capacity)
+    const int* count_dev;  // device entry count read when the launch runs, or null
     const float* tri_normals;  // 3 per global triangle
     const int* tri_material;
     int n_tris;
@@ -102,7 +103,8 @@ struct BounceArgs {
 
 __global__ __launch_bounds__(256) void k_bounce(BounceArgs a) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= a.count) return;
+    const int n = a.count_dev ? min(max(*a.count_dev, 0), a.count) : a.count;
+    if (i >= n) return;
     PtState s = a.states[i];
     const int4 h = a.hits[i];
     if (h.x < 0 || h.x >= a.n_tris) {
@@ -208,10 +210,10 @@ int imba_pt_camera(const float* raster_to_world16, const float* cam_pos3, int wi
     return IMBA_OK;
 }
 
-int imba_pt_bounce(const ImbaRay* rays, const ImbaHit* hits, void* states, int count, const float* tri_normals,
-                   const int* tri_materials, int tri_count, const float* albedo, const float* emission,
-                   int material_count, int max_path_len, int rr_start, float offset, float* image, ImbaRay* rays_out,
-                   void* stream) {
+static int pt_bounce(const ImbaRay* rays, const ImbaHit* hits, void* states, int count, const int* count_dev,
+                     const float* tri_normals, const int* tri_materials, int tri_count, const float* albedo,
+                     const float* emission, int material_count, int max_path_len, int rr_start, float offset,
+                     float* image, ImbaRay* rays_out, void* stream) {
     IMBA_CHECK_ARG(count >= 0, "count < 0");
     if (count == 0) return IMBA_OK;
     IMBA_CHECK_ARG(rays && hits && states && tri_normals && tri_materials && albedo && emission && rays_out,
@@ -222,6 +224,7 @@ int imba_pt_bounce(const ImbaRay* rays, const ImbaHit* hits, void* states, int c
     a.states = (PtState*)states;
     a.rays_out = (float4*)rays_out;
     a.count = count;
+    a.count_dev = count_dev;
     a.tri_normals = tri_normals;
     a.tri_material = tri_materials;
     a.n_tris = tri_count;
@@ -238,6 +241,23 @@ int imba_pt_bounce(const ImbaRay* rays, const ImbaHit* hits, void* states, int c
     IMBA_HIP_TRY(hipGetLastError());
     if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
     return IMBA_OK;
+}
+
+int imba_pt_bounce(const ImbaRay* rays, const ImbaHit* hits, void* states, int count, const float* tri_normals,
+                   const int* tri_materials, int tri_count, const float* albedo, const float* emission,
+                   int material_count, int max_path_len, int rr_start, float offset, float* image, ImbaRay* rays_out,
+                   void* stream) {
+    return pt_bounce(rays, hits, states, count, nullptr, tri_normals, tri_materials, tri_count, albedo, emission,
+                     material_count, max_path_len, rr_start, offset, image, rays_out, stream);
+}
+
+int imba_pt_bounce_dev(const ImbaRay* rays, const ImbaHit* hits, void* states, int capacity, const int* count_dev,
+                       const float* tri_normals, const int* tri_materials, int tri_count, const float* albedo,
+                       const float* emission, int material_count, int max_path_len, int rr_start, float offset,
+                       float* image, ImbaRay* rays_out, void* stream) {
+    IMBA_CHECK_ARG(count_dev, "count_dev is NULL");
+    return pt_bounce(rays, hits, states, capacity, count_dev, tri_normals, tri_materials, tri_count, albedo, emission,
+                     material_count, max_path_len, rr_start, offset, image, rays_out, stream);
 }
 
 }  // extern "C"

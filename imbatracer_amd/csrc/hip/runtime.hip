@@ -177,6 +177,11 @@ int validate_scene(const ImbaSceneDesc& d, SceneInfo& info) {
 
 using namespace imba;
 
+struct ImbaGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
 extern "C" {
 
 int imba_device_count(int* count) {
@@ -371,6 +376,50 @@ int imba_scene_footprint(const ImbaScene* s, size_t* bytes) {
 int imba_trace_error_count(const ImbaScene* s, int* count) {
     IMBA_CHECK_ARG(s && count, "NULL argument");
     IMBA_HIP_TRY(hipMemcpy(count, s->d_error, sizeof(int), hipMemcpyDeviceToHost));
+    return IMBA_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// Graphs: stream capture of the library's asynchronous calls.
+int imba_graph_begin(void* stream) {
+    IMBA_CHECK_ARG(stream, "capture needs a created stream, not NULL");
+    IMBA_HIP_TRY(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    return IMBA_OK;
+}
+
+int imba_graph_end(void* stream, ImbaGraph** out) {
+    IMBA_CHECK_ARG(stream && out, "NULL argument");
+    *out = nullptr;
+    hipGraph_t g = nullptr;
+    IMBA_HIP_TRY(hipStreamEndCapture((hipStream_t)stream, &g));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(g);
+        return fail(IMBA_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    }
+    auto* r = new ImbaGraph();
+    r->graph = g;
+    r->exec = exec;
+    *out = r;
+    return IMBA_OK;
+}
+
+int imba_graph_launch(ImbaGraph* g, void* stream) {
+    IMBA_CHECK_ARG(g, "graph is NULL");
+    bool sync;
+    hipStream_t s = resolve_stream(stream, &sync);
+    IMBA_HIP_TRY(hipGraphLaunch(g->exec, s));
+    if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
+    return IMBA_OK;
+}
+
+int imba_graph_release(ImbaGraph* g) {
+    if (!g) return IMBA_OK;
+    IMBA_HIP_TRY(hipDeviceSynchronize());
+    hipGraphExecDestroy(g->exec);
+    hipGraphDestroy(g->graph);
+    delete g;
     return IMBA_OK;
 }
 

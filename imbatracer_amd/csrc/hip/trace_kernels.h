@@ -44,7 +44,13 @@ struct TraceArgs {
     int* spill;           // global stack entries beyond the LDS part
     int spill_depth;      // spill entries per thread
     int* work_counter;    // persistent variant: next ray to fetch
+    const int* count_dev; // device ray count read when the launch runs (<= count), or null
 };
+
+// Rays of this launch: count, or the device count clamped to [0, count].
+__device__ __forceinline__ int launch_count(const TraceArgs& a) {
+    return a.count_dev ? min(max(*a.count_dev, 0), a.count) : a.count;
+}
 
 constexpr int kNoNode = INT32_MIN;
 constexpr int kDummyChild = 0x76543210;
@@ -366,7 +372,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     LaneStack<BLOCK, STACK_LDS> st;
     stack_init(st, lds_stack, a);
     const int stride = gridDim.x * BLOCK;
-    for (int id = blockIdx.x * BLOCK + threadIdx.x; id < a.count; id += stride) {
+    const int count = launch_count(a);
+    for (int id = blockIdx.x * BLOCK + threadIdx.x; id < count; id += stride) {
         TraceState s;
         state_init(s, a.rays[2 * (size_t)id], a.rays[2 * (size_t)id + 1], a.root);
         st.sp = 0;
@@ -390,6 +397,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dyn(TraceArgs a, int refill) {
     int* counter = a.work_counter;
     int id = -1;             // ray of this lane, -1 = none
     bool exhausted = false;  // wave-uniform: no rays left to fetch
+    const int count = launch_count(a);
     TraceState s;
     for (;;) {
         const unsigned long long idle = __ballot(id < 0);
@@ -399,11 +407,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dyn(TraceArgs a, int refill) {
             int base = 0;
             if (lane == leader) base = atomicAdd(&counter[0], n);
             base = __shfl(base, leader, 64);
-            if (base + n >= a.count) exhausted = true;
+            if (base + n >= count) exhausted = true;
             if (id < 0) {
                 const int r = base + __builtin_amdgcn_mbcnt_hi((unsigned)(idle >> 32),
                                                                __builtin_amdgcn_mbcnt_lo((unsigned)idle, 0u));
-                if (r < a.count) {
+                if (r < count) {
                     id = r;
                     state_init(s, a.rays[2 * (size_t)r], a.rays[2 * (size_t)r + 1], a.root);
                     st.sp = 0;
@@ -420,6 +428,90 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dyn(TraceArgs a, int refill) {
         }
     }
     // the last wave out resets the counters for the launch that reuses this slot
+    if (lane == 0) {
+        const int waves = gridDim.x * (BLOCK / 64);
+        if (atomicAdd(&counter[1], 1) == waves - 1) {
+            atomicExch(&counter[0], 0);
+            atomicExch(&counter[1], 0);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Batched traversal: many ray queues, each with its own scene, in one launch per kernel
+// variant.  The queues of a launch are cut into chunks of 64 consecutive rays; every wave
+// takes the next chunk from a global counter, finds its queue by binary search over the
+// queues' first chunks (wave-uniform, so the queue's arguments sit in scalar registers) and
+// traces one ray per lane with the same per-ray steps as k_trace, so every queue's hits are
+// those of a launch of its own.
+struct BatchItem {
+    TraceArgs a;           // a.count: rays of this launch (set by k_batch_prefix)
+    const int* count_dev;  // device ray count read when the launch runs, or null
+    int capacity;          // rays (count_dev null) or the upper bound of *count_dev
+    int chunk_begin;       // first chunk of this queue within its launch group
+    int group;             // launch group: 2 * any_hit + masked
+};
+
+struct BatchGroups {
+    int first[5];  // items of group g: [first[g], first[g + 1])
+};
+
+// One thread: ray counts of this launch and each queue's first chunk; chunks per group.
+__global__ void k_batch_prefix(BatchItem* items, BatchGroups groups, int* group_chunks) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int g = 0; g < 4; g++) {
+        int c = 0;
+        for (int i = groups.first[g]; i < groups.first[g + 1]; i++) {
+            int n = items[i].capacity;
+            if (items[i].count_dev) n = min(max(*items[i].count_dev, 0), n);
+            items[i].a.count = n;
+            items[i].chunk_begin = c;
+            c += (n + 63) >> 6;
+        }
+        group_chunks[g] = c;
+    }
+}
+
+template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED>
+__global__ __launch_bounds__(BLOCK, (ANY || MASKED) ? 1 : 8) void k_trace_batch(const BatchItem* __restrict__ items, int first, int n_items,
+                                                       const int* __restrict__ total_chunks, int* counter, int* spill,
+                                                       int spill_depth) {
+    __shared__ int lds_stack[STACK_LDS * BLOCK];
+    LaneStack<BLOCK, STACK_LDS> st;
+    st.lds = lds_stack;
+    st.spill = spill;
+    st.spill_depth = spill_depth;
+    st.lane_slot = threadIdx.x;
+    st.spill_slot = blockIdx.x * BLOCK + threadIdx.x;
+    st.spill_stride = gridDim.x * BLOCK;
+    st.sp = 0;
+    const int lane = __lane_id();
+    const int total = __builtin_amdgcn_readfirstlane(*total_chunks);
+    for (;;) {
+        int c = 0;
+        if (lane == 0) c = atomicAdd(&counter[0], 1);
+        c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, 64));
+        if (c >= total) break;
+        // the queue owning chunk c: the last one whose first chunk is <= c (queues without
+        // rays share their first chunk with the next queue and are skipped)
+        int lo = first, hi = first + n_items - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (__builtin_amdgcn_readfirstlane(items[mid].chunk_begin) <= c) lo = mid;
+            else hi = mid - 1;
+        }
+        lo = __builtin_amdgcn_readfirstlane(lo);
+        const TraceArgs a = items[lo].a;
+        const int id = (c - items[lo].chunk_begin) * 64 + lane;
+        if (id < a.count) {
+            TraceState s;
+            state_init(s, a.rays[2 * (size_t)id], a.rays[2 * (size_t)id + 1], a.root);
+            st.sp = 0;
+            while (!advance<BLOCK, STACK_LDS, ANY, MASKED, false>(a, s, st)) {
+            }
+            write_result<false>(a, s, id);
+        }
+    }
     if (lane == 0) {
         const int waves = gridDim.x * (BLOCK / 64);
         if (atomicAdd(&counter[1], 1) == waves - 1) {

@@ -42,6 +42,16 @@ int counter_slot(int dev, int** slot) {
     return IMBA_OK;
 }
 
+// Allocation is not allowed while `s` is being captured into a graph.
+int not_capturing(hipStream_t s, const char* what) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    IMBA_HIP_TRY(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+        return fail(IMBA_ERR_INVALID_ARG, "graph capture: this launch needs %s; make the same call once "
+                                          "on this stream before capturing", what);
+    return IMBA_OK;
+}
+
 template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS>
 int launch_one(const TraceArgs& args_in, int max_stack, int blocks_per_cu, int refill, hipStream_t s) {
     TraceArgs a = args_in;
@@ -65,12 +75,21 @@ int launch_one(const TraceArgs& args_in, int max_stack, int blocks_per_cu, int r
         a.spill_depth = max_stack - STACK_LDS;
         std::lock_guard<std::mutex> lock(g_scratch_mutex);
         Scratch& sc = g_spill[std::make_pair(p->device, s)];
+        const size_t need = (size_t)a.spill_depth * grid * BLOCK * sizeof(int);
+        if (sc.bytes < need) {
+            int r = not_capturing(s, "a larger spill stack");
+            if (r) return r;
+        }
         // growing frees the old buffer after a device-wide synchronisation (grow_buffer)
-        int r = grow_buffer(sc, (size_t)a.spill_depth * grid * BLOCK * sizeof(int));
+        int r = grow_buffer(sc, need);
         if (r) return r;
         a.spill = (int*)sc.ptr;
     }
     if (dyn) {
+        if (!g_counters[p->device & 63]) {
+            int r = not_capturing(s, "the persistent kernel's work counters");
+            if (r) return r;
+        }
         int r = counter_slot(p->device & 63, &a.work_counter);
         if (r) return r;
         hipLaunchKernelGGL(kern_dyn, dim3(grid), dim3(BLOCK), 0, s, a, std::min(refill, 64));
@@ -111,11 +130,13 @@ int dispatch_shape(const TraceArgs& a, int max_stack, const ImbaTraceConfig* cfg
 }
 
 int launch_trace(const ImbaSceneDesc& d, const SceneInfo& info, int* d_err, const ImbaRay* rays, ImbaHit* hits,
-                 uint32_t* counts, int count, bool any, bool masked, const ImbaTraceConfig* cfg, void* stream) {
+                 uint32_t* counts, int count, bool any, bool masked, const ImbaTraceConfig* cfg, void* stream,
+                 const int* count_dev = nullptr) {
     if (count < 0) return fail(IMBA_ERR_INVALID_ARG, "count < 0");
     if (count == 0) return IMBA_OK;
     if (!rays || !hits) return fail(IMBA_ERR_INVALID_ARG, "rays or hits is NULL");
     TraceArgs a{};
+    a.count_dev = count_dev;
     a.nodes = (const float4*)d.nodes;       a.n_nodes = d.node_count;
     a.inst = (const float4*)d.instances;    a.n_inst = d.instance_count;
     a.tris = (const float4*)d.tris;         a.n_tris4 = d.tri_vec4_count;
@@ -264,12 +285,241 @@ void ffi_trace(bool any, int root, void* nodes, void* instances, void* tris, voi
                    imba_last_error());
 }
 
+TraceArgs scene_args(const ImbaSceneDesc& d, int* d_err) {
+    TraceArgs a{};
+    a.nodes = (const float4*)d.nodes;       a.n_nodes = d.node_count;
+    a.inst = (const float4*)d.instances;    a.n_inst = d.instance_count;
+    a.tris = (const float4*)d.tris;         a.n_tris4 = d.tri_vec4_count;
+    a.indices = (const int4*)d.indices;     a.n_tri_ids = d.index_count / 4;
+    a.texcoords = (const float2*)d.texcoords; a.n_texcoords = d.texcoord_count;
+    a.masks = (const int*)d.masks;          a.n_masks = d.mask_count;
+    a.mask_buf = d.mask_buffer;             a.mask_buf_size = d.mask_buffer_size;
+    a.root = d.root;
+    a.err = d_err;
+    return a;
+}
+
+}  // namespace
+}  // namespace imba
+
+// A batch of ray queues traversed together (imba_trace_batch_*): the queue table lives in
+// device memory, with per-group work counters and spill stacks of its own, so that a launch
+// allocates nothing and can be captured into a graph.
+struct ImbaTraceBatch {
+    int device = 0;
+    int n = 0;
+    imba::BatchGroups groups{};
+    int max_stack[4] = {0, 0, 0, 0};
+    long long capacity_chunks[4] = {0, 0, 0, 0};
+    imba::BatchItem* d_items = nullptr;
+    int* d_group_chunks = nullptr;  // 4
+    int* d_counters = nullptr;      // 2 per group
+    imba::Scratch spill[4];
+};
+
+namespace imba {
+namespace {
+
+template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED>
+int launch_batch_group(ImbaTraceBatch* b, int g, int blocks_per_cu, hipStream_t s) {
+    const DeviceProps* p = device_props();
+    if (!p) return fail(IMBA_ERR_NO_DEVICE, "no HIP device");
+    auto kern = k_trace_batch<BLOCK, STACK_LDS, ANY, MASKED>;
+    int resident = 0;
+    IMBA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern, BLOCK, 0));
+    if (resident <= 0) resident = 1;
+    if (blocks_per_cu > 0) resident = std::min(resident, blocks_per_cu);
+    const long long needed = (b->capacity_chunks[g] * 64 + BLOCK - 1) / BLOCK;
+    const int grid = (int)std::max<long long>(1, std::min<long long>(needed, (long long)resident * p->cu_count));
+    int* spill = nullptr;
+    int spill_depth = 0;
+    if (b->max_stack[g] > STACK_LDS) {
+        spill_depth = b->max_stack[g] - STACK_LDS;
+        const size_t need = (size_t)spill_depth * grid * BLOCK * sizeof(int);
+        if (b->spill[g].bytes < need) {
+            int r = not_capturing(s, "a larger spill stack");
+            if (r) return r;
+            r = grow_buffer(b->spill[g], need);
+            if (r) return r;
+        }
+        spill = (int*)b->spill[g].ptr;
+    }
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, s, b->d_items, b->groups.first[g],
+                       b->groups.first[g + 1] - b->groups.first[g], b->d_group_chunks + g, b->d_counters + 2 * g,
+                       spill, spill_depth);
+    IMBA_HIP_TRY(hipGetLastError());
+    return IMBA_OK;
+}
+
+template <bool ANY, bool MASKED>
+int dispatch_batch(ImbaTraceBatch* b, int g, const ImbaTraceConfig* cfg, hipStream_t s) {
+    const int block = cfg && cfg->block_size ? cfg->block_size : kDefaultBlock;
+    const int stack = cfg && cfg->stack_lds ? cfg->stack_lds : kDefaultStack;
+    const int bpc = cfg ? cfg->blocks_per_cu : 0;
+#define IMBA_SHAPE(B, S) \
+    if (block == B && stack == S) return launch_batch_group<B, S, ANY, MASKED>(b, g, bpc, s);
+    IMBA_SHAPE(256, 16)
+    IMBA_SHAPE(256, 8)
+    IMBA_SHAPE(256, 32)
+    IMBA_SHAPE(128, 16)
+    IMBA_SHAPE(128, 32)
+    IMBA_SHAPE(64, 32)
+    IMBA_SHAPE(64, 64)
+#undef IMBA_SHAPE
+    return fail(IMBA_ERR_INVALID_ARG, "unsupported trace shape block=%d stack_lds=%d "
+                "(supported: 256/16, 256/8, 256/32, 128/16, 128/32, 64/32, 64/64)", block, stack);
+}
+
+void free_batch(ImbaTraceBatch* b) {
+    if (!b) return;
+    hipFree(b->d_items);
+    hipFree(b->d_group_chunks);
+    hipFree(b->d_counters);
+    for (auto& sp : b->spill) hipFree(sp.ptr);
+    delete b;
+}
+
 }  // namespace
 }  // namespace imba
 
 using namespace imba;
 
 extern "C" {
+
+int imba_trace_batch_create(const ImbaTraceItem* items, int n_items, ImbaTraceBatch** out) {
+    IMBA_CHECK_ARG(out && (items || n_items == 0), "NULL argument");
+    IMBA_CHECK_ARG(n_items >= 0, "n_items < 0");
+    *out = nullptr;
+    std::vector<BatchItem> table;
+    table.reserve(n_items);
+    auto* b = new ImbaTraceBatch();
+    IMBA_HIP_TRY(hipGetDevice(&b->device));
+    // group-contiguous table, queues in the caller's order within a group
+    for (int g = 0; g < 4; g++) {
+        b->groups.first[g] = (int)table.size();
+        for (int i = 0; i < n_items; i++) {
+            const ImbaTraceItem& it = items[i];
+            if (!it.scene || it.count < 0 || (it.count > 0 && (!it.rays || !it.hits))) {
+                delete b;
+                return fail(IMBA_ERR_INVALID_ARG, "queue %d: scene NULL, count < 0 or NULL rays/hits", i);
+            }
+            if (it.scene->device != b->device) {
+                delete b;
+                return fail(IMBA_ERR_INVALID_ARG, "queue %d: scene lives on device %d, not the current device %d", i,
+                            it.scene->device, b->device);
+            }
+            const int masked = !it.scene->info.all_opaque;
+            if (2 * (it.any_hit != 0) + masked != g) continue;
+            BatchItem e{};
+            e.a = scene_args(it.scene->dev, it.scene->d_error);
+            e.a.rays = (const float4*)it.rays;
+            e.a.hits = (int4*)it.hits;
+            e.a.count = it.count;
+            e.count_dev = it.count_dev;
+            e.capacity = it.count;
+            e.group = g;
+            b->max_stack[g] = std::max(b->max_stack[g], it.scene->info.max_stack);
+            b->capacity_chunks[g] += ceil_div(it.count, 64);
+            table.push_back(e);
+        }
+    }
+    b->groups.first[4] = (int)table.size();
+    b->n = (int)table.size();
+    hipError_t e = hipSuccess;
+    if (!table.empty()) e = hipMalloc((void**)&b->d_items, sizeof(BatchItem) * table.size());
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_group_chunks, sizeof(int) * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_counters, sizeof(int) * 8);
+    if (e == hipSuccess && !table.empty())
+        e = hipMemcpy(b->d_items, table.data(), sizeof(BatchItem) * table.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(b->d_counters, 0, sizeof(int) * 8);
+    if (e == hipSuccess) e = hipMemset(b->d_group_chunks, 0, sizeof(int) * 4);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        free_batch(b);
+        return fail(e == hipErrorOutOfMemory ? IMBA_ERR_OUT_OF_MEMORY : IMBA_ERR_HIP, "batch upload failed: %s",
+                    hipGetErrorString(e));
+    }
+    *out = b;
+    return IMBA_OK;
+}
+
+int imba_trace_batch_launch(ImbaTraceBatch* b, const ImbaTraceConfig* cfg, void* stream) {
+    IMBA_CHECK_ARG(b, "batch is NULL");
+    int dev = 0;
+    IMBA_HIP_TRY(hipGetDevice(&dev));
+    if (dev != b->device) return fail(IMBA_ERR_INVALID_ARG, "batch belongs to device %d, current is %d", b->device, dev);
+    if (b->n == 0) return IMBA_OK;
+    bool sync;
+    hipStream_t s = resolve_stream(stream, &sync);
+    hipLaunchKernelGGL(k_batch_prefix, dim3(1), dim3(64), 0, s, b->d_items, b->groups, b->d_group_chunks);
+    IMBA_HIP_TRY(hipGetLastError());
+    for (int g = 0; g < 4; g++) {
+        if (b->groups.first[g + 1] == b->groups.first[g] || b->capacity_chunks[g] == 0) continue;
+        int r;
+        switch (g) {
+            case 0: r = dispatch_batch<false, false>(b, g, cfg, s); break;
+            case 1: r = dispatch_batch<false, true>(b, g, cfg, s); break;
+            case 2: r = dispatch_batch<true, false>(b, g, cfg, s); break;
+            default: r = dispatch_batch<true, true>(b, g, cfg, s); break;
+        }
+        if (r) return r;
+    }
+    if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
+    return IMBA_OK;
+}
+
+int imba_autotune(const ImbaScene* sc, const ImbaRay* rays, ImbaHit* hits, int count, int any_hit, int reps,
+                  ImbaTraceConfig* best, float* best_ms, void* stream) {
+    IMBA_CHECK_ARG(sc && rays && hits && best, "NULL argument");
+    IMBA_CHECK_ARG(count > 0 && reps > 0, "count and reps must be positive");
+    static const int kShapes[][2] = {{256, 16}, {256, 8}, {256, 32}, {128, 16}, {128, 32}, {64, 32}, {64, 64}};
+    static const int kRefill[] = {0, 32};
+    bool sync;
+    hipStream_t s = resolve_stream(stream, &sync);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    IMBA_HIP_TRY(hipEventCreate(&e0));
+    IMBA_HIP_TRY(hipEventCreate(&e1));
+    float best_t = -1.0f;
+    int r = IMBA_OK;
+    for (const auto& shape : kShapes) {
+        for (int refill : kRefill) {
+            ImbaTraceConfig cfg{shape[0], shape[1], refill, 0};
+            r = launch_trace(sc->dev, sc->info, sc->d_error, rays, hits, nullptr, count, any_hit != 0,
+                             !sc->info.all_opaque, &cfg, s);
+            if (r) break;
+            hipEventRecord(e0, s);
+            for (int k = 0; k < reps && !r; k++)
+                r = launch_trace(sc->dev, sc->info, sc->d_error, rays, hits, nullptr, count, any_hit != 0,
+                                 !sc->info.all_opaque, &cfg, s);
+            if (r) break;
+            hipEventRecord(e1, s);
+            float ms = 0.0f;
+            if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                r = fail(IMBA_ERR_HIP, "event timing failed");
+                break;
+            }
+            ms /= (float)reps;
+            if (best_t < 0.0f || ms < best_t) {
+                best_t = ms;
+                *best = cfg;
+            }
+        }
+        if (r) break;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (r) return r;
+    if (best_ms) *best_ms = best_t;
+    return IMBA_OK;
+}
+
+int imba_trace_batch_release(ImbaTraceBatch* b) {
+    if (b) {
+        hipDeviceSynchronize();
+        free_batch(b);
+    }
+    return IMBA_OK;
+}
 
 int imba_intersect(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int count, const ImbaTraceConfig* cfg,
                    void* stream) {
@@ -281,6 +531,20 @@ int imba_occluded(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int co
                   void* stream) {
     IMBA_CHECK_ARG(s, "scene is NULL");
     return launch_trace(s->dev, s->info, s->d_error, rays, hits, nullptr, count, true, !s->info.all_opaque, cfg, stream);
+}
+
+int imba_intersect_dev(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int capacity, const int* count_dev,
+                       const ImbaTraceConfig* cfg, void* stream) {
+    IMBA_CHECK_ARG(s && count_dev, "scene or count_dev is NULL");
+    return launch_trace(s->dev, s->info, s->d_error, rays, hits, nullptr, capacity, false, !s->info.all_opaque, cfg,
+                        stream, count_dev);
+}
+
+int imba_occluded_dev(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int capacity, const int* count_dev,
+                      const ImbaTraceConfig* cfg, void* stream) {
+    IMBA_CHECK_ARG(s && count_dev, "scene or count_dev is NULL");
+    return launch_trace(s->dev, s->info, s->d_error, rays, hits, nullptr, capacity, true, !s->info.all_opaque, cfg,
+                        stream, count_dev);
 }
 
 int imba_intersect_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts, int count,

@@ -64,6 +64,68 @@ void build_mesh_bvh(const MeshData& mesh, int builder, int tri_base, MeshBvh& ou
     out.single_leaf = w.single_leaf;
 }
 
+struct MeshBvh4 {
+    std::vector<ImbaNode4> nodes;
+    std::vector<ImbaVec4> tris;
+};
+
+void build_mesh_bvh4(const MeshData& mesh, int builder, int tri_base, MeshBvh4& out) {
+    const TriSource src = mesh.source();
+    Bvh4MeshWriter w(out.nodes, out.tris, src, tri_base);
+    auto on_node = [&](const BBox& p, int c, const BBox* b) { w.node(p, c, b); };
+    auto on_leaf = [&](const BBox& bb, int c, const int* ids) { w.leaf(bb, c, ids); };
+    if (builder == IMBA_BUILDER_SBVH) {
+        // CpuMeshAdapter: SplitBvhBuilder<4>, leaf threshold 2, alpha 1e-4 (cpu_adapter.cpp:37,54)
+        SplitBvhBuilder<4, CpuCost> b;
+        b.build(src, on_node, on_leaf, 2, 1e-4f);
+    } else {
+        BinnedSahBuilder<4, CpuCost> b;
+        b.build_mesh(src, on_node, on_leaf);
+    }
+}
+
+// Runs fn(m) for m in [0, n) on up to `threads` threads.
+template <typename Fn>
+void parallel_over(int n, int threads, Fn&& fn) {
+    const int nt = std::max(1, std::min(threads, n));
+    std::atomic<int> next(0);
+    auto work = [&]() {
+        for (int m = next++; m < n; m = next++) fn(m);
+    };
+    if (nt == 1) {
+        work();
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; t++) pool.emplace_back(work);
+    for (auto& th : pool) th.join();
+}
+
+// World-space instance boxes and centres from the mesh boxes (gpu_adapter.cpp:164-179,
+// cpu_adapter.cpp:227-238): centre = M * c, extent = |M| * e.
+void instance_bounds(const std::vector<MeshData>& meshes, const std::vector<InstanceDesc>& instances,
+                     std::vector<BBox>& bounds, std::vector<float3>& centres) {
+    const int ni = (int)instances.size();
+    bounds.resize(ni);
+    centres.resize(ni);
+    for (int i = 0; i < ni; i++) {
+        const InstanceDesc& d = instances[i];
+        const BBox bb = meshes[d.mesh].bbox;
+        const float4 c4((bb.max + bb.min) * 0.5f, 1.0f);
+        const float4 e4((bb.max - bb.min) * 0.5f, 0.0f);
+        float3 c, e;
+        for (int r = 0; r < 3; r++) {
+            const float4 row(d.mat[r][0], d.mat[r][1], d.mat[r][2], d.mat[r][3]);
+            const float4 arow(std::fabs(row.x), std::fabs(row.y), std::fabs(row.z), std::fabs(row.w));
+            c[r] = dot4(row, c4);
+            e[r] = dot4(arow, e4);
+        }
+        centres[i] = c;
+        bounds[i].min = c - e;
+        bounds[i].max = c + e;
+    }
+}
+
 }  // namespace imba
 
 using namespace imba;
@@ -85,6 +147,14 @@ struct ImbaSceneBuild {
     std::vector<uint8_t> mask_buffer;
     std::vector<float> geom_normals;  // 3 per global triangle (object space)
     std::vector<int> tri_materials;
+    std::vector<int> tri_layout;      // first global triangle id of each mesh
+
+    // CPU layout (imba_build_cpu_layout)
+    bool cpu_built = false;
+    int cpu_root = 0;
+    std::vector<ImbaNode4> cpu_nodes;
+    std::vector<ImbaInstanceNode> cpu_inst_nodes;
+    std::vector<ImbaVec4> cpu_tris;
 };
 
 extern "C" {
@@ -192,23 +262,11 @@ int imba_build_run(ImbaSceneBuild* b, int builder, int threads) {
             tri_offset += src.tri_count;
         }
     }
+    b->tri_layout = tri_layout;
 
     // Mesh BVHs, in parallel over meshes; each one is built with mesh-local links.
     std::vector<MeshBvh> per_mesh(nm);
-    {
-        const int nt = std::max(1, std::min(threads, nm));
-        std::atomic<int> next(0);
-        auto work = [&]() {
-            for (int m = next++; m < nm; m = next++) build_mesh_bvh(b->meshes[m], builder, tri_layout[m], per_mesh[m]);
-        };
-        if (nt == 1) {
-            work();
-        } else {
-            std::vector<std::thread> pool;
-            for (int t = 0; t < nt; t++) pool.emplace_back(work);
-            for (auto& th : pool) th.join();
-        }
-    }
+    parallel_over(nm, threads, [&](int m) { build_mesh_bvh(b->meshes[m], builder, tri_layout[m], per_mesh[m]); });
 
     // Concatenate with global links (what one writer appending all meshes would produce).
     b->nodes.clear();
@@ -235,24 +293,9 @@ int imba_build_run(ImbaSceneBuild* b, int builder, int threads) {
 
     // Top level (GpuTopLevelAdapter::build_accel, gpu_adapter.cpp:164-185).
     const int ni = (int)b->instances.size();
-    std::vector<BBox> bounds(ni);
-    std::vector<float3> centres(ni);
-    for (int i = 0; i < ni; i++) {
-        const InstanceDesc& d = b->instances[i];
-        const BBox bb = b->meshes[d.mesh].bbox;
-        const float4 c4((bb.max + bb.min) * 0.5f, 1.0f);
-        const float4 e4((bb.max - bb.min) * 0.5f, 0.0f);
-        float3 c, e;
-        for (int r = 0; r < 3; r++) {
-            const float4 row(d.mat[r][0], d.mat[r][1], d.mat[r][2], d.mat[r][3]);
-            const float4 arow(std::fabs(row.x), std::fabs(row.y), std::fabs(row.z), std::fabs(row.w));
-            c[r] = dot4(row, c4);
-            e[r] = dot4(arow, e4);
-        }
-        centres[i] = c;
-        bounds[i].min = c - e;
-        bounds[i].max = c + e;
-    }
+    std::vector<BBox> bounds;
+    std::vector<float3> centres;
+    instance_bounds(b->meshes, b->instances, bounds, centres);
     std::vector<ImbaNode> top;
     b->inst_nodes.clear();
     {
@@ -281,6 +324,77 @@ int imba_build_run(ImbaSceneBuild* b, int builder, int threads) {
         b->mask_descs.push_back(desc);
     }
     b->built = true;
+    b->cpu_built = false;
+    return IMBA_OK;
+}
+
+int imba_build_cpu_layout(ImbaSceneBuild* b, int builder, int threads) {
+    IMBA_CHECK_ARG(b, "NULL argument");
+    IMBA_CHECK_ARG(builder == IMBA_BUILDER_SBVH || builder == IMBA_BUILDER_BINNED_SAH, "unknown builder");
+    if (!b->built) return fail(IMBA_ERR_INVALID_ARG, "imba_build_run has not been called");
+    const int nm = (int)b->meshes.size();
+    std::vector<MeshBvh4> per_mesh(nm);
+    parallel_over(nm, threads, [&](int m) { build_mesh_bvh4(b->meshes[m], builder, b->tri_layout[m], per_mesh[m]); });
+
+    // Concatenate with global links: node children > 0 shift by the mesh's first node, leaf
+    // links by its first Vec4; 0 (unused slot) stays.
+    b->cpu_nodes.clear();
+    b->cpu_tris.clear();
+    std::vector<int> layout(nm);
+    for (int m = 0; m < nm; m++) {
+        const int node_base = (int)b->cpu_nodes.size();
+        const int tri_base = (int)b->cpu_tris.size();
+        layout[m] = node_base;
+        for (ImbaNode4 n : per_mesh[m].nodes) {
+            for (int j = 0; j < 4; j++) {
+                const int c = n.children[j];
+                n.children[j] = c > 0 ? c + node_base : c < 0 ? ~(~c + tri_base) : 0;
+            }
+            b->cpu_nodes.push_back(n);
+        }
+        b->cpu_tris.insert(b->cpu_tris.end(), per_mesh[m].tris.begin(), per_mesh[m].tris.end());
+        std::vector<ImbaNode4>().swap(per_mesh[m].nodes);
+        std::vector<ImbaVec4>().swap(per_mesh[m].tris);
+    }
+    const int node_count = (int)b->cpu_nodes.size();
+
+    // Top level (CpuTopLevelAdapter::build_accel, cpu_adapter.cpp:227-247).
+    std::vector<BBox> bounds;
+    std::vector<float3> centres;
+    instance_bounds(b->meshes, b->instances, bounds, centres);
+    std::vector<ImbaNode4> top;
+    b->cpu_inst_nodes.clear();
+    {
+        Bvh4TopWriter w(top, b->cpu_inst_nodes, b->instances, layout, node_count);
+        BinnedSahBuilder<4, CpuCost> tb;
+        tb.build(bounds.data(), centres.data(), (int)b->instances.size(),
+                 [&](const BBox& p, int c, const BBox* bx) { w.node(p, c, bx); },
+                 [&](const BBox& bb, int c, const int* ids) { w.leaf(bb, c, ids); });
+    }
+    b->cpu_nodes.insert(b->cpu_nodes.end(), top.begin(), top.end());
+    b->cpu_root = node_count;
+    b->cpu_built = true;
+    return IMBA_OK;
+}
+
+int imba_build_get_cpu_desc(const ImbaSceneBuild* b, ImbaCpuSceneDesc* out) {
+    IMBA_CHECK_ARG(b && out, "NULL argument");
+    if (!b->cpu_built) return fail(IMBA_ERR_INVALID_ARG, "imba_build_cpu_layout has not been called");
+    out->root = b->cpu_root;
+    out->nodes = b->cpu_nodes.data();
+    out->node_count = (int)b->cpu_nodes.size();
+    out->instances = b->cpu_inst_nodes.data();
+    out->instance_count = (int)b->cpu_inst_nodes.size();
+    out->tris = b->cpu_tris.data();
+    out->tri_vec4_count = (int)b->cpu_tris.size();
+    out->indices = b->indices.data();
+    out->index_count = (int)b->indices.size();
+    out->texcoords = b->texcoords.data();
+    out->texcoord_count = (int)b->texcoords.size();
+    out->masks = b->mask_descs.data();
+    out->mask_count = (int)b->mask_descs.size();
+    out->mask_buffer = b->mask_buffer.data();
+    out->mask_buffer_size = (int)b->mask_buffer.size();
     return IMBA_OK;
 }
 

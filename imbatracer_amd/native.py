@@ -29,8 +29,11 @@ NODE_DTYPE = np.dtype([("left_bb", np.dtype(BBOX_FIELDS)), ("right_bb", np.dtype
                        ("left", "<i4"), ("right", "<i4"), ("pad", "<i4", (2,))])
 INSTANCE_DTYPE = np.dtype([("transf", "<f4", (3, 4)), ("id", "<i4"), ("next", "<i4"), ("pad", "<i4", (2,))])
 MASK_DTYPE = np.dtype([("width", "<i4"), ("height", "<i4"), ("offset", "<i4")])
+NODE4_DTYPE = np.dtype([("min_x", "<f4", (4,)), ("max_x", "<f4", (4,)), ("min_y", "<f4", (4,)), ("max_y", "<f4", (4,)),
+                        ("min_z", "<f4", (4,)), ("max_z", "<f4", (4,)), ("children", "<i4", (4,)), ("pad", "<i4", (4,))])
 assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16
 assert NODE_DTYPE.itemsize == 64 and INSTANCE_DTYPE.itemsize == 64 and MASK_DTYPE.itemsize == 12
+assert NODE4_DTYPE.itemsize == 128
 
 DUMMY_CHILD = 0x76543210
 LEAF_END_BITS = 0x80000000
@@ -67,6 +70,11 @@ class SceneDesc(C.Structure):
                 ("mask_buffer", C.c_void_p), ("mask_buffer_size", C.c_int)]
 
 
+class CpuSceneDesc(C.Structure):
+    """ImbaCpuSceneDesc: the fields of SceneDesc, with 4-wide nodes."""
+    _fields_ = SceneDesc._fields_
+
+
 class MeshDesc(C.Structure):
     _fields_ = [("vertices", C.c_void_p), ("vertex_count", C.c_int),
                 ("indices", C.c_void_p), ("tri_count", C.c_int),
@@ -75,6 +83,11 @@ class MeshDesc(C.Structure):
 
 class TraceConfig(C.Structure):
     _fields_ = [("block_size", C.c_int), ("stack_lds", C.c_int), ("persistent", C.c_int), ("blocks_per_cu", C.c_int)]
+
+
+class TraceItem(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("rays", C.c_void_p), ("hits", C.c_void_p), ("count", C.c_int),
+                ("count_dev", C.c_void_p), ("any_hit", C.c_int)]
 
 
 class HashGridInfo(C.Structure):
@@ -121,6 +134,8 @@ _sig("imba_build_set_mask", i32, vp, i32, i32, i32, vp)
 _sig("imba_build_run", i32, vp, i32, i32)
 _sig("imba_build_get_desc", i32, vp, C.POINTER(SceneDesc))
 _sig("imba_build_get_tri_attrs", i32, vp, C.POINTER(vp), C.POINTER(vp), ip)
+_sig("imba_build_cpu_layout", i32, vp, i32, i32)
+_sig("imba_build_get_cpu_desc", i32, vp, C.POINTER(CpuSceneDesc))
 _sig("imba_bvh_emit_mesh", i32, C.POINTER(MeshDesc), i32, i32, i32, vp, sz, C.POINTER(sz))
 _sig("imba_bvh_emit_boxes", i32, vp, vp, i32, i32, i32, vp, sz, C.POINTER(sz))
 _sig("imba_scene_validate", i32, C.POINTER(SceneDesc), ip, ip)
@@ -131,11 +146,23 @@ _sig("imba_scene_device_desc", i32, vp, C.POINTER(SceneDesc))
 _sig("imba_scene_footprint", i32, vp, C.POINTER(sz))
 _sig("imba_intersect", i32, vp, vp, vp, i32, C.POINTER(TraceConfig), vp)
 _sig("imba_occluded", i32, vp, vp, vp, i32, C.POINTER(TraceConfig), vp)
+_sig("imba_intersect_dev", i32, vp, vp, vp, i32, vp, C.POINTER(TraceConfig), vp)
+_sig("imba_occluded_dev", i32, vp, vp, vp, i32, vp, C.POINTER(TraceConfig), vp)
 _sig("imba_intersect_stats", i32, vp, vp, vp, vp, i32, vp)
 _sig("imba_occluded_stats", i32, vp, vp, vp, vp, i32, vp)
 _sig("imba_trace_error_count", i32, vp, ip)
+_sig("imba_trace_batch_create", i32, C.POINTER(TraceItem), i32, C.POINTER(vp))
+_sig("imba_trace_batch_launch", i32, vp, C.POINTER(TraceConfig), vp)
+_sig("imba_trace_batch_release", i32, vp)
+_sig("imba_autotune", i32, vp, vp, vp, i32, i32, i32, C.POINTER(TraceConfig), C.POINTER(f32), vp)
+_sig("imba_graph_begin", i32, vp)
+_sig("imba_graph_end", i32, vp, C.POINTER(vp))
+_sig("imba_graph_launch", i32, vp, vp)
+_sig("imba_graph_release", i32, vp)
 _sig("imba_compact_hits", i32, vp, vp, vp, i32, i32, vp, vp, vp, ip, vp)
 _sig("imba_compact_rays", i32, vp, vp, i32, i32, vp, vp, ip, vp)
+_sig("imba_compact_workspace_size", sz, i32)
+_sig("imba_compact_rays_dev", i32, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp)
 _sig("imba_hashgrid_create", i32, C.POINTER(vp))
 _sig("imba_hashgrid_release", i32, vp)
 _sig("imba_hashgrid_build", i32, vp, vp, i32, i32, i32, f32, vp)
@@ -149,6 +176,7 @@ _sig("imba_gen_camera_rays", i32, vp, vp, vp, f32, i32, i32, vp)
 _sig("imba_gen_camera_matrix", i32, vp, vp, vp, f32, i32, i32, vp)
 _sig("imba_pt_camera", i32, vp, vp, i32, i32, i32, C.c_uint32, i32, i32, vp, vp, vp)
 _sig("imba_pt_bounce", i32, vp, vp, vp, i32, vp, vp, i32, vp, vp, i32, i32, i32, f32, vp, vp, vp)
+_sig("imba_pt_bounce_dev", i32, vp, vp, vp, i32, vp, vp, vp, i32, vp, vp, i32, i32, i32, f32, vp, vp, vp)
 _sig("intersect_gpu_masked_instanced", None, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32)
 _sig("occluded_gpu_masked_instanced", None, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32)
 

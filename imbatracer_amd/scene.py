@@ -98,10 +98,26 @@ class SceneBuild:
         check(N.lib.imba_build_get_desc(self.h, C.byref(d)), "imba_build_get_desc")
         return d
 
+    def cpu_layout(self, builder: int = BUILDER_SBVH, threads: int = 1):
+        """Also writes the CPU layout (BVH4 nodes, 4-triangle packets) of the built scene."""
+        check(N.lib.imba_build_cpu_layout(self.h, builder, threads), "imba_build_cpu_layout")
+        return self
+
+    def cpu_desc(self) -> N.CpuSceneDesc:
+        d = N.CpuSceneDesc()
+        check(N.lib.imba_build_get_cpu_desc(self.h, C.byref(d)), "imba_build_get_cpu_desc")
+        return d
+
+    def cpu_arrays(self) -> dict:
+        """Copies of the CPU-layout host arrays (same keys as arrays(); nodes are NODE4_DTYPE)."""
+        return self._copy(self.cpu_desc(), N.NODE4_DTYPE)
+
     def arrays(self) -> dict:
         """Copies of the host arrays."""
-        d = self.desc()
+        return self._copy(self.desc(), NODE_DTYPE)
 
+    @staticmethod
+    def _copy(d, node_dtype) -> dict:
         def arr(p, n, dt):
             if n == 0 or not p:
                 return np.zeros(0, dt)
@@ -110,7 +126,7 @@ class SceneBuild:
 
         return {
             "root": d.root,
-            "nodes": arr(d.nodes, d.node_count, NODE_DTYPE),
+            "nodes": arr(d.nodes, d.node_count, node_dtype),
             "instances": arr(d.instances, d.instance_count, INSTANCE_DTYPE),
             "tris": arr(d.tris, d.tri_vec4_count, np.dtype(("<f4", 4))).reshape(-1, 4),
             "indices": arr(d.indices, d.index_count, np.int32),
@@ -218,9 +234,70 @@ class Scene:
         (self.occluded if any_hit else self.intersect)(rb, hb, len(rays), cfg)
         return hb.download(HIT_DTYPE, len(rays))
 
+    def autotune(self, rays: DeviceBuffer, hits: DeviceBuffer, count: int, any_hit: bool = False, reps: int = 3,
+                 stream=None):
+        """Fastest launch shape on these rays: (TraceConfig, mean ms per launch)."""
+        cfg, ms = N.TraceConfig(), C.c_float()
+        check(N.lib.imba_autotune(self.h, rays.ptr, hits.ptr, count, int(any_hit), reps, C.byref(cfg), C.byref(ms),
+                                  N._s(stream)), "imba_autotune")
+        return cfg, ms.value
+
     def __del__(self):
         try:
             N.lib.imba_scene_release(self.h)
+        except Exception:
+            pass
+
+
+class TraceBatch:
+    """Many queues (scene, rays, hits, count[, device count], any_hit) traversed together."""
+
+    def __init__(self, items):
+        arr = (N.TraceItem * max(1, len(items)))()
+        self._keep = []
+        for k, it in enumerate(items):
+            scene, rays, hits, count = it[:4]
+            count_dev = it[4] if len(it) > 4 else None
+            any_hit = it[5] if len(it) > 5 else False
+            cdev = count_dev.ptr if isinstance(count_dev, DeviceBuffer) else count_dev
+            arr[k] = N.TraceItem(scene.h, rays.ptr, hits.ptr, count, cdev, int(any_hit))
+            self._keep.append(scene)
+        self.h = C.c_void_p()
+        check(N.lib.imba_trace_batch_create(arr, len(items), C.byref(self.h)), "imba_trace_batch_create")
+
+    def launch(self, cfg: N.TraceConfig | None = None, stream=None):
+        check(N.lib.imba_trace_batch_launch(self.h, C.byref(cfg) if cfg else None, N._s(stream)),
+              "imba_trace_batch_launch")
+
+    def __del__(self):
+        try:
+            N.lib.imba_trace_batch_release(self.h)
+        except Exception:
+            pass
+
+
+class Graph:
+    """A captured sequence of library calls on one stream: `with Graph.capture(stream) as g:`."""
+
+    def __init__(self, h):
+        self.h = h
+
+    @staticmethod
+    def begin(stream):
+        check(N.lib.imba_graph_begin(N._s(stream)), "imba_graph_begin")
+
+    @staticmethod
+    def end(stream) -> "Graph":
+        h = C.c_void_p()
+        check(N.lib.imba_graph_end(N._s(stream), C.byref(h)), "imba_graph_end")
+        return Graph(h)
+
+    def launch(self, stream=None):
+        check(N.lib.imba_graph_launch(self.h, N._s(stream)), "imba_graph_launch")
+
+    def __del__(self):
+        try:
+            N.lib.imba_graph_release(self.h)
         except Exception:
             pass
 

@@ -22,6 +22,42 @@ from . import scene as S
 
 STATE_BYTES = 32
 
+C1_CAMERA = ((0.5, 0.5, -1.5), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0), 60.0)
+C1_LIGHT = np.array([0.5, 2.0, 0.5], np.float32)
+
+
+def random_tri_scene(n: int, builder: int = N.BUILDER_BINNED_SAH, seed: int = 0x1234, threads: int = 1):
+    """n seeded random triangles (one mesh, identity instance), built into a SceneBuild."""
+    v, i = S.random_triangles(n, seed)
+    b = S.SceneBuild()
+    b.add_mesh(v, i)
+    b.run(builder, threads)
+    return b, v, i
+
+
+def primary_rays(w: int, h: int) -> np.ndarray:
+    """Pinhole rays of the random-triangle workload's camera, one per pixel."""
+    pos, d, up, fov = C1_CAMERA
+    return S.camera_rays(pos, d, up, fov, w, h)
+
+
+def shadow_rays(primary: np.ndarray, hits: np.ndarray, light=C1_LIGHT, seed: int = 99) -> np.ndarray:
+    """One occlusion ray per primary ray towards the point light (pt.cpp:33-41 conventions):
+    origin at the primary hit (or a seeded point in the unit cube on a miss),
+    tmin = 1e-3 * dist, tmax = dist - tmin."""
+    rng = np.random.default_rng(seed)
+    o = primary["org"][:, :3].astype(np.float32)
+    d = primary["dir"][:, :3].astype(np.float32)
+    t = hits["tmax"].astype(np.float32)[:, None]
+    pts = (o + t * d).astype(np.float32)
+    miss = hits["tri_id"] < 0
+    pts[miss] = rng.random((int(miss.sum()), 3), dtype=np.float32)
+    to = (light[None, :] - pts).astype(np.float32)
+    dist = np.sqrt((to * to).sum(1)).astype(np.float32)
+    dirs = (to / dist[:, None]).astype(np.float32)
+    off = (np.float32(1e-3) * dist).astype(np.float32)
+    return S.make_rays(pts, dirs, off, (dist - off).astype(np.float32))
+
 
 def mwc_uniform(seed: int, n: int) -> np.ndarray:
     """n draws of the MWC64X generator's random_float() (render/random.h:19-46)."""
@@ -122,6 +158,8 @@ class PathTracer:
         self.image = N.DeviceBuffer(width * height * 12)
         self.stream = N.Stream()
         self.events = []
+        self.dev_counts = None
+        self.graph = None
 
     def frame(self, stats: bool = False, keep_queues: int = 0, timed: bool = False):
         """Runs one frame. Returns per-bounce ray counts, and with stats=True the per-bounce
@@ -170,3 +208,68 @@ class PathTracer:
 
     def trace_times_ms(self):
         return [(e0.elapsed_ms(e1), n) for e0, e1, n in self.events]
+
+    # -- device-resident frame: counts stay on the GPU, the frame is one graph launch ----------
+    def _frame_dev(self, batches):
+        """Enqueues one frame whose queue sizes live in self.dev_counts (entry b = rays of bounce
+        b): every bounce is launched, a bounce whose queue is empty does no work."""
+        st = self.stream.handle
+        n = self.n
+        N.check(N.lib.imba_pt_camera(N.ptr(self.r2w), N.ptr(self.cam_pos), self.w, self.h, self.spp, self.seed_base, 0,
+                                     n, self.rays[0].ptr, self.states[0].ptr, st), "camera")
+        cur = 0
+        for bounce in range(self.max_path_len):
+            cnt = self.dev_counts.at(4 * bounce)
+            if batches:
+                batches[bounce].launch(self.cfg, self.stream)
+            else:
+                N.check(N.lib.imba_intersect_dev(self.gpu_scene.h, self.rays[cur].ptr, self.hits.ptr, n, cnt,
+                                                 C.byref(self.cfg) if self.cfg else None, st), "intersect_dev")
+            N.check(N.lib.imba_pt_bounce_dev(self.rays[cur].ptr, self.hits.ptr, self.states[cur].ptr, n, cnt,
+                                             self.tri_n.ptr, self.tri_m.ptr, self.s.tri_count, self.albedo.ptr,
+                                             self.emission.ptr, len(self.s.albedo), self.max_path_len, self.rr_start,
+                                             self.s.offset, self.image.ptr, self.rays[cur].ptr, st), "bounce_dev")
+            N.check(N.lib.imba_compact_rays_dev(self.rays[cur].ptr, self.states[cur].ptr, STATE_BYTES, n, cnt,
+                                                self.rays[1 - cur].ptr, self.states[1 - cur].ptr,
+                                                self.dev_counts.at(4 * (bounce + 1)), self.workspace.ptr, st),
+                    "compact_rays_dev")
+            cur = 1 - cur
+
+    def build_graph(self, use_batch: bool = False):
+        """Captures one device-resident frame into a graph (after one eager run of the same
+        calls, which sizes the spill stacks and counters the captured launches use).
+        use_batch traverses with the batched multi-queue kernel (one queue) instead of k_trace."""
+        from .scene import Graph, TraceBatch
+
+        n = self.n
+        self.dev_counts = N.DeviceBuffer(4 * (self.max_path_len + 1))
+        counts0 = np.zeros(self.max_path_len + 1, np.int32)
+        counts0[0] = n
+        self.dev_counts.upload(counts0)
+        self.workspace = N.DeviceBuffer(int(N.lib.imba_compact_workspace_size(n)))
+        batches = None
+        if use_batch:
+            batches = [TraceBatch([(self.gpu_scene, self.rays[b % 2], self.hits, n, self.dev_counts.at(4 * b), False)])
+                       for b in range(self.max_path_len)]
+        self._batches = batches
+        self._frame_dev(batches)
+        self.stream.sync()
+        Graph.begin(self.stream)
+        try:
+            self._frame_dev(batches)
+        finally:
+            g = Graph.end(self.stream)
+        self.graph = g
+        return g
+
+    def frame_graph(self):
+        """One frame by graph launch; returns the per-bounce ray counts (non-empty bounces)."""
+        self.graph.launch(self.stream)
+        c = self.dev_counts.download(np.int32, self.max_path_len + 1, self.stream)
+        self.stream.sync()
+        out = []
+        for v in c[:self.max_path_len]:
+            if v == 0:
+                break
+            out.append(int(v))
+        return out

@@ -153,6 +153,48 @@ int imba_build_get_desc(const ImbaSceneBuild* b, ImbaSceneDesc* out);
 /* Per-global-triangle geometric normal (3 floats) and material, for shading kernels. */
 int imba_build_get_tri_attrs(const ImbaSceneBuild* b, const float** geom_normals, const int** materials, int* tri_count);
 
+/* CPU traversal layout, the arrays the reference's CPU traversal reads
+ * (intersect_cpu_masked_instanced, ray_queue.h:254-265), written as CpuMeshAdapter and
+ * CpuTopLevelAdapter write them (core/cpu_adapter.cpp:14-206, 221-366):
+ *   - a node has four child slots: box j and children[j].  children[j] > 0 is a node index
+ *     (children always follow their parent, so 0 never names a child); < 0 is ~(first Vec4 of
+ *     the leaf's packets) at mesh level and ~(first instance record of the leaf) at top level;
+ *     0 is an unused slot, whose box is [FLT_MAX, -FLT_MAX] (all zero in the dummy parent of a
+ *     tree that is a single leaf).  Top-level nodes follow the mesh nodes; root = their first.
+ *   - a leaf is a run of 13-Vec4 packets of up to four triangles, one per lane:
+ *     v0.x, v0.y, v0.z, e1.x, e1.y, e1.z, e2.x, e2.y, e2.z, n.x, n.y, n.z, id, with
+ *     e1 = v0 - v1, e2 = v2 - v0, n = e1 x e2 (fma form: n.x = fma(e1.y, e2.z, -(e1.z * e2.y))
+ *     and cyclic) and id the global triangle id as int bits; an unused lane has zero
+ *     geometry and id INT_MIN.  After a leaf's packets comes one Vec4 whose four lanes are
+ *     0x80000000.
+ *   - instance records as in the GPU layout, in the order of the top-level BVH4's leaves. */
+typedef struct ImbaNode4 {
+    float min_x[4], max_x[4];
+    float min_y[4], max_y[4];
+    float min_z[4], max_z[4];
+    int children[4];
+    int pad[4];
+} ImbaNode4;                                  /* 128 B: two cache lines */
+
+typedef struct ImbaCpuSceneDesc {
+    int root;
+    const ImbaNode4* nodes;                int node_count;
+    const ImbaInstanceNode* instances;     int instance_count;
+    const ImbaVec4* tris;                  int tri_vec4_count;
+    const int* indices;                    int index_count;
+    const ImbaVec2* texcoords;             int texcoord_count;
+    const ImbaTransparencyMask* masks;     int mask_count;
+    const uint8_t* mask_buffer;            int mask_buffer_size;
+} ImbaCpuSceneDesc;
+
+/* Builds the CPU layout of the scene last built by imba_build_run: mesh BVH4s with `builder`
+ * (SBVH with leaf threshold 2 and alpha 1e-4 as CpuMeshAdapter, cpu_adapter.cpp:37; or binned
+ * SAH), the top-level BVH4 with the binned-SAH builder (cpu_adapter.cpp:243-247), costs
+ * ((count - 1) / 4 + 1) * area and 0.5 * area (cpu_adapter.cpp:45-52). */
+int imba_build_cpu_layout(ImbaSceneBuild* b, int builder, int threads);
+/* Host pointers stay valid until imba_build_release or the next imba_build_cpu_layout. */
+int imba_build_get_cpu_desc(const ImbaSceneBuild* b, ImbaCpuSceneDesc* out);
+
 /* Runs one builder and writes the sequence of nodes and leaves it emits (for checking the
  * builders and for custom layout writers).  Records, as int32 (floats as their bits):
  *   node: 0, count, parent box, count child boxes      leaf: 1, count, box, count primitive ids
@@ -201,6 +243,12 @@ int imba_intersect(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int c
                    const ImbaTraceConfig* cfg, void* stream);
 int imba_occluded(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int count,
                   const ImbaTraceConfig* cfg, void* stream);
+/* Device-count variants (capturable in a graph): the queue holds *count_dev rays (clamped to
+ * [0, capacity]) when the launch runs; the grid is sized for `capacity`. */
+int imba_intersect_dev(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int capacity, const int* count_dev,
+                       const ImbaTraceConfig* cfg, void* stream);
+int imba_occluded_dev(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int capacity, const int* count_dev,
+                      const ImbaTraceConfig* cfg, void* stream);
 /* Same traversal, also writes per-ray visit counts (device, 3 uint32 per ray:
  * inner nodes whose children were box-tested, triangles tested, instances entered). */
 int imba_intersect_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts,
@@ -213,6 +261,46 @@ int imba_occluded_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, 
  * hit, 1 occluded. */
 int imba_trace_desc(const ImbaSceneDesc* dev, const ImbaRay* rays, ImbaHit* hits, int count, int any_hit,
                     void* stream);
+
+/* Batched traversal of many queues (the loop of RayQueue::traverse_gpu / traverse_occluded_gpu
+ * calls that QueueScheduler::process_queues and the TileScheduler threads issue,
+ * render/scheduling/queue_scheduler.h:193-218, tile_scheduler.h:120-132), one launch per
+ * kernel variant (closest / occluded x opaque / masked scene).  Every queue's hits equal those
+ * of its own imba_intersect / imba_occluded call.  The batch keeps the queue table, work
+ * counters and spill stacks in device memory of its own: a launch allocates nothing (after a
+ * first launch of a shape that needs spill stacks) and may be captured into a graph
+ * (imba_graph_*).  Launches of one batch must be ordered (one stream). */
+typedef struct ImbaTraceItem {
+    const ImbaScene* scene;
+    const ImbaRay* rays;       /* device */
+    ImbaHit* hits;             /* device */
+    int count;                 /* rays; with count_dev, the most rays *count_dev may hold */
+    const int* count_dev;      /* NULL, or a device int with the ray count, read when the launch runs */
+    int any_hit;               /* 0: closest hit, 1: occluded */
+} ImbaTraceItem;
+typedef struct ImbaTraceBatch ImbaTraceBatch;
+int imba_trace_batch_create(const ImbaTraceItem* items, int n_items, ImbaTraceBatch** out);
+/* cfg: block_size / stack_lds / blocks_per_cu as for imba_intersect; every launch is persistent. */
+int imba_trace_batch_launch(ImbaTraceBatch* b, const ImbaTraceConfig* cfg, void* stream);
+int imba_trace_batch_release(ImbaTraceBatch* b);
+
+/* Launch shape autotuning: times every supported shape (static and persistent) on the given
+ * device queue with HIP events, `reps` launches each after one warm-up, and returns the
+ * fastest in *best (and its mean time per launch in *best_ms, nullable).  Results do not
+ * depend on the shape. */
+int imba_autotune(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int count, int any_hit, int reps,
+                  ImbaTraceConfig* best, float* best_ms, void* stream);
+
+/* ------------------------------ graphs ----------------------------------------------------
+ * Capture of a sequence of this library's asynchronous calls on one stream into a replayable
+ * HIP graph (the bounce loop of a frame, for instance).  Calls to be captured take a created
+ * stream (not NULL), device-side counts (the *_dev variants) and workspaces sized
+ * beforehand; a call that would synchronize or allocate fails during capture. */
+typedef struct ImbaGraph ImbaGraph;
+int imba_graph_begin(void* stream);
+int imba_graph_end(void* stream, ImbaGraph** out);
+int imba_graph_launch(ImbaGraph* g, void* stream);
+int imba_graph_release(ImbaGraph* g);
 /* Number of rays the last launches found a malformed BVH for (0 for a valid scene). */
 int imba_trace_error_count(const ImbaScene* s, int* count);
 
@@ -230,6 +318,14 @@ int imba_compact_hits(const ImbaRay* rays_in, const ImbaHit* hits_in, const void
                       int* new_count, void* stream);
 int imba_compact_rays(const ImbaRay* rays_in, const void* states_in, int state_stride, int count,
                       ImbaRay* rays_out, void* states_out, int* new_count, void* stream);
+/* Device-count variant of compact_rays for graph-captured loops: the queue holds *count_dev
+ * (clamped to [0, capacity]) entries when the launch runs and the new size is written to the
+ * device int *new_count_dev; nothing is read back to the host.  `workspace` is device memory
+ * of at least imba_compact_workspace_size(capacity) bytes. */
+size_t imba_compact_workspace_size(int capacity);
+int imba_compact_rays_dev(const ImbaRay* rays_in, const void* states_in, int state_stride, int capacity,
+                          const int* count_dev, ImbaRay* rays_out, void* states_out, int* new_count_dev,
+                          void* workspace, void* stream);
 
 /* ------------------------------ photon hash grid ------------------------------------------
  * HashGrid::build / query (rangesearch/rangesearch.h:25-148).  Photons are records of
@@ -278,6 +374,12 @@ int imba_pt_bounce(const ImbaRay* rays, const ImbaHit* hits, void* states, int c
                    const int* tri_materials, int tri_count, const float* albedo, const float* emission,
                    int material_count, int max_path_len, int rr_start, float offset, float* image,
                    ImbaRay* rays_out, void* stream);
+/* Same with the queue size read from the device int *count_dev (clamped to [0, capacity])
+ * when the launch runs. */
+int imba_pt_bounce_dev(const ImbaRay* rays, const ImbaHit* hits, void* states, int capacity, const int* count_dev,
+                       const float* tri_normals, const int* tri_materials, int tri_count, const float* albedo,
+                       const float* emission, int material_count, int max_path_len, int rr_start, float offset,
+                       float* image, ImbaRay* rays_out, void* stream);
 
 /* ------------------------------ synthetic workloads ---------------------------------------
  * Seeded generators for the measured configurations (host memory outputs). */

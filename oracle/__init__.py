@@ -62,6 +62,10 @@ def lib():
         L.oracle_trace.restype = i32
         L.oracle_trace_mt.argtypes = [C.POINTER(SceneDesc), vp, i32, i32, vp, i32]
         L.oracle_trace_mt.restype = i32
+        L.cpu4_trace.argtypes = [C.POINTER(SceneDesc), vp, i32, i32, vp]
+        L.cpu4_trace.restype = i32
+        L.cpu4_trace_mt.argtypes = [C.POINTER(SceneDesc), vp, i32, i32, vp, i32]
+        L.cpu4_trace_mt.restype = i32
         L.oracle_brute_closest.argtypes = [vp, i32, vp, i32, vp, vp]
         L.oracle_brute_closest.restype = None
         L.oracle_compact_hits.argtypes = [vp, vp, vp, i32, i32]
@@ -132,6 +136,19 @@ def trace(arrays: dict, rays: np.ndarray, any_hit: bool = False, counts: bool = 
         bad = lib().oracle_trace_mt(C.byref(d), _p(rays), len(rays), int(any_hit), _p(hits), threads)
     else:
         bad = lib().oracle_trace(C.byref(d), _p(rays), len(rays), int(any_hit), _p(hits), None)
+    return hits, bad
+
+
+def trace_cpu4(arrays: dict, rays: np.ndarray, any_hit: bool = False, threads: int = 1):
+    """The CPU-layout traversal (BVH4 + 4-triangle packets; arrays from SceneBuild.cpu_arrays()).
+    Returns (hits, number of rays that met a malformed structure)."""
+    d, keep = scene_desc(arrays)
+    rays = np.ascontiguousarray(rays)
+    hits = np.empty(len(rays), HIT_DTYPE)
+    if threads > 1:
+        bad = lib().cpu4_trace_mt(C.byref(d), _p(rays), len(rays), int(any_hit), _p(hits), threads)
+    else:
+        bad = lib().cpu4_trace(C.byref(d), _p(rays), len(rays), int(any_hit), _p(hits))
     return hits, bad
 
 

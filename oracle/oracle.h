@@ -29,6 +29,14 @@ int oracle_trace(const ImbaSceneDesc* scene, const ImbaRay* rays, int count, int
 int oracle_trace_mt(const ImbaSceneDesc* scene, const ImbaRay* rays, int count, int any_hit, ImbaHit* hits,
                     int threads);
 
+/* ---- CPU baseline (cpu_bvh4.c): the reference's CPU layout, BVH4 + 4-triangle packets ---- */
+/* Closest hit or any hit over an ImbaCpuSceneDesc (imba_build_cpu_layout); returns the number
+ * of rays that met a malformed structure. */
+int cpu4_trace(const ImbaCpuSceneDesc* scene, const ImbaRay* rays, int count, int any_hit, ImbaHit* hits);
+/* Same on `threads` host threads taking chunks of 256 rays from a shared counter. */
+int cpu4_trace_mt(const ImbaCpuSceneDesc* scene, const ImbaRay* rays, int count, int any_hit, ImbaHit* hits,
+                  int threads);
+
 /* Brute force in double precision over explicit world-space triangles (9 floats each,
  * ids = index).  Closest t in [tmin, tmax] (edges inclusive), ties to the lower id. */
 void oracle_brute_closest(const float* tri_verts, int tri_count, const ImbaRay* rays, int count,

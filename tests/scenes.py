@@ -6,39 +6,11 @@ import numpy as np
 from imbatracer_amd import native as N
 from imbatracer_amd import scene as S
 
-C1_CAMERA = ((0.5, 0.5, -1.5), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0), 60.0)
-C1_LIGHT = np.array([0.5, 2.0, 0.5], np.float32)
+from imbatracer_amd.workloads import C1_CAMERA, C1_LIGHT, primary_rays, random_tri_scene as _rts, shadow_rays  # noqa: F401
 
 
 def random_tri_scene(n: int, builder=N.BUILDER_BINNED_SAH, seed: int = 0x1234):
-    v, i = S.random_triangles(n, seed)
-    b = S.SceneBuild()
-    b.add_mesh(v, i)
-    b.run(builder, 1)
-    return b, v, i
-
-
-def primary_rays(w: int, h: int) -> np.ndarray:
-    pos, d, up, fov = C1_CAMERA
-    return S.camera_rays(pos, d, up, fov, w, h)
-
-
-def shadow_rays(primary: np.ndarray, hits: np.ndarray, light=C1_LIGHT, seed: int = 99) -> np.ndarray:
-    """One occlusion ray per primary ray towards the point light (pt.cpp:33-41 conventions):
-    origin at the primary hit (or a seeded point in the unit cube on a miss),
-    tmin = 1e-3 * dist, tmax = dist - tmin."""
-    rng = np.random.default_rng(seed)
-    o = primary["org"][:, :3].astype(np.float32)
-    d = primary["dir"][:, :3].astype(np.float32)
-    t = hits["tmax"].astype(np.float32)[:, None]
-    pts = (o + t * d).astype(np.float32)
-    miss = hits["tri_id"] < 0
-    pts[miss] = rng.random((int(miss.sum()), 3), dtype=np.float32)
-    to = (light[None, :] - pts).astype(np.float32)
-    dist = np.sqrt((to * to).sum(1)).astype(np.float32)
-    dirs = (to / dist[:, None]).astype(np.float32)
-    off = (np.float32(1e-3) * dist).astype(np.float32)
-    return S.make_rays(pts, dirs, off, (dist - off).astype(np.float32))
+    return _rts(n, builder, seed)
 
 
 def random_rays(n: int, seed: int = 5, box=(-0.2, 1.2)) -> np.ndarray:
