@@ -42,10 +42,11 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def ray_bytes(visits, rays):
-    """Algorithmic bytes of `rays` rays with summed visit counts (nodes, tris, instances)."""
+def ray_bytes(visits, rays, node_bytes=64):
+    """Algorithmic bytes of `rays` rays with summed visit counts (nodes, tris, instances); a
+    BVH2 node is 64 bytes, a 4-wide node 128."""
     nodes, tris, inst = (float(x) for x in visits)
-    return 32.0 * rays + 16.0 * rays + 64.0 * nodes + 48.0 * tris + 64.0 * inst
+    return 32.0 * rays + 16.0 * rays + node_bytes * nodes + 48.0 * tris + 64.0 * inst
 
 
 def measured_traffic(workload: str, kernel: str):
@@ -112,13 +113,15 @@ def run_pt(args):
     threads = max(1, min(16, os.cpu_count() or 1))
     scene = W.SphereGridScene(side=args.side, level=5, builder=N.BUILDER_SBVH, threads=threads)
     log("scene: %d triangles, %d nodes, built in %.1f s" % (scene.tri_count, scene.desc.node_count, time.time() - t0))
-    cfg = N.TraceConfig(args.block, args.stack, args.persistent, args.blocks_per_cu) if (args.block or args.persistent or args.blocks_per_cu) else None
+    cfg = N.TraceConfig(args.block, args.stack, args.persistent, args.blocks_per_cu, args.bvh_width) \
+        if (args.block or args.persistent or args.blocks_per_cu or args.bvh_width) else None
+    node_bytes = 128 if args.bvh_width == 4 else 64
     pt = W.PathTracer(scene, args.width, args.height, spp=1, max_path_len=8, rr_start=3, trace_cfg=cfg)
     log("uploaded %.2f GB" % (pt.gpu_scene.footprint() / 1e9))
 
     # untimed: visit counts of every bounce's queue (same seeds -> same rays as the timed frames)
     counts, visits, queues = pt.frame(stats=True, keep_queues=2)
-    stat_bytes = [ray_bytes(v, n) for v, n in zip(visits, counts)]
+    stat_bytes = [ray_bytes(v, n, node_bytes) for v, n in zip(visits, counts)]
     log("rays per bounce %s" % counts)
 
     # eager frames (host reads each bounce's queue size): traversal kernel times for the roofline
@@ -178,6 +181,7 @@ def run_pt(args):
                    "frame": "one graph launch per frame (queue sizes on the device)" if args.mode == "graph"
                             else "eager launches, host reads each bounce's queue size",
                    "trace_kernel": args.trace_kernel,
+                   "bvh_width": args.bvh_width or 2,
                    "trace_shape": [args.block, args.stack] if args.block else "default"},
         "eager_frames": {"value": rays_eager / wall_eager / 1e6, "ms_per_step": wall_eager * 1e3 / args.steps,
                          "frame_latency_ms": {"p50": float(np.percentile(eager_ms, 50)),
@@ -210,14 +214,16 @@ def run_c1(args):
     rb = N.DeviceBuffer.from_array(prim)
     hb = N.DeviceBuffer(n * 16)
     cb = N.DeviceBuffer(n * 12)
-    cfg = N.TraceConfig(args.block, args.stack, args.persistent, args.blocks_per_cu) if (args.block or args.persistent or args.blocks_per_cu) else None
-    sc.intersect_stats(rb, hb, cb, n)
+    cfg = N.TraceConfig(args.block, args.stack, args.persistent, args.blocks_per_cu, args.bvh_width) \
+        if (args.block or args.persistent or args.blocks_per_cu or args.bvh_width) else None
+    node_bytes = 128 if args.bvh_width == 4 else 64
+    sc.trace_stats(rb, hb, cb, n, cfg=cfg)
     hits = hb.download(N.HIT_DTYPE, n)
     vis_p = cb.download(np.uint32, 3 * n).reshape(-1, 3).astype(np.int64).sum(0)
     shadow = T.shadow_rays(prim, hits)
     sb = N.DeviceBuffer.from_array(shadow)
     ob = N.DeviceBuffer(n * 16)
-    sc.intersect_stats(sb, ob, cb, n, any_hit=True)
+    sc.trace_stats(sb, ob, cb, n, any_hit=True, cfg=cfg)
     vis_s = cb.download(np.uint32, 3 * n).reshape(-1, 3).astype(np.int64).sum(0)
     st = N.Stream()
     for _ in range(args.warmup):
@@ -237,7 +243,7 @@ def run_c1(args):
     st.sync()
     wall = time.perf_counter() - t0
     t_closest = sum(a.elapsed_ms(b) for a, b, c in ev)
-    achieved = ray_bytes(vis_p, n) * args.steps / (t_closest * 1e-3) / 1e9
+    achieved = ray_bytes(vis_p, n, node_bytes) * args.steps / (t_closest * 1e-3) / 1e9
     workload = "c1-65536-random-tris-binned-sah-1M-primary-1M-occlusion"
     traffic, traffic_src = measured_traffic(workload, "k_trace_closest") if cfg is None else (None, None)
     res = {
@@ -245,12 +251,12 @@ def run_c1(args):
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded random triangles, pinhole camera, point light)",
-        "config": {"workload": workload},
+        "config": {"workload": workload, "bvh_width": args.bvh_width or 2},
         "closest_mrays_s": n * args.steps / (t_closest * 1e-3) / 1e6,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": "k_trace (closest hit)", "algorithmic_bytes_per_launch": ray_bytes(vis_p, n),
-                     "bytes_per_ray": ray_bytes(vis_p, n) / n},
+                     "kernel": "k_trace (closest hit)", "algorithmic_bytes_per_launch": ray_bytes(vis_p, n, node_bytes),
+                     "bytes_per_ray": ray_bytes(vis_p, n, node_bytes) / n},
     }
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(b, N.BUILDER_BINNED_SAH, 1, [prim, shadow], [False, True],
@@ -286,13 +292,15 @@ def run_c4(args):
         if q < 2:
             queues[-1] += (rays,)
     log("%d queues x %d rays uploaded" % (n_queues, per_queue))
+    cfg = N.TraceConfig(args.block, args.stack, 0, args.blocks_per_cu, args.bvh_width) \
+        if (args.block or args.blocks_per_cu or args.bvh_width) else None
     # visit counts per queue (untimed)
     cb = N.DeviceBuffer(per_queue * 12)
     bytes_step = 0.0
+    node_bytes = 128 if args.bvh_width == 4 else 64
     for sc, rb, hb, n, _, _ in [q[:6] for q in queues]:
-        sc.intersect_stats(rb, hb, cb, n)
-        bytes_step += ray_bytes(cb.download(np.uint32, 3 * n).reshape(-1, 3).astype(np.int64).sum(0), n)
-    cfg = N.TraceConfig(args.block, args.stack, 0, args.blocks_per_cu) if (args.block or args.blocks_per_cu) else None
+        sc.trace_stats(rb, hb, cb, n, cfg=cfg)
+        bytes_step += ray_bytes(cb.download(np.uint32, 3 * n).reshape(-1, 3).astype(np.int64).sum(0), n, node_bytes)
     batch = S.TraceBatch([q[:6] for q in queues])
     st = N.Stream()
 
@@ -367,6 +375,8 @@ def main():
                     help="pt: frames as one captured graph (default) or as eager launches")
     ap.add_argument("--trace-kernel", default="plain", choices=["plain", "batch"],
                     help="pt graph frames: k_trace with a device count, or the batched queue kernel")
+    ap.add_argument("--bvh-width", type=int, default=0, choices=[0, 2, 4],
+                    help="tree the traversal walks: 2 = the scene's BVH2, 4 = the derived 4-wide tree (0 = default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

@@ -25,8 +25,9 @@ CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 
 HOST_SOURCES = ["host/vec.cpp", "host/status.cpp", "host/scene_build.cpp", "host/gen.cpp", "host/bvh_emit.cpp",
-                "host/abi_check.cpp"]
-HIP_SOURCES = ["hip/runtime.hip", "hip/traverse.hip", "hip/compact.hip", "hip/hashgrid.hip", "hip/pt.hip"]
+                "host/abi_check.cpp", "host/bvh_cache.cpp", "host/wide_bvh.cpp"]
+HIP_SOURCES = ["hip/runtime.hip", "hip/traverse.hip", "hip/compact.hip", "hip/hashgrid.hip", "hip/pt.hip",
+               "hip/sort.hip"]
 
 HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-I" + INCLUDE, "-pthread"]
 HIP_FLAGS = [
@@ -65,7 +66,7 @@ def build(verbose: bool = False) -> str:
                 if verbose and out.strip():
                     print(out)
         tmp_lib = LIB_PATH + ".tmp"
-        _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp_lib] + objs + ["-pthread"])
+        _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp_lib] + objs + ["-pthread"])
         os.replace(tmp_lib, LIB_PATH)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)

@@ -12,6 +12,7 @@
 
 #include "common.h"
 #include "scene.h"
+#include "../host/wide_bvh.h"
 
 namespace imba {
 
@@ -69,6 +70,48 @@ int grow_buffer(Scratch& s, size_t bytes) {
     IMBA_HIP_TRY(hipMalloc(&s.ptr, rounded));
     s.bytes = rounded;
     return IMBA_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// The 4-wide tree of a scene (derived on the host, uploaded next to the BVH2).
+int upload_wide(const ImbaSceneDesc& host, const ImbaSceneDesc& dev, const SceneInfo& info, ImbaSceneDesc* dev4,
+                SceneInfo* info4, size_t* bytes) {
+    WideScene w;
+    int r = collapse_scene4(host, w);
+    if (r) return r;
+    *dev4 = dev;
+    dev4->nodes = nullptr;
+    dev4->instances = nullptr;
+    void* pn = nullptr;
+    void* pi = nullptr;
+    const size_t nb = sizeof(ImbaNode4) * w.nodes.size(), ib = sizeof(ImbaInstanceNode) * w.instances.size();
+    IMBA_HIP_TRY(hipMalloc(&pn, nb));
+    if (hipMalloc(&pi, ib) != hipSuccess) {
+        hipFree(pn);
+        return fail(IMBA_ERR_OUT_OF_MEMORY, "4-wide instance records");
+    }
+    if (hipMemcpy(pn, w.nodes.data(), nb, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pi, w.instances.data(), ib, hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(pn);
+        hipFree(pi);
+        return fail(IMBA_ERR_HIP, "upload of the 4-wide tree failed");
+    }
+    dev4->nodes = (const ImbaNode*)pn;
+    dev4->node_count = (int)w.nodes.size();
+    dev4->instances = (const ImbaInstanceNode*)pi;
+    dev4->instance_count = (int)w.instances.size();
+    dev4->root = w.root;
+    *info4 = info;
+    info4->max_stack = w.max_stack;
+    *bytes = nb + ib;
+    return IMBA_OK;
+}
+
+void free_wide(ImbaSceneDesc& dev4) {
+    if (dev4.nodes) (void)hipFree(const_cast<ImbaNode*>(dev4.nodes));
+    if (dev4.instances) (void)hipFree(const_cast<ImbaInstanceNode*>(dev4.instances));
+    dev4.nodes = nullptr;
+    dev4.instances = nullptr;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -341,6 +384,11 @@ int imba_scene_upload(const ImbaSceneDesc* h, ImbaScene** out) {
     dd.masks = (const ImbaTransparencyMask*)p;
     if ((r = up(h->mask_buffer, (size_t)h->mask_buffer_size, &p))) goto err;
     dd.mask_buffer = (const uint8_t*)p;
+    {
+        size_t wb = 0;
+        if ((r = upload_wide(*h, dd, info, &s->dev4, &s->info4, &wb))) goto err;
+        s->footprint += wb;
+    }
     if (hipMalloc((void**)&s->d_error, sizeof(int)) != hipSuccess) { r = fail(IMBA_ERR_OUT_OF_MEMORY, "error counter"); goto err; }
     if (hipMemset(s->d_error, 0, sizeof(int)) != hipSuccess) { r = fail(IMBA_ERR_HIP, "error counter"); goto err; }
     s->footprint += sizeof(int);
@@ -357,6 +405,7 @@ int imba_scene_release(ImbaScene* s) {
                            s->dev.masks, s->dev.mask_buffer, s->d_error};
     for (const void* p : owned)
         if (p) (void)hipFree(const_cast<void*>(p));
+    free_wide(s->dev4);
     delete s;
     return IMBA_OK;
 }

@@ -14,6 +14,13 @@ struct SceneInfo {
 
 int validate_scene(const ImbaSceneDesc& d, SceneInfo& info);
 
+// Derives the 4-wide tree of a validated host scene and uploads its nodes and instance records:
+// *dev4 = dev with nodes, node_count, instances and root replaced; info4 = info with the
+// 4-wide tree's stack bound.  *bytes receives the device bytes allocated.
+int upload_wide(const ImbaSceneDesc& host, const ImbaSceneDesc& dev, const SceneInfo& info, ImbaSceneDesc* dev4,
+                SceneInfo* info4, size_t* bytes);
+void free_wide(ImbaSceneDesc& dev4);
+
 }  // namespace imba
 
 struct ImbaScene {
@@ -21,6 +28,10 @@ struct ImbaScene {
     imba::SceneInfo info;
     ImbaSceneDesc counts;  // host view as given at upload (pointers not owned)
     ImbaSceneDesc dev;     // device arrays (owned)
+    // the 4-wide tree derived at upload: nodes (ImbaNode4 records) and instance records of its
+    // own; the other arrays are those of `dev`
+    ImbaSceneDesc dev4;
+    imba::SceneInfo info4;
     int* d_error = nullptr;
     size_t footprint = 0;
 };

@@ -248,10 +248,98 @@ __device__ __forceinline__ void resolve_leaf(const TraceArgs& a, TraceState& s, 
     }
 }
 
+// One inner node of the BVH2: both child boxes are tested, the nearer hit child (ties: left)
+// is next and the other is pushed.
+template <int BLOCK, int STACK_LDS, bool STATS>
+__device__ __forceinline__ void node_step2(const TraceArgs& a, TraceState& s, LaneStack<BLOCK, STACK_LDS>& st) {
+    const int n = s.node;
+    if (n >= a.n_nodes) {
+        s.flags |= kBad;
+        s.node = kNoNode;
+        return;
+    }
+    const float4 n0 = a.nodes[4 * n + 0];
+    const float4 n1 = a.nodes[4 * n + 1];
+    const float4 n2 = a.nodes[4 * n + 2];
+    const int4 n3 = reinterpret_cast<const int4*>(a.nodes)[4 * n + 3];
+    if (STATS) s.cn++;
+    float tl = 0.0f, tr = 0.0f;
+    bool hl = box_hit(s.cs, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, s.tmin, s.best_t, tl);
+    bool hr = n3.y != kDummyChild && box_hit(s.cs, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, s.tmin, s.best_t, tr);
+    if (hl && n3.x >= 0 && n3.x <= n) { hl = false; s.flags |= kBad; }
+    if (hr && n3.y >= 0 && n3.y <= n) { hr = false; s.flags |= kBad; }
+    if (hl && hr) {
+        const bool left_first = tl <= tr;
+        if (!st.push(left_first ? n3.y : n3.x)) s.flags |= kBad;
+        s.node = left_first ? n3.x : n3.y;
+    } else {
+        s.node = hl ? n3.x : (hr ? n3.y : kNoNode);
+    }
+    resolve_leaf<STATS>(a, s, st.sp);
+}
+
+// Ordered exchange of two (distance, link) pairs: the nearer first, the lower slot on a tie.
+__device__ __forceinline__ void order2(float& da, int& ca, float& db, int& cb) {
+    const bool sw = db < da;
+    const float d = sw ? db : da;
+    const int c = sw ? cb : ca;
+    db = sw ? da : db;
+    cb = sw ? ca : cb;
+    da = d;
+    ca = c;
+}
+
+// One node of the 4-wide tree (ImbaNode4 layout: 8 float4 = min_x, max_x, min_y, max_y, min_z,
+// max_z, children, count): the used child boxes are tested, the nearest hit child is next and
+// the other hit children are pushed, farthest deepest.
+template <int BLOCK, int STACK_LDS, bool STATS>
+__device__ __forceinline__ void node_step4(const TraceArgs& a, TraceState& s, LaneStack<BLOCK, STACK_LDS>& st) {
+    const int n = s.node;
+    if (n >= a.n_nodes) {
+        s.flags |= kBad;
+        s.node = kNoNode;
+        return;
+    }
+    const float4* p = a.nodes + 8 * (size_t)n;
+    const float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
+    const int4 ch = reinterpret_cast<const int4*>(p)[6];
+    if (STATS) s.cn++;
+    float d0, d1, d2, d3;
+    bool h0 = ch.x != 0 && box_hit(s.cs, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, s.tmin, s.best_t, d0);
+    bool h1 = ch.y != 0 && box_hit(s.cs, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, s.tmin, s.best_t, d1);
+    bool h2 = ch.z != 0 && box_hit(s.cs, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, s.tmin, s.best_t, d2);
+    bool h3 = ch.w != 0 && box_hit(s.cs, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, s.tmin, s.best_t, d3);
+    if (h0 && ch.x > 0 && ch.x <= n) { h0 = false; s.flags |= kBad; }
+    if (h1 && ch.y > 0 && ch.y <= n) { h1 = false; s.flags |= kBad; }
+    if (h2 && ch.z > 0 && ch.z <= n) { h2 = false; s.flags |= kBad; }
+    if (h3 && ch.w > 0 && ch.w <= n) { h3 = false; s.flags |= kBad; }
+    const int cnt = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+    const float inf = __int_as_float(0x7f800000);
+    float e0 = h0 ? d0 : inf, e1 = h1 ? d1 : inf, e2 = h2 ? d2 : inf, e3 = h3 ? d3 : inf;
+    int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+    // sorting network; misses (at infinity) end up last
+    order2(e0, c0, e1, c1);
+    order2(e2, c2, e3, c3);
+    order2(e0, c0, e2, c2);
+    order2(e1, c1, e3, c3);
+    order2(e1, c1, e2, c2);
+    if (cnt == 0) {
+        s.node = kNoNode;
+    } else {
+        bool ok = true;
+        if (cnt >= 4) ok &= st.push(c3);
+        if (cnt >= 3) ok &= st.push(c2);
+        if (cnt >= 2) ok &= st.push(c1);
+        if (!ok) s.flags |= kBad;
+        s.node = c0;
+    }
+    resolve_leaf<STATS>(a, s, st.sp);
+}
+
 // Advances one ray by one step: bookkeeping (next instance, back to world space, pop) if
 // nothing is pending, then one triangle test or one inner node.  Returns true when the ray
 // is finished.
-template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS>
+template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS, bool WIDE>
 __device__ __forceinline__ bool advance(const TraceArgs& a, TraceState& s, LaneStack<BLOCK, STACK_LDS>& st) {
     if (s.tri < 0 && s.node == kNoNode) {
         // the current instance's tree is exhausted: next instance of the leaf or world space
@@ -315,30 +403,8 @@ __device__ __forceinline__ bool advance(const TraceArgs& a, TraceState& s, LaneS
         return false;
     }
     if (s.node >= 0) {
-        const int n = s.node;
-        if (n >= a.n_nodes) {
-            s.flags |= kBad;
-            s.node = kNoNode;
-            return false;
-        }
-        const float4 n0 = a.nodes[4 * n + 0];
-        const float4 n1 = a.nodes[4 * n + 1];
-        const float4 n2 = a.nodes[4 * n + 2];
-        const int4 n3 = reinterpret_cast<const int4*>(a.nodes)[4 * n + 3];
-        if (STATS) s.cn++;
-        float tl = 0.0f, tr = 0.0f;
-        bool hl = box_hit(s.cs, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, s.tmin, s.best_t, tl);
-        bool hr = n3.y != kDummyChild && box_hit(s.cs, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, s.tmin, s.best_t, tr);
-        if (hl && n3.x >= 0 && n3.x <= n) { hl = false; s.flags |= kBad; }
-        if (hr && n3.y >= 0 && n3.y <= n) { hr = false; s.flags |= kBad; }
-        if (hl && hr) {
-            const bool left_first = tl <= tr;
-            if (!st.push(left_first ? n3.y : n3.x)) s.flags |= kBad;
-            s.node = left_first ? n3.x : n3.y;
-        } else {
-            s.node = hl ? n3.x : (hr ? n3.y : kNoNode);
-        }
-        resolve_leaf<STATS>(a, s, st.sp);
+        if (WIDE) node_step4<BLOCK, STACK_LDS, STATS>(a, s, st);
+        else node_step2<BLOCK, STACK_LDS, STATS>(a, s, st);
     }
     return false;
 }
@@ -366,7 +432,7 @@ __device__ __forceinline__ void write_result(const TraceArgs& a, const TraceStat
 }
 
 // One ray per thread, rays strided over a grid sized to the resident waves.
-template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS>
+template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     __shared__ int lds_stack[STACK_LDS * BLOCK];
     LaneStack<BLOCK, STACK_LDS> st;
@@ -377,7 +443,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
         TraceState s;
         state_init(s, a.rays[2 * (size_t)id], a.rays[2 * (size_t)id + 1], a.root);
         st.sp = 0;
-        while (!advance<BLOCK, STACK_LDS, ANY, MASKED, STATS>(a, s, st)) {
+        while (!advance<BLOCK, STACK_LDS, ANY, MASKED, STATS, WIDE>(a, s, st)) {
         }
         write_result<STATS>(a, s, id);
     }
@@ -388,7 +454,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
 // on a global counter (ballot + popcount + mbcnt).  Each iteration advances every active lane
 // by one step of `advance`, so the per-ray results are those of k_trace.
 // counter[0] = next ray, counter[1] = finished waves; the last wave to finish resets both.
-template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS>
+template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_trace_dyn(TraceArgs a, int refill) {
     __shared__ int lds_stack[STACK_LDS * BLOCK];
     LaneStack<BLOCK, STACK_LDS> st;
@@ -422,7 +488,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dyn(TraceArgs a, int refill) {
             if (exhausted) break;
             continue;
         }
-        if (id >= 0 && advance<BLOCK, STACK_LDS, ANY, MASKED, STATS>(a, s, st)) {
+        if (id >= 0 && advance<BLOCK, STACK_LDS, ANY, MASKED, STATS, WIDE>(a, s, st)) {
             write_result<STATS>(a, s, id);
             id = -1;
         }
@@ -472,7 +538,7 @@ __global__ void k_batch_prefix(BatchItem* items, BatchGroups groups, int* group_
     }
 }
 
-template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED>
+template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool WIDE>
 __global__ __launch_bounds__(BLOCK, (ANY || MASKED) ? 1 : 8) void k_trace_batch(const BatchItem* __restrict__ items, int first, int n_items,
                                                        const int* __restrict__ total_chunks, int* counter, int* spill,
                                                        int spill_depth) {
@@ -507,7 +573,7 @@ __global__ __launch_bounds__(BLOCK, (ANY || MASKED) ? 1 : 8) void k_trace_batch(
             TraceState s;
             state_init(s, a.rays[2 * (size_t)id], a.rays[2 * (size_t)id + 1], a.root);
             st.sp = 0;
-            while (!advance<BLOCK, STACK_LDS, ANY, MASKED, false>(a, s, st)) {
+            while (!advance<BLOCK, STACK_LDS, ANY, MASKED, false, WIDE>(a, s, st)) {
             }
             write_result<false>(a, s, id);
         }

@@ -11,6 +11,7 @@
 #include "common.h"
 #include "scene.h"
 #include "trace_kernels.h"
+#include "../host/wide_bvh.h"
 
 namespace imba {
 
@@ -52,14 +53,14 @@ int not_capturing(hipStream_t s, const char* what) {
     return IMBA_OK;
 }
 
-template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS>
+template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS, bool WIDE>
 int launch_one(const TraceArgs& args_in, int max_stack, int blocks_per_cu, int refill, hipStream_t s) {
     TraceArgs a = args_in;
     const DeviceProps* p = device_props();
     if (!p) return fail(IMBA_ERR_NO_DEVICE, "no HIP device");
     const bool dyn = refill > 0;
-    void (*kern)(TraceArgs) = k_trace<BLOCK, STACK_LDS, ANY, MASKED, STATS>;
-    void (*kern_dyn)(TraceArgs, int) = k_trace_dyn<BLOCK, STACK_LDS, ANY, MASKED, STATS>;
+    void (*kern)(TraceArgs) = k_trace<BLOCK, STACK_LDS, ANY, MASKED, STATS, WIDE>;
+    void (*kern_dyn)(TraceArgs, int) = k_trace_dyn<BLOCK, STACK_LDS, ANY, MASKED, STATS, WIDE>;
     int resident = 0;
     if (dyn)
         IMBA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern_dyn, BLOCK, 0));
@@ -104,18 +105,25 @@ int launch_one(const TraceArgs& args_in, int max_stack, int blocks_per_cu, int r
 // fastest on both the 65K-triangle and the 21M-triangle workloads).
 constexpr int kDefaultBlock = 256;
 constexpr int kDefaultStack = 16;
+constexpr int kDefaultWidth = 2;
 
-template <bool ANY, bool MASKED, bool STATS>
+// Tree width a configuration asks for (2 or 4); 0 for an invalid value.
+int width_of(const ImbaTraceConfig* cfg) {
+    const int w = cfg && cfg->bvh_width ? cfg->bvh_width : kDefaultWidth;
+    return (w == 2 || w == 4) ? w : 0;
+}
+
+template <bool ANY, bool MASKED, bool STATS, bool WIDE>
 int dispatch_shape(const TraceArgs& a, int max_stack, const ImbaTraceConfig* cfg, hipStream_t s) {
     const int block = cfg && cfg->block_size ? cfg->block_size : kDefaultBlock;
     const int stack = cfg && cfg->stack_lds ? cfg->stack_lds : kDefaultStack;
     const int bpc = cfg ? cfg->blocks_per_cu : 0;
     const int refill = cfg ? cfg->persistent : 0;
     if (STATS) {  // visit counts do not depend on the launch shape
-        return launch_one<kDefaultBlock, kDefaultStack, ANY, MASKED, STATS>(a, max_stack, 0, 0, s);
+        return launch_one<kDefaultBlock, kDefaultStack, ANY, MASKED, STATS, WIDE>(a, max_stack, 0, 0, s);
     } else {
 #define IMBA_SHAPE(B, S) \
-    if (block == B && stack == S) return launch_one<B, S, ANY, MASKED, STATS>(a, max_stack, bpc, refill, s);
+    if (block == B && stack == S) return launch_one<B, S, ANY, MASKED, STATS, WIDE>(a, max_stack, bpc, refill, s);
         IMBA_SHAPE(256, 16)
         IMBA_SHAPE(256, 8)
         IMBA_SHAPE(256, 32)
@@ -129,12 +137,28 @@ int dispatch_shape(const TraceArgs& a, int max_stack, const ImbaTraceConfig* cfg
                 "(supported: 256/16, 256/8, 256/32, 128/16, 128/32, 64/32, 64/64)", block, stack);
 }
 
-int launch_trace(const ImbaSceneDesc& d, const SceneInfo& info, int* d_err, const ImbaRay* rays, ImbaHit* hits,
-                 uint32_t* counts, int count, bool any, bool masked, const ImbaTraceConfig* cfg, void* stream,
-                 const int* count_dev = nullptr) {
+template <bool STATS, bool WIDE>
+int dispatch_flags(const TraceArgs& a, int max_stack, bool any, bool masked, const ImbaTraceConfig* cfg,
+                   hipStream_t s) {
+    return any ? (masked ? dispatch_shape<true, true, STATS, WIDE>(a, max_stack, cfg, s)
+                         : dispatch_shape<true, false, STATS, WIDE>(a, max_stack, cfg, s))
+               : (masked ? dispatch_shape<false, true, STATS, WIDE>(a, max_stack, cfg, s)
+                         : dispatch_shape<false, false, STATS, WIDE>(a, max_stack, cfg, s));
+}
+
+// Scene arrays of one tree width: the BVH2 as given or the derived 4-wide tree.
+struct TreeView {
+    const ImbaSceneDesc* d;
+    const SceneInfo* info;
+    bool wide;
+};
+
+int launch_trace(const TreeView& t, int* d_err, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts, int count,
+                 bool any, bool masked, const ImbaTraceConfig* cfg, void* stream, const int* count_dev = nullptr) {
     if (count < 0) return fail(IMBA_ERR_INVALID_ARG, "count < 0");
     if (count == 0) return IMBA_OK;
     if (!rays || !hits) return fail(IMBA_ERR_INVALID_ARG, "rays or hits is NULL");
+    const ImbaSceneDesc& d = *t.d;
     TraceArgs a{};
     a.count_dev = count_dev;
     a.nodes = (const float4*)d.nodes;       a.n_nodes = d.node_count;
@@ -152,21 +176,25 @@ int launch_trace(const ImbaSceneDesc& d, const SceneInfo& info, int* d_err, cons
     a.err = d_err;
     bool sync;
     hipStream_t s = resolve_stream(stream, &sync);
+    const int ms = t.info->max_stack;
     int r;
-    const bool stats = counts != nullptr;
-    if (stats) {
-        r = any ? (masked ? dispatch_shape<true, true, true>(a, info.max_stack, nullptr, s)
-                          : dispatch_shape<true, false, true>(a, info.max_stack, nullptr, s))
-                : (masked ? dispatch_shape<false, true, true>(a, info.max_stack, nullptr, s)
-                          : dispatch_shape<false, false, true>(a, info.max_stack, nullptr, s));
+    if (counts) {
+        r = t.wide ? dispatch_flags<true, true>(a, ms, any, masked, nullptr, s)
+                   : dispatch_flags<true, false>(a, ms, any, masked, nullptr, s);
     } else {
-        r = any ? (masked ? dispatch_shape<true, true, false>(a, info.max_stack, cfg, s)
-                          : dispatch_shape<true, false, false>(a, info.max_stack, cfg, s))
-                : (masked ? dispatch_shape<false, true, false>(a, info.max_stack, cfg, s)
-                          : dispatch_shape<false, false, false>(a, info.max_stack, cfg, s));
+        r = t.wide ? dispatch_flags<false, true>(a, ms, any, masked, cfg, s)
+                   : dispatch_flags<false, false>(a, ms, any, masked, cfg, s);
     }
     if (r) return r;
     if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
+    return IMBA_OK;
+}
+
+// The tree of `s` that `cfg` asks for.
+int tree_of(const ImbaScene* s, const ImbaTraceConfig* cfg, TreeView* t) {
+    const int w = width_of(cfg);
+    if (!w) return fail(IMBA_ERR_INVALID_ARG, "bvh_width must be 0, 2 or 4");
+    *t = w == 4 ? TreeView{&s->dev4, &s->info4, true} : TreeView{&s->dev, &s->info, false};
     return IMBA_OK;
 }
 
@@ -188,8 +216,17 @@ struct FfiKey {
 struct FfiScene {
     ImbaSceneDesc dev;
     SceneInfo info;
+    ImbaSceneDesc dev4;  // derived 4-wide tree (nodes and instance records owned)
+    SceneInfo info4;
     int* d_err = nullptr;
 };
+
+int ffi_tree(FfiScene* fs, const ImbaTraceConfig* cfg, TreeView* t) {
+    const int w = width_of(cfg);
+    if (!w) return fail(IMBA_ERR_INVALID_ARG, "bvh_width must be 0, 2 or 4");
+    *t = w == 4 ? TreeView{&fs->dev4, &fs->info4, true} : TreeView{&fs->dev, &fs->info, false};
+    return IMBA_OK;
+}
 std::mutex g_ffi_mutex;
 std::map<FfiKey, FfiScene> g_ffi_scenes;
 
@@ -259,6 +296,9 @@ int ffi_scene(const ImbaSceneDesc& given, FfiScene** out) {
     fs.dev.texcoords = (const ImbaVec2*)texcoords;
     fs.dev.masks = (const ImbaTransparencyMask*)masks;
     fs.dev.mask_buffer = (const uint8_t*)mask_buffer;
+    size_t wide_bytes = 0;
+    r = upload_wide(h, fs.dev, fs.info, &fs.dev4, &fs.info4, &wide_bytes);
+    if (r) return r;
     IMBA_HIP_TRY(hipMalloc((void**)&fs.d_err, sizeof(int)));
     IMBA_HIP_TRY(hipMemset(fs.d_err, 0, sizeof(int)));
     auto ins = g_ffi_scenes.emplace(key, fs);
@@ -279,7 +319,9 @@ void ffi_trace(bool any, int root, void* nodes, void* instances, void* tris, voi
     given.masks = (const ImbaTransparencyMask*)masks;
     given.mask_buffer = (const uint8_t*)mask_buffer;
     int r = ffi_scene(given, &fs);
-    if (!r) r = launch_trace(fs->dev, fs->info, fs->d_err, (const ImbaRay*)rays, (ImbaHit*)hits, nullptr, count, any,
+    TreeView t{};
+    if (!r) r = ffi_tree(fs, nullptr, &t);
+    if (!r) r = launch_trace(t, fs->d_err, (const ImbaRay*)rays, (ImbaHit*)hits, nullptr, count, any,
                              !fs->info.all_opaque, nullptr, nullptr);
     if (r) fprintf(stderr, "imbatracer_amd: %s_gpu_masked_instanced failed: %s\n", any ? "occluded" : "intersect",
                    imba_last_error());
@@ -309,9 +351,9 @@ struct ImbaTraceBatch {
     int device = 0;
     int n = 0;
     imba::BatchGroups groups{};
-    int max_stack[4] = {0, 0, 0, 0};
+    int max_stack[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};  // per tree width (2, 4) and group
     long long capacity_chunks[4] = {0, 0, 0, 0};
-    imba::BatchItem* d_items = nullptr;
+    imba::BatchItem* d_items[2] = {nullptr, nullptr};      // queue tables for the BVH2 and the 4-wide tree
     int* d_group_chunks = nullptr;  // 4
     int* d_counters = nullptr;      // 2 per group
     imba::Scratch spill[4];
@@ -320,11 +362,12 @@ struct ImbaTraceBatch {
 namespace imba {
 namespace {
 
-template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED>
+template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool WIDE>
 int launch_batch_group(ImbaTraceBatch* b, int g, int blocks_per_cu, hipStream_t s) {
     const DeviceProps* p = device_props();
     if (!p) return fail(IMBA_ERR_NO_DEVICE, "no HIP device");
-    auto kern = k_trace_batch<BLOCK, STACK_LDS, ANY, MASKED>;
+    auto kern = k_trace_batch<BLOCK, STACK_LDS, ANY, MASKED, WIDE>;
+    const int max_stack = b->max_stack[WIDE ? 1 : 0][g];
     int resident = 0;
     IMBA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern, BLOCK, 0));
     if (resident <= 0) resident = 1;
@@ -333,8 +376,8 @@ int launch_batch_group(ImbaTraceBatch* b, int g, int blocks_per_cu, hipStream_t 
     const int grid = (int)std::max<long long>(1, std::min<long long>(needed, (long long)resident * p->cu_count));
     int* spill = nullptr;
     int spill_depth = 0;
-    if (b->max_stack[g] > STACK_LDS) {
-        spill_depth = b->max_stack[g] - STACK_LDS;
+    if (max_stack > STACK_LDS) {
+        spill_depth = max_stack - STACK_LDS;
         const size_t need = (size_t)spill_depth * grid * BLOCK * sizeof(int);
         if (b->spill[g].bytes < need) {
             int r = not_capturing(s, "a larger spill stack");
@@ -344,20 +387,20 @@ int launch_batch_group(ImbaTraceBatch* b, int g, int blocks_per_cu, hipStream_t 
         }
         spill = (int*)b->spill[g].ptr;
     }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, s, b->d_items, b->groups.first[g],
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, s, b->d_items[WIDE ? 1 : 0], b->groups.first[g],
                        b->groups.first[g + 1] - b->groups.first[g], b->d_group_chunks + g, b->d_counters + 2 * g,
                        spill, spill_depth);
     IMBA_HIP_TRY(hipGetLastError());
     return IMBA_OK;
 }
 
-template <bool ANY, bool MASKED>
+template <bool ANY, bool MASKED, bool WIDE>
 int dispatch_batch(ImbaTraceBatch* b, int g, const ImbaTraceConfig* cfg, hipStream_t s) {
     const int block = cfg && cfg->block_size ? cfg->block_size : kDefaultBlock;
     const int stack = cfg && cfg->stack_lds ? cfg->stack_lds : kDefaultStack;
     const int bpc = cfg ? cfg->blocks_per_cu : 0;
 #define IMBA_SHAPE(B, S) \
-    if (block == B && stack == S) return launch_batch_group<B, S, ANY, MASKED>(b, g, bpc, s);
+    if (block == B && stack == S) return launch_batch_group<B, S, ANY, MASKED, WIDE>(b, g, bpc, s);
     IMBA_SHAPE(256, 16)
     IMBA_SHAPE(256, 8)
     IMBA_SHAPE(256, 32)
@@ -372,7 +415,8 @@ int dispatch_batch(ImbaTraceBatch* b, int g, const ImbaTraceConfig* cfg, hipStre
 
 void free_batch(ImbaTraceBatch* b) {
     if (!b) return;
-    hipFree(b->d_items);
+    hipFree(b->d_items[0]);
+    hipFree(b->d_items[1]);
     hipFree(b->d_group_chunks);
     hipFree(b->d_counters);
     for (auto& sp : b->spill) hipFree(sp.ptr);
@@ -390,8 +434,9 @@ int imba_trace_batch_create(const ImbaTraceItem* items, int n_items, ImbaTraceBa
     IMBA_CHECK_ARG(out && (items || n_items == 0), "NULL argument");
     IMBA_CHECK_ARG(n_items >= 0, "n_items < 0");
     *out = nullptr;
-    std::vector<BatchItem> table;
+    std::vector<BatchItem> table, table4;
     table.reserve(n_items);
+    table4.reserve(n_items);
     auto* b = new ImbaTraceBatch();
     IMBA_HIP_TRY(hipGetDevice(&b->device));
     // group-contiguous table, queues in the caller's order within a group
@@ -418,19 +463,30 @@ int imba_trace_batch_create(const ImbaTraceItem* items, int n_items, ImbaTraceBa
             e.count_dev = it.count_dev;
             e.capacity = it.count;
             e.group = g;
-            b->max_stack[g] = std::max(b->max_stack[g], it.scene->info.max_stack);
+            b->max_stack[0][g] = std::max(b->max_stack[0][g], it.scene->info.max_stack);
+            b->max_stack[1][g] = std::max(b->max_stack[1][g], it.scene->info4.max_stack);
             b->capacity_chunks[g] += ceil_div(it.count, 64);
             table.push_back(e);
+            const TraceArgs a4 = scene_args(it.scene->dev4, it.scene->d_error);
+            e.a.nodes = a4.nodes;
+            e.a.n_nodes = a4.n_nodes;
+            e.a.inst = a4.inst;
+            e.a.n_inst = a4.n_inst;
+            e.a.root = a4.root;
+            table4.push_back(e);
         }
     }
     b->groups.first[4] = (int)table.size();
     b->n = (int)table.size();
     hipError_t e = hipSuccess;
-    if (!table.empty()) e = hipMalloc((void**)&b->d_items, sizeof(BatchItem) * table.size());
+    if (!table.empty()) e = hipMalloc((void**)&b->d_items[0], sizeof(BatchItem) * table.size());
+    if (e == hipSuccess && !table.empty()) e = hipMalloc((void**)&b->d_items[1], sizeof(BatchItem) * table.size());
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_group_chunks, sizeof(int) * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_counters, sizeof(int) * 8);
     if (e == hipSuccess && !table.empty())
-        e = hipMemcpy(b->d_items, table.data(), sizeof(BatchItem) * table.size(), hipMemcpyHostToDevice);
+        e = hipMemcpy(b->d_items[0], table.data(), sizeof(BatchItem) * table.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !table.empty())
+        e = hipMemcpy(b->d_items[1], table4.data(), sizeof(BatchItem) * table4.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(b->d_counters, 0, sizeof(int) * 8);
     if (e == hipSuccess) e = hipMemset(b->d_group_chunks, 0, sizeof(int) * 4);
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -449,18 +505,26 @@ int imba_trace_batch_launch(ImbaTraceBatch* b, const ImbaTraceConfig* cfg, void*
     IMBA_HIP_TRY(hipGetDevice(&dev));
     if (dev != b->device) return fail(IMBA_ERR_INVALID_ARG, "batch belongs to device %d, current is %d", b->device, dev);
     if (b->n == 0) return IMBA_OK;
+    const int width = width_of(cfg);
+    if (!width) return fail(IMBA_ERR_INVALID_ARG, "bvh_width must be 0, 2 or 4");
+    const bool wide = width == 4;
     bool sync;
     hipStream_t s = resolve_stream(stream, &sync);
-    hipLaunchKernelGGL(k_batch_prefix, dim3(1), dim3(64), 0, s, b->d_items, b->groups, b->d_group_chunks);
+    hipLaunchKernelGGL(k_batch_prefix, dim3(1), dim3(64), 0, s, b->d_items[wide ? 1 : 0], b->groups,
+                       b->d_group_chunks);
     IMBA_HIP_TRY(hipGetLastError());
     for (int g = 0; g < 4; g++) {
         if (b->groups.first[g + 1] == b->groups.first[g] || b->capacity_chunks[g] == 0) continue;
         int r;
-        switch (g) {
-            case 0: r = dispatch_batch<false, false>(b, g, cfg, s); break;
-            case 1: r = dispatch_batch<false, true>(b, g, cfg, s); break;
-            case 2: r = dispatch_batch<true, false>(b, g, cfg, s); break;
-            default: r = dispatch_batch<true, true>(b, g, cfg, s); break;
+        switch (g + 4 * wide) {
+            case 0: r = dispatch_batch<false, false, false>(b, g, cfg, s); break;
+            case 1: r = dispatch_batch<false, true, false>(b, g, cfg, s); break;
+            case 2: r = dispatch_batch<true, false, false>(b, g, cfg, s); break;
+            case 3: r = dispatch_batch<true, true, false>(b, g, cfg, s); break;
+            case 4: r = dispatch_batch<false, false, true>(b, g, cfg, s); break;
+            case 5: r = dispatch_batch<false, true, true>(b, g, cfg, s); break;
+            case 6: r = dispatch_batch<true, false, true>(b, g, cfg, s); break;
+            default: r = dispatch_batch<true, true, true>(b, g, cfg, s); break;
         }
         if (r) return r;
     }
@@ -474,6 +538,7 @@ int imba_autotune(const ImbaScene* sc, const ImbaRay* rays, ImbaHit* hits, int c
     IMBA_CHECK_ARG(count > 0 && reps > 0, "count and reps must be positive");
     static const int kShapes[][2] = {{256, 16}, {256, 8}, {256, 32}, {128, 16}, {128, 32}, {64, 32}, {64, 64}};
     static const int kRefill[] = {0, 32};
+    static const int kWidth[] = {2, 4};
     bool sync;
     hipStream_t s = resolve_stream(stream, &sync);
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -481,16 +546,20 @@ int imba_autotune(const ImbaScene* sc, const ImbaRay* rays, ImbaHit* hits, int c
     IMBA_HIP_TRY(hipEventCreate(&e1));
     float best_t = -1.0f;
     int r = IMBA_OK;
+    for (int width : kWidth)
     for (const auto& shape : kShapes) {
         for (int refill : kRefill) {
-            ImbaTraceConfig cfg{shape[0], shape[1], refill, 0};
-            r = launch_trace(sc->dev, sc->info, sc->d_error, rays, hits, nullptr, count, any_hit != 0,
-                             !sc->info.all_opaque, &cfg, s);
+            ImbaTraceConfig cfg{shape[0], shape[1], refill, 0, width};
+            TreeView t{};
+            r = tree_of(sc, &cfg, &t);
+            if (!r)
+                r = launch_trace(t, sc->d_error, rays, hits, nullptr, count, any_hit != 0, !sc->info.all_opaque, &cfg,
+                                 s);
             if (r) break;
             hipEventRecord(e0, s);
             for (int k = 0; k < reps && !r; k++)
-                r = launch_trace(sc->dev, sc->info, sc->d_error, rays, hits, nullptr, count, any_hit != 0,
-                                 !sc->info.all_opaque, &cfg, s);
+                r = launch_trace(t, sc->d_error, rays, hits, nullptr, count, any_hit != 0, !sc->info.all_opaque, &cfg,
+                                 s);
             if (r) break;
             hipEventRecord(e1, s);
             float ms = 0.0f;
@@ -521,44 +590,55 @@ int imba_trace_batch_release(ImbaTraceBatch* b) {
     return IMBA_OK;
 }
 
+static int trace_entry(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts, int count, bool any,
+                       const ImbaTraceConfig* cfg, void* stream, const int* count_dev) {
+    IMBA_CHECK_ARG(s, "scene is NULL");
+    TreeView t{};
+    int r = tree_of(s, cfg, &t);
+    if (r) return r;
+    return launch_trace(t, s->d_error, rays, hits, counts, count, any, !s->info.all_opaque, cfg, stream, count_dev);
+}
+
 int imba_intersect(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int count, const ImbaTraceConfig* cfg,
                    void* stream) {
-    IMBA_CHECK_ARG(s, "scene is NULL");
-    return launch_trace(s->dev, s->info, s->d_error, rays, hits, nullptr, count, false, !s->info.all_opaque, cfg, stream);
+    return trace_entry(s, rays, hits, nullptr, count, false, cfg, stream, nullptr);
 }
 
 int imba_occluded(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int count, const ImbaTraceConfig* cfg,
                   void* stream) {
-    IMBA_CHECK_ARG(s, "scene is NULL");
-    return launch_trace(s->dev, s->info, s->d_error, rays, hits, nullptr, count, true, !s->info.all_opaque, cfg, stream);
+    return trace_entry(s, rays, hits, nullptr, count, true, cfg, stream, nullptr);
 }
 
 int imba_intersect_dev(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int capacity, const int* count_dev,
                        const ImbaTraceConfig* cfg, void* stream) {
-    IMBA_CHECK_ARG(s && count_dev, "scene or count_dev is NULL");
-    return launch_trace(s->dev, s->info, s->d_error, rays, hits, nullptr, capacity, false, !s->info.all_opaque, cfg,
-                        stream, count_dev);
+    IMBA_CHECK_ARG(count_dev, "count_dev is NULL");
+    return trace_entry(s, rays, hits, nullptr, capacity, false, cfg, stream, count_dev);
 }
 
 int imba_occluded_dev(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int capacity, const int* count_dev,
                       const ImbaTraceConfig* cfg, void* stream) {
-    IMBA_CHECK_ARG(s && count_dev, "scene or count_dev is NULL");
-    return launch_trace(s->dev, s->info, s->d_error, rays, hits, nullptr, capacity, true, !s->info.all_opaque, cfg,
-                        stream, count_dev);
+    IMBA_CHECK_ARG(count_dev, "count_dev is NULL");
+    return trace_entry(s, rays, hits, nullptr, capacity, true, cfg, stream, count_dev);
 }
 
 int imba_intersect_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts, int count,
                          void* stream) {
-    IMBA_CHECK_ARG(s && counts, "NULL argument");
-    return launch_trace(s->dev, s->info, s->d_error, rays, hits, counts, count, false, !s->info.all_opaque, nullptr,
-                        stream);
+    IMBA_CHECK_ARG(counts, "counts is NULL");
+    const ImbaTraceConfig bvh2{0, 0, 0, 0, 2};
+    return trace_entry(s, rays, hits, counts, count, false, &bvh2, stream, nullptr);
 }
 
 int imba_occluded_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts, int count,
                         void* stream) {
-    IMBA_CHECK_ARG(s && counts, "NULL argument");
-    return launch_trace(s->dev, s->info, s->d_error, rays, hits, counts, count, true, !s->info.all_opaque, nullptr,
-                        stream);
+    IMBA_CHECK_ARG(counts, "counts is NULL");
+    const ImbaTraceConfig bvh2{0, 0, 0, 0, 2};
+    return trace_entry(s, rays, hits, counts, count, true, &bvh2, stream, nullptr);
+}
+
+int imba_trace_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts, int count,
+                     int any_hit, const ImbaTraceConfig* cfg, void* stream) {
+    IMBA_CHECK_ARG(counts, "counts is NULL");
+    return trace_entry(s, rays, hits, counts, count, any_hit != 0, cfg, stream, nullptr);
 }
 
 int imba_trace_desc(const ImbaSceneDesc* dev, const ImbaRay* rays, ImbaHit* hits, int count, int any_hit,
@@ -567,8 +647,10 @@ int imba_trace_desc(const ImbaSceneDesc* dev, const ImbaRay* rays, ImbaHit* hits
     FfiScene* fs = nullptr;
     int r = ffi_scene(*dev, &fs);
     if (r) return r;
-    return launch_trace(fs->dev, fs->info, fs->d_err, rays, hits, nullptr, count, any_hit != 0, !fs->info.all_opaque,
-                        nullptr, stream);
+    TreeView t{};
+    r = ffi_tree(fs, nullptr, &t);
+    if (r) return r;
+    return launch_trace(t, fs->d_err, rays, hits, nullptr, count, any_hit != 0, !fs->info.all_opaque, nullptr, stream);
 }
 
 // Reference FFI (render/scheduling/ray_queue.h:297-308, 347-358): synchronous, device pointers.

@@ -11,11 +11,14 @@
 //     use the 1x1 texel at offset 0 (core/mask.h:22-45)
 #include <algorithm>
 #include <atomic>
+#include <cstring>
 #include <memory>
+#include <string>
 #include <thread>
 #include <vector>
 
 #include "bvh_build.h"
+#include "bvh_cache.h"
 #include "imba_hip.h"
 #include "layout.h"
 #include "status.h"
@@ -46,6 +49,12 @@ struct MeshBvh {
     std::vector<ImbaVec4> tris;
     bool single_leaf = false;
 };
+
+// Loads mesh m's tree from its cache file (links relative to the mesh, global triangle ids).
+bool load_cached(const std::string& path, int type, int tri_base, std::vector<char>& nodes, std::vector<ImbaVec4>& tris) {
+    if (path.empty() || !cache_has_block(path.c_str(), type)) return false;
+    return cache_load(path.c_str(), type, 0, 0, tri_base, nodes, tris) == IMBA_OK;
+}
 
 void build_mesh_bvh(const MeshData& mesh, int builder, int tri_base, MeshBvh& out) {
     const TriSource src = mesh.source();
@@ -148,6 +157,8 @@ struct ImbaSceneBuild {
     std::vector<float> geom_normals;  // 3 per global triangle (object space)
     std::vector<int> tri_materials;
     std::vector<int> tri_layout;      // first global triangle id of each mesh
+    std::vector<std::string> cache;   // per mesh: cache file or ""
+    int cache_loaded = 0, cache_built = 0;
 
     // CPU layout (imba_build_cpu_layout)
     bool cpu_built = false;
@@ -266,7 +277,24 @@ int imba_build_run(ImbaSceneBuild* b, int builder, int threads) {
 
     // Mesh BVHs, in parallel over meshes; each one is built with mesh-local links.
     std::vector<MeshBvh> per_mesh(nm);
-    parallel_over(nm, threads, [&](int m) { build_mesh_bvh(b->meshes[m], builder, tri_layout[m], per_mesh[m]); });
+    std::atomic<int> loaded(0);
+    b->cache.resize(nm);
+    parallel_over(nm, threads, [&](int m) {
+        std::vector<char> raw;
+        if (load_cached(b->cache[m], IMBA_CACHE_BVH2, tri_layout[m], raw, per_mesh[m].tris)) {
+            per_mesh[m].nodes.resize(raw.size() / sizeof(ImbaNode));
+            std::memcpy(per_mesh[m].nodes.data(), raw.data(), raw.size());
+            per_mesh[m].single_leaf = !per_mesh[m].nodes.empty() && per_mesh[m].nodes[0].right == IMBA_DUMMY_CHILD;
+            loaded++;
+            return;
+        }
+        build_mesh_bvh(b->meshes[m], builder, tri_layout[m], per_mesh[m]);
+        if (!b->cache[m].empty())
+            cache_store(b->cache[m].c_str(), IMBA_CACHE_BVH2, per_mesh[m].nodes.data(), (int)per_mesh[m].nodes.size(),
+                        per_mesh[m].tris.data(), (int)per_mesh[m].tris.size(), 0, 0, tri_layout[m]);
+    });
+    b->cache_loaded = loaded;
+    b->cache_built = nm - loaded;
 
     // Concatenate with global links (what one writer appending all meshes would produce).
     b->nodes.clear();
@@ -334,7 +362,23 @@ int imba_build_cpu_layout(ImbaSceneBuild* b, int builder, int threads) {
     if (!b->built) return fail(IMBA_ERR_INVALID_ARG, "imba_build_run has not been called");
     const int nm = (int)b->meshes.size();
     std::vector<MeshBvh4> per_mesh(nm);
-    parallel_over(nm, threads, [&](int m) { build_mesh_bvh4(b->meshes[m], builder, b->tri_layout[m], per_mesh[m]); });
+    std::atomic<int> loaded(0);
+    b->cache.resize(nm);
+    parallel_over(nm, threads, [&](int m) {
+        std::vector<char> raw;
+        if (load_cached(b->cache[m], IMBA_CACHE_BVH4, b->tri_layout[m], raw, per_mesh[m].tris)) {
+            per_mesh[m].nodes.resize(raw.size() / sizeof(ImbaNode4));
+            std::memcpy(per_mesh[m].nodes.data(), raw.data(), raw.size());
+            loaded++;
+            return;
+        }
+        build_mesh_bvh4(b->meshes[m], builder, b->tri_layout[m], per_mesh[m]);
+        if (!b->cache[m].empty())
+            cache_store(b->cache[m].c_str(), IMBA_CACHE_BVH4, per_mesh[m].nodes.data(), (int)per_mesh[m].nodes.size(),
+                        per_mesh[m].tris.data(), (int)per_mesh[m].tris.size(), 0, 0, b->tri_layout[m]);
+    });
+    b->cache_loaded = loaded;
+    b->cache_built = nm - loaded;
 
     // Concatenate with global links: node children > 0 shift by the mesh's first node, leaf
     // links by its first Vec4; 0 (unused slot) stays.
@@ -374,6 +418,21 @@ int imba_build_cpu_layout(ImbaSceneBuild* b, int builder, int threads) {
     b->cpu_nodes.insert(b->cpu_nodes.end(), top.begin(), top.end());
     b->cpu_root = node_count;
     b->cpu_built = true;
+    return IMBA_OK;
+}
+
+int imba_build_set_cache(ImbaSceneBuild* b, int mesh_id, const char* path) {
+    IMBA_CHECK_ARG(b, "NULL argument");
+    IMBA_CHECK_ARG(mesh_id >= 0 && mesh_id < (int)b->meshes.size(), "mesh_id out of range");
+    if ((int)b->cache.size() < (int)b->meshes.size()) b->cache.resize(b->meshes.size());
+    b->cache[mesh_id] = path ? path : "";
+    return IMBA_OK;
+}
+
+int imba_build_cache_stats(const ImbaSceneBuild* b, int* loaded, int* built) {
+    IMBA_CHECK_ARG(b && loaded && built, "NULL argument");
+    *loaded = b->cache_loaded;
+    *built = b->cache_built;
     return IMBA_OK;
 }
 

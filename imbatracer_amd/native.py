@@ -40,6 +40,7 @@ LEAF_END_BITS = 0x80000000
 BUILDER_SBVH = 0
 BUILDER_BINNED_SAH = 1
 COST_GPU_MESH, COST_GPU_TOP, COST_CPU = 0, 1, 2
+CACHE_BVH2, CACHE_BVH4 = 1, 2
 
 IMBA_OK = 0
 ERRORS = {1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "bad BVH", 5: "capacity",
@@ -82,7 +83,8 @@ class MeshDesc(C.Structure):
 
 
 class TraceConfig(C.Structure):
-    _fields_ = [("block_size", C.c_int), ("stack_lds", C.c_int), ("persistent", C.c_int), ("blocks_per_cu", C.c_int)]
+    _fields_ = [("block_size", C.c_int), ("stack_lds", C.c_int), ("persistent", C.c_int), ("blocks_per_cu", C.c_int),
+                ("bvh_width", C.c_int)]
 
 
 class TraceItem(C.Structure):
@@ -136,6 +138,11 @@ _sig("imba_build_get_desc", i32, vp, C.POINTER(SceneDesc))
 _sig("imba_build_get_tri_attrs", i32, vp, C.POINTER(vp), C.POINTER(vp), ip)
 _sig("imba_build_cpu_layout", i32, vp, i32, i32)
 _sig("imba_build_get_cpu_desc", i32, vp, C.POINTER(CpuSceneDesc))
+_sig("imba_bvh_store", i32, C.c_char_p, i32, vp, i32, vp, i32, i32, i32, i32)
+_sig("imba_bvh_load", i32, C.c_char_p, i32, i32, i32, i32, vp, i32, vp, i32, ip, ip)
+_sig("imba_bvh_has_block", i32, C.c_char_p, i32)
+_sig("imba_build_set_cache", i32, vp, i32, C.c_char_p)
+_sig("imba_build_cache_stats", i32, vp, ip, ip)
 _sig("imba_bvh_emit_mesh", i32, C.POINTER(MeshDesc), i32, i32, i32, vp, sz, C.POINTER(sz))
 _sig("imba_bvh_emit_boxes", i32, vp, vp, i32, i32, i32, vp, sz, C.POINTER(sz))
 _sig("imba_scene_validate", i32, C.POINTER(SceneDesc), ip, ip)
@@ -151,6 +158,10 @@ _sig("imba_occluded_dev", i32, vp, vp, vp, i32, vp, C.POINTER(TraceConfig), vp)
 _sig("imba_intersect_stats", i32, vp, vp, vp, vp, i32, vp)
 _sig("imba_occluded_stats", i32, vp, vp, vp, vp, i32, vp)
 _sig("imba_trace_error_count", i32, vp, ip)
+_sig("imba_trace_stats", i32, vp, vp, vp, vp, i32, i32, C.POINTER(TraceConfig), vp)
+_sig("imba_wide_create", i32, C.POINTER(SceneDesc), C.POINTER(vp))
+_sig("imba_wide_get", i32, vp, C.POINTER(vp), ip, C.POINTER(vp), ip, ip, ip)
+_sig("imba_wide_release", i32, vp)
 _sig("imba_trace_batch_create", i32, C.POINTER(TraceItem), i32, C.POINTER(vp))
 _sig("imba_trace_batch_launch", i32, vp, C.POINTER(TraceConfig), vp)
 _sig("imba_trace_batch_release", i32, vp)
@@ -163,6 +174,7 @@ _sig("imba_compact_hits", i32, vp, vp, vp, i32, i32, vp, vp, vp, ip, vp)
 _sig("imba_compact_rays", i32, vp, vp, i32, i32, vp, vp, ip, vp)
 _sig("imba_compact_workspace_size", sz, i32)
 _sig("imba_compact_rays_dev", i32, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp)
+_sig("imba_sort_by_material", i32, vp, i32, vp, vp, i32, i32, vp, vp, vp)
 _sig("imba_hashgrid_create", i32, C.POINTER(vp))
 _sig("imba_hashgrid_release", i32, vp)
 _sig("imba_hashgrid_build", i32, vp, vp, i32, i32, i32, f32, vp)
@@ -213,12 +225,12 @@ class DeviceBuffer:
     def upload(self, a: np.ndarray, stream=None, offset: int = 0):
         a = np.ascontiguousarray(a)
         assert offset + a.nbytes <= self.nbytes
-        check(lib.imba_memcpy_h2d(C.c_void_p(self.ptr.value + offset), ptr(a), a.nbytes, stream), "h2d")
+        check(lib.imba_memcpy_h2d(C.c_void_p(self.ptr.value + offset), ptr(a), a.nbytes, _s(stream)), "h2d")
 
     def download(self, dtype, count: int, stream=None, offset: int = 0) -> np.ndarray:
         out = np.empty(count, dtype=dtype)
         assert offset + out.nbytes <= self.nbytes
-        check(lib.imba_memcpy_d2h(ptr(out), C.c_void_p(self.ptr.value + offset), out.nbytes, stream), "d2h")
+        check(lib.imba_memcpy_d2h(ptr(out), C.c_void_p(self.ptr.value + offset), out.nbytes, _s(stream)), "d2h")
         return out
 
     def at(self, offset: int):

@@ -135,6 +135,17 @@ class SceneBuild:
             "mask_buffer": arr(d.mask_buffer, d.mask_buffer_size, np.uint8),
         }
 
+    def set_cache(self, mesh_id: int, path):
+        """Cache file of a mesh's acceleration structures (loaded if present, else written)."""
+        check(N.lib.imba_build_set_cache(self.h, mesh_id, None if path is None else str(path).encode()),
+              "imba_build_set_cache")
+
+    def cache_stats(self):
+        """(meshes loaded from cache files, meshes built) by the last run / cpu_layout."""
+        lo, bu = C.c_int(), C.c_int()
+        check(N.lib.imba_build_cache_stats(self.h, C.byref(lo), C.byref(bu)), "imba_build_cache_stats")
+        return lo.value, bu.value
+
     def tri_attrs(self):
         n = C.c_void_p()
         m = C.c_void_p()
@@ -165,6 +176,50 @@ def desc_from_arrays(a: dict) -> tuple[N.SceneDesc, list]:
                     p(a["texcoords"]), len(a["texcoords"]), p(a["masks"]), len(a["masks"]),
                     p(a["mask_buffer"]), len(a["mask_buffer"]))
     return d, keep
+
+
+def bvh_store(path, block_type: int, nodes: np.ndarray, tris: np.ndarray, node_offset: int = 0, tris_offset: int = 0,
+              tri_id_offset: int = 0):
+    """Writes one tree (node records, Vec4 records) as a cache-file block."""
+    n = np.ascontiguousarray(nodes)
+    t = np.ascontiguousarray(tris, np.float32).reshape(-1, 4)
+    check(N.lib.imba_bvh_store(str(path).encode(), block_type, n.ctypes.data if n.size else None, len(n),
+                               t.ctypes.data if t.size else None, len(t), node_offset, tris_offset, tri_id_offset),
+          "imba_bvh_store")
+
+
+def bvh_load(path, block_type: int, node_offset: int = 0, tris_offset: int = 0, tri_id_offset: int = 0):
+    """Reads a cache-file block: (nodes, Vec4 records (n, 4) float32)."""
+    nc, vc = C.c_int(), C.c_int()
+    p = str(path).encode()
+    code = N.lib.imba_bvh_load(p, block_type, node_offset, tris_offset, tri_id_offset, None, 0, None, 0,
+                               C.byref(nc), C.byref(vc))
+    if code not in (5, N.IMBA_OK):  # 5: capacity (counts filled in)
+        check(code, "imba_bvh_load")
+    dt = NODE_DTYPE if block_type == N.CACHE_BVH2 else N.NODE4_DTYPE
+    nodes = np.empty(nc.value, dt)
+    tris = np.empty((vc.value, 4), np.float32)
+    check(N.lib.imba_bvh_load(p, block_type, node_offset, tris_offset, tri_id_offset, nodes.ctypes.data, len(nodes),
+                              tris.ctypes.data, len(tris), C.byref(nc), C.byref(vc)), "imba_bvh_load")
+    return nodes, tris
+
+
+def wide_tree(arrays: dict) -> dict:
+    """The 4-wide tree the upload derives from host arrays: nodes (NODE4_DTYPE), instances,
+    root, max_stack."""
+    d, keep = desc_from_arrays(arrays)
+    h = C.c_void_p()
+    check(N.lib.imba_wide_create(C.byref(d), C.byref(h)), "imba_wide_create")
+    try:
+        pn, pi = C.c_void_p(), C.c_void_p()
+        nn, ni, root, ms = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(N.lib.imba_wide_get(h, C.byref(pn), C.byref(nn), C.byref(pi), C.byref(ni), C.byref(root), C.byref(ms)),
+              "imba_wide_get")
+        nodes = np.frombuffer((C.c_char * (nn.value * 128)).from_address(pn.value), N.NODE4_DTYPE).copy()
+        inst = np.frombuffer((C.c_char * (ni.value * 64)).from_address(pi.value), INSTANCE_DTYPE).copy()
+        return {"nodes": nodes, "instances": inst, "root": root.value, "max_stack": ms.value}
+    finally:
+        N.lib.imba_wide_release(h)
 
 
 def emit_mesh(vertices, indices, builder: int, width: int, cost: int) -> np.ndarray:
@@ -226,6 +281,12 @@ class Scene:
                         stream=None):
         f = N.lib.imba_occluded_stats if any_hit else N.lib.imba_intersect_stats
         check(f(self.h, rays.ptr, hits.ptr, counts.ptr, count, N._s(stream)), "stats")
+
+    def trace_stats(self, rays: DeviceBuffer, hits: DeviceBuffer, counts: DeviceBuffer, count: int, any_hit=False,
+                    cfg: N.TraceConfig | None = None, stream=None):
+        """Visit counts of the tree `cfg` selects (bvh_width 2 or 4)."""
+        check(N.lib.imba_trace_stats(self.h, rays.ptr, hits.ptr, counts.ptr, count, int(any_hit),
+                                     C.byref(cfg) if cfg else None, N._s(stream)), "imba_trace_stats")
 
     def trace_host(self, rays: np.ndarray, any_hit: bool = False, cfg: N.TraceConfig | None = None) -> np.ndarray:
         """Convenience: upload rays, trace, download hits."""

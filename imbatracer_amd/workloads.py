@@ -180,8 +180,8 @@ class PathTracer:
             if len(queues) < keep_queues:
                 queues.append(self.rays[cur].download(N.RAY_DTYPE, n, st))
             if stats:
-                N.check(N.lib.imba_intersect_stats(self.gpu_scene.h, self.rays[cur].ptr, self.hits.ptr,
-                                                   self.counts.ptr, n, st), "intersect_stats")
+                N.check(N.lib.imba_trace_stats(self.gpu_scene.h, self.rays[cur].ptr, self.hits.ptr, self.counts.ptr,
+                                               n, 0, C.byref(self.cfg) if self.cfg else None, st), "trace_stats")
                 c = self.counts.download(np.uint32, 3 * n, st).reshape(-1, 3)
                 visit.append(c.astype(np.int64).sum(0))
             else:

@@ -230,6 +230,21 @@ public:
         for (int i = 0; i < count; ++i) sorted_indices_[start[mat[i]]++] = i;
     }
 
+    // The same sort on the GPU with the material lookup of the integrators (pt.cpp:85-91):
+    // the material of a hit is data.indices[4 tri_id + 3].  Hits without a valid material
+    // come last.
+    template <typename Node>
+    void sort_by_material_gpu(const TraversalData<Node>& data, int num_mats, int count) {
+        if (count <= 0) return;
+        stage(count, true);
+        DeviceArray<int> order((size_t)count);
+        check(imba_sort_by_material(data.indices.data(), (int)(data.indices.size() / 4), dev_rays_.data(),
+                                    dev_hits_.data(), count, num_mats, order.data(), nullptr, nullptr),
+              "imba_sort_by_material");
+        dev_rays_.download(ray_buffer_.data(), count);
+        order.download(sorted_indices_.data(), count);
+    }
+
     // Traverses all rays of the queue on the GPU (closest hit).
     template <typename Node>
     void traverse_gpu(const TraversalData<Node>& c_data) {

@@ -195,6 +195,29 @@ int imba_build_cpu_layout(ImbaSceneBuild* b, int builder, int threads);
 /* Host pointers stay valid until imba_build_release or the next imba_build_cpu_layout. */
 int imba_build_get_cpu_desc(const ImbaSceneBuild* b, ImbaCpuSceneDesc* out);
 
+/* Acceleration-structure cache files (the container of loaders/load_bvh.cpp): u32 magic
+ * 0x313F1A57, then blocks {u64 size, u32 type, u32 node_count, u32 prim_count, node records,
+ * Vec4 records}; type IMBA_CACHE_BVH2 (1) holds a mesh's BVH2 layout (ImbaNode, 3 Vec4 per
+ * triangle), IMBA_CACHE_BVH4 (2) its CPU layout (ImbaNode4, triangle packets).  A block stores
+ * one tree with links relative to its first node / first Vec4 and triangle ids relative to the
+ * mesh's first id.  store rebases by -offsets and refuses a file that already holds a block of
+ * the type; load rebases by +offsets (load_bvh.cpp:66-95, 121-139).  With nodes_out or tris_out
+ * NULL or too small, load returns IMBA_ERR_CAPACITY with the counts filled in. */
+enum { IMBA_CACHE_BVH2 = 1, IMBA_CACHE_BVH4 = 2 };
+int imba_bvh_store(const char* path, int type, const void* nodes, int node_count, const ImbaVec4* tris,
+                   int vec4_count, int node_offset, int tris_offset, int tri_id_offset);
+int imba_bvh_load(const char* path, int type, int node_offset, int tris_offset, int tri_id_offset,
+                  void* nodes_out, int node_cap, ImbaVec4* tris_out, int vec4_cap, int* node_count,
+                  int* vec4_count);
+/* 1 if the file holds a block of `type`, else 0. */
+int imba_bvh_has_block(const char* path, int type);
+/* Per-mesh cache file for imba_build_run / imba_build_cpu_layout (Scene::build_mesh_accels,
+ * render/scene.cpp:49-70): a mesh whose file holds the layout's block is loaded instead of
+ * built; a built mesh is stored to its file.  path NULL or "" removes the file. */
+int imba_build_set_cache(ImbaSceneBuild* b, int mesh_id, const char* path);
+/* Meshes loaded from cache files / built by the last imba_build_run or imba_build_cpu_layout. */
+int imba_build_cache_stats(const ImbaSceneBuild* b, int* loaded, int* built);
+
 /* Runs one builder and writes the sequence of nodes and leaves it emits (for checking the
  * builders and for custom layout writers).  Records, as int32 (floats as their bits):
  *   node: 0, count, parent box, count child boxes      leaf: 1, count, box, count primitive ids
@@ -207,6 +230,17 @@ int imba_bvh_emit_mesh(const ImbaMeshDesc* mesh, int builder, int width, int cos
 /* Binned SAH over given primitive boxes (6 floats) and centres (3 floats). */
 int imba_bvh_emit_boxes(const float* boxes, const float* centres, int n, int width, int cost,
                         int32_t* out, size_t cap, size_t* len);
+
+/* The 4-wide tree the upload derives from a host scene (for inspection): every node holds up to
+ * four children of a BVH2 node and its absorbed inner children, chosen largest surface area
+ * first; leaves and their boxes are the BVH2's.  Nodes: ImbaNode4, children[j] > 0 node, < 0 the
+ * BVH2's leaf link, 0 unused; instance records with `next` = 4-wide mesh root (-1 for records
+ * the top level does not reach).  Pointers stay valid until imba_wide_release. */
+typedef struct ImbaWideBvh ImbaWideBvh;
+int imba_wide_create(const ImbaSceneDesc* host, ImbaWideBvh** out);
+int imba_wide_get(const ImbaWideBvh* w, const ImbaNode4** nodes, int* node_count,
+                  const ImbaInstanceNode** instances, int* instance_count, int* root, int* max_stack);
+int imba_wide_release(ImbaWideBvh* w);
 
 /* ------------------------------ device-resident scenes -----------------------------------
  * Scene::upload_* (render/scene.cpp:119-188).  The scene is validated on upload: child
@@ -237,6 +271,8 @@ typedef struct ImbaTraceConfig {
     int stack_lds;         /* per-lane stack entries kept in LDS (0 = default)             */
     int persistent;        /* 1: persistent waves fetching rays with a global counter      */
     int blocks_per_cu;     /* persistent grid size per CU (0 = default)                    */
+    int bvh_width;         /* 2: the scene's BVH2 as given; 4: the 4-wide tree derived from
+                              it at upload (same leaves); 0 = default                      */
 } ImbaTraceConfig;
 
 int imba_intersect(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int count,
@@ -255,6 +291,11 @@ int imba_intersect_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits,
                          int count, void* stream);
 int imba_occluded_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts,
                         int count, void* stream);
+/* Visit counts of the tree a configuration traverses (cfg->bvh_width; the *_stats calls above
+ * count on the BVH2, as the CPU restatement does): nodes whose children were box-tested
+ * (4-wide nodes for width 4), triangles tested, instances entered. */
+int imba_trace_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts, int count,
+                     int any_hit, const ImbaTraceConfig* cfg, void* stream);
 /* Traversal over caller-owned device arrays described by `dev` (device pointers and
  * element counts), for callers that manage their own TraversalData.  The arrays are copied
  * to the host and validated once per distinct set of pointers and counts.  any_hit: 0 closest
@@ -326,6 +367,17 @@ size_t imba_compact_workspace_size(int capacity);
 int imba_compact_rays_dev(const ImbaRay* rays_in, const void* states_in, int state_stride, int capacity,
                           const int* count_dev, ImbaRay* rays_out, void* states_out, int* new_count_dev,
                           void* workspace, void* stream);
+
+/* RayQueue::sort_by_material (ray_queue.h:211-244) on the device: for the first `count`
+ * entries (the hits, after compact_hits), the material of the hit triangle, indices[4 tri_id + 3]
+ * (the lookup of pt.cpp:85-91), is written to rays[i].dir.w and `sorted` receives the entry
+ * indices grouped by material in increasing material order, and in queue order within a
+ * material (the reference's parallel scatter leaves that order open).  Entries whose triangle
+ * or material is out of range (a miss, for instance) get key num_mats and come last.
+ * bucket_start (num_mats + 1 ints, nullable): first position of each material and of that last
+ * group.  indices: device, 4 ints per triangle, tri_count triangles. */
+int imba_sort_by_material(const int* indices, int tri_count, ImbaRay* rays, const ImbaHit* hits, int count,
+                          int num_mats, int* sorted, int* bucket_start, void* stream);
 
 /* ------------------------------ photon hash grid ------------------------------------------
  * HashGrid::build / query (rangesearch/rangesearch.h:25-148).  Photons are records of

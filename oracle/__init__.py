@@ -72,6 +72,8 @@ def lib():
         L.oracle_compact_hits.restype = i32
         L.oracle_compact_rays.argtypes = [vp, vp, i32, i32]
         L.oracle_compact_rays.restype = i32
+        L.oracle_sort_by_material.argtypes = [vp, i32, vp, vp, i32, i32, vp, vp]
+        L.oracle_sort_by_material.restype = None
         L.oracle_grid_build.argtypes = [C.POINTER(OracleGrid), vp, i32, i32, C.c_float]
         L.oracle_grid_build.restype = None
         L.oracle_grid_query.argtypes = [C.POINTER(OracleGrid), vp, i32, vp, vp]
@@ -150,6 +152,18 @@ def trace_cpu4(arrays: dict, rays: np.ndarray, any_hit: bool = False, threads: i
     else:
         bad = lib().cpu4_trace(C.byref(d), _p(rays), len(rays), int(any_hit), _p(hits))
     return hits, bad
+
+
+def sort_by_material(indices, rays, hits, num_mats):
+    """(rays with dir.w = material key, sorted indices, bucket starts)."""
+    ind = np.ascontiguousarray(indices, np.int32)
+    rays = np.ascontiguousarray(rays).copy()
+    hits = np.ascontiguousarray(hits)
+    out = np.empty(len(rays), np.int32)
+    starts = np.empty(num_mats + 1, np.int32)
+    lib().oracle_sort_by_material(_p(ind), ind.size // 4, _p(rays), _p(hits), len(rays), num_mats, _p(out),
+                                  _p(starts))
+    return rays, out, starts
 
 
 def brute_closest(tri_verts: np.ndarray, rays: np.ndarray):

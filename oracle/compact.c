@@ -8,9 +8,13 @@
  * (state.pixel_id >= 0, the first int of the state) are moved forward in order; hits are
  * not moved; the size only changes when at least one ray was removed.
  *
+ * oracle_sort_by_material restates RayQueue::sort_by_material (ray_queue.h:211-244) on one
+ * thread (queue order inside a material).
+ *
  * Pinned by: brute force in tests/test_oracle_compact.py (a hand-worked 6-element example
  * from the reference's loop, the stable order of the hit prefix / survivors, the multiset
- * of the miss suffix, the returned counts, all-hit / all-miss / empty queues).
+ * of the miss suffix, the returned counts, all-hit / all-miss / empty queues; for the material
+ * sort a hand-worked example and the permutation / grouping / stability properties).
  */
 #include <stdlib.h>
 #include <string.h>
@@ -61,4 +65,37 @@ int oracle_compact_rays(ImbaRay* rays, void* states, int stride, int count) {
         }
     }
     return first_dead != -1 ? first_dead : count;
+}
+
+/*
+ * RayQueue::sort_by_material (ray_queue.h:211-244) run on one thread: material of every hit
+ * (indices[4 tri_id + 3], pt.cpp:85-91) written to dir.w, per-material counts, exclusive
+ * scan, scatter in queue order.  With one thread the reference's scatter visits entries in
+ * increasing index, which fixes the order inside a material.  Entries without a valid material
+ * take key num_mats (last).  starts: num_mats + 1 ints (nullable).
+ */
+void oracle_sort_by_material(const int* indices, int tri_count, ImbaRay* rays, const ImbaHit* hits, int count,
+                             int num_mats, int* sorted, int* starts) {
+    int* cnt = (int*)calloc((size_t)num_mats + 2, sizeof(int));
+    for (int i = 0; i < count; i++) {
+        const int tri = hits[i].tri_id;
+        int mat = -1;
+        if (tri >= 0 && tri < tri_count) mat = indices[4 * (size_t)tri + 3];
+        const int key = (mat >= 0 && mat < num_mats) ? mat : num_mats;
+        memcpy(&rays[i].dir.w, &key, sizeof(int));
+        cnt[key]++;
+    }
+    int accum = 0;
+    for (int k = 0; k <= num_mats; k++) {
+        const int c = cnt[k];
+        cnt[k] = accum;
+        if (starts) starts[k] = accum;
+        accum += c;
+    }
+    for (int i = 0; i < count; i++) {
+        int key;
+        memcpy(&key, &rays[i].dir.w, sizeof(int));
+        sorted[cnt[key]++] = i;
+    }
+    free(cnt);
 }

@@ -49,6 +49,11 @@ int oracle_compact_hits(ImbaRay* rays, ImbaHit* hits, void* states, int state_st
 /* RayQueue::compact_rays: returns the new queue size (count if nothing was removed). */
 int oracle_compact_rays(ImbaRay* rays, void* states, int state_stride, int count);
 
+/* RayQueue::sort_by_material on one thread: dir.w = material, `sorted` grouped by material in
+ * queue order, starts[k] = first position of material k (num_mats + 1 entries, nullable). */
+void oracle_sort_by_material(const int* indices, int tri_count, ImbaRay* rays, const ImbaHit* hits, int count,
+                             int num_mats, int* sorted, int* starts);
+
 /* ---- photon hash grid (hashgrid.c) ---- */
 typedef struct OracleGrid {
     int photon_count;

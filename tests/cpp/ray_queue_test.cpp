@@ -60,6 +60,7 @@ int main() {
     std::vector<float> verts(12 * ntri);
     std::vector<uint32_t> idx(4 * ntri);
     imba::check(imba_gen_random_triangles(0x1234, ntri, verts.data(), idx.data()), "gen");
+    for (int t = 0; t < ntri; t++) idx[4 * t + 3] = t % 7;  // seven materials
     ImbaSceneBuild* b = nullptr;
     imba::check(imba_build_create(&b), "build_create");
     ImbaMeshDesc md{verts.data(), 3 * ntri, idx.data(), ntri, nullptr};
@@ -103,6 +104,21 @@ int main() {
         EXPECT(q.hit(i).tri_id >= 0, "sorted index %d not reset", i);
     }
     q.shrink(n);
+
+    // sort_by_material on the GPU (the material lookup of pt.cpp:85-91) against the serial
+    // restatement: same keys in dir.w, same grouped order
+    {
+        std::vector<ImbaRay> rs(q.rays(), q.rays() + n);
+        std::vector<ImbaHit> hs(q.hits(), q.hits() + n);
+        std::vector<int> order_ref(n);
+        oracle_sort_by_material(host.indices, host.index_count / 4, rs.data(), hs.data(), n, 7, order_ref.data(),
+                                nullptr);
+        q.sort_by_material_gpu(data, 7, n);
+        for (int i = 0; i < n; i++) {
+            EXPECT(&q.hit(i) - q.hits() == order_ref[i], "sort_by_material position %d differs", i);
+            EXPECT(std::memcmp(&q.rays()[i], &rs[i], sizeof(ImbaRay)) == 0, "sort_by_material key of %d differs", i);
+        }
+    }
 
     // terminate every third path and compact the rays (pt.cpp:150 / compact_rays)
     for (int i = 0; i < q.size(); i += 3) q.states()[i].pixel_id = -1;

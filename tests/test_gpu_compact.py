@@ -78,3 +78,32 @@ def test_compact_rays(gpu, n, frac, words):
     s = so.download(np.int32, states.size).reshape(states.shape)
     assert np.array_equal(_bytes(r[:k]), _bytes(r_ref[:k]))
     assert np.array_equal(s[:k], s_ref[:k])
+
+
+@pytest.mark.parametrize("count,num_mats", [(0, 5), (1, 1), (1000, 3), (5000, 255), (5000, 256), (300000, 1026),
+                                            (70000, 70000)])
+def test_sort_by_material_matches_oracle(gpu, count, num_mats):
+    """Keys written to dir.w, the stable grouping and the bucket starts equal the serial
+    restatement; misses and out-of-range triangles / materials go last."""
+    rng = np.random.default_rng(count + num_mats)
+    n_tris = 4000
+    indices = rng.integers(0, 1000, (n_tris, 4)).astype(np.int32)
+    indices[:, 3] = rng.integers(0, num_mats + 2, n_tris)   # some materials out of range
+    hits = np.zeros(count, N.HIT_DTYPE)
+    hits["tri_id"] = np.where(rng.random(count) < 0.9, rng.integers(0, n_tris + 50, count), -1)
+    rays = np.zeros(count, N.RAY_DTYPE)
+    rays["org"][:, 0] = np.arange(count)
+    want_rays, want_order, want_starts = oracle.sort_by_material(indices, rays, hits, num_mats)
+    ib = N.DeviceBuffer.from_array(indices)
+    rb = N.DeviceBuffer.from_array(rays) if count else N.DeviceBuffer(32)
+    hb = N.DeviceBuffer.from_array(hits) if count else N.DeviceBuffer(16)
+    ob = N.DeviceBuffer(max(count, 1) * 4)
+    sb = N.DeviceBuffer((num_mats + 1) * 4)
+    for _ in range(2):
+        N.check(N.lib.imba_sort_by_material(ib.ptr, n_tris, rb.ptr, hb.ptr, count, num_mats, ob.ptr, sb.ptr, None),
+                "sort_by_material")
+        assert np.array_equal(sb.download(np.int32, num_mats + 1), want_starts)
+        if count:
+            assert np.array_equal(ob.download(np.int32, count), want_order)
+            got_rays = rb.download(N.RAY_DTYPE, count)
+            assert np.array_equal(got_rays.view(np.int32), want_rays.view(np.int32))

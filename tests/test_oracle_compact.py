@@ -62,3 +62,37 @@ def test_compact_rays_hand_worked_and_properties():
     states[~alive, 0] = -1
     n, r, s = oracle.compact_rays(rays, states)
     assert n == alive.sum() and np.array_equal(s[:n, 0], np.nonzero(alive)[0])
+
+
+def _mat_queue(tri_ids):
+    rays = np.zeros(len(tri_ids), oracle.RAY_DTYPE)
+    hits = np.zeros(len(tri_ids), oracle.HIT_DTYPE)
+    hits["tri_id"] = tri_ids
+    return rays, hits
+
+
+def test_sort_by_material_hand_worked():
+    # triangles 0..4 with materials 2, 0, 2, 1, 7 (7 is out of range for num_mats = 3)
+    indices = np.array([[0, 0, 0, 2], [0, 0, 0, 0], [0, 0, 0, 2], [0, 0, 0, 1], [0, 0, 0, 7]], np.int32)
+    rays, hits = _mat_queue([2, 0, 3, -1, 1, 2, 4, 9])
+    r, order, starts = oracle.sort_by_material(indices, rays, hits, 3)
+    # keys per entry: 2, 2, 1, 3(miss), 0, 2, 3(mat 7), 3(tri 9 out of range)
+    assert r["dir"][:, 3].view(np.int32).tolist() == [2, 2, 1, 3, 0, 2, 3, 3]
+    assert order.tolist() == [4, 2, 0, 1, 5, 3, 6, 7]
+    assert starts.tolist() == [0, 1, 2, 5]
+
+
+def test_sort_by_material_properties():
+    rng = np.random.default_rng(11)
+    mats = rng.integers(0, 50, 3000)
+    indices = np.zeros((3000, 4), np.int32)
+    indices[:, 3] = mats
+    rays, hits = _mat_queue(np.where(rng.random(20000) < 0.9, rng.integers(0, 3000, 20000), -1))
+    r, order, starts = oracle.sort_by_material(indices, rays, hits, 50)
+    keys = r["dir"][:, 3].view(np.int32)
+    assert sorted(order.tolist()) == list(range(20000))                # a permutation
+    assert np.all(np.diff(keys[order]) >= 0)                            # grouped by material
+    for k in range(51):                                                 # queue order inside a group
+        grp = order[starts[k]:(starts[k + 1] if k < 50 else 20000)]
+        assert np.all(np.diff(grp) > 0) and np.all(keys[grp] == k)
+    assert (keys == 50).sum() == (hits["tri_id"] < 0).sum()
