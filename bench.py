@@ -2,14 +2,18 @@
 
 Default workload (the configuration the headline metric is quoted on): path tracing at
 1280x720, 1 spp, maximum path length 8 with Russian roulette from bounce 3, over the
-sphere-grid scene (32 x 32 level-5 icospheres + open box + triangle light, 20,971,544
+sphere-grid scene (32 x 32 level-5 icospheres + open box + triangle light, 20,971,533
 triangles, SBVH per mesh).  One step = one frame: camera rays, then per bounce closest-hit
-traversal, Lambertian bounce and queue compaction, all on the device.  `value` is the number of
-closest-hit rays traversed per second of frame time (inputs resident in HBM; the scene build
-and upload are outside the timed region).
+traversal, Lambertian bounce and queue compaction, all on the device, captured once into a
+graph whose queue sizes stay in device memory (--mode eager: one launch per call, the host
+reading every bounce's queue size).  `value` is the number of closest-hit rays traversed per
+second of frame time (inputs resident in HBM; the scene build and upload are outside the
+timed region).
 
 --workload c1 runs the CPU-runnable configuration instead: 65,536 random triangles
 (binned SAH), 1,048,576 primary closest-hit rays plus 1,048,576 occlusion rays per step.
+--workload c4: 256 queues over 8 scenes, 40M incoherent closest-hit rays per step in one
+batched launch, against the same queues launched one at a time.
 
 Also reported: the roofline of the traversal kernel (algorithmic bytes per ray
 32 + 16 + 64 N_node + 48 N_tri + 64 N_inst with visit counts measured by the stats variant of
@@ -271,27 +275,31 @@ def run_c4(args):
     from imbatracer_amd import scene as S
     from imbatracer_amd import workloads as W
 
-    sizes = [65536, 131072, 262144, 524288, 65536, 131072, 262144, 524288]
-    builds, scenes = [], []
+    sizes = W.C4_SCENE_TRIS
+    builds, scenes, geometry = [], [], []
     t0 = time.time()
     for k, n in enumerate(sizes):
-        b, _, _ = W.random_tri_scene(n, N.BUILDER_SBVH if k % 2 else N.BUILDER_BINNED_SAH, seed=0x1234 + k)
+        b, v, i = W.random_tri_scene(n, N.BUILDER_SBVH if n <= 250_000 else N.BUILDER_BINNED_SAH, seed=0x1234 + k)
         builds.append(b)
+        geometry.append((v, i))
         scenes.append(S.Scene(b.desc()))
     log("8 scenes built in %.1f s" % (time.time() - t0))
-    n_queues, per_queue = 256, args.c4_rays // 256
-    rng = np.random.default_rng(2024)
+    q_sizes = W.c4_queue_sizes(256)
+    if args.c4_rays:
+        q_sizes = np.maximum(1, (q_sizes * (args.c4_rays / q_sizes.sum())).astype(np.int64))
+    n_queues = len(q_sizes)
     queues = []
+    t0 = time.time()
     for q in range(n_queues):
-        o = rng.uniform(-0.2, 1.2, (per_queue, 3)).astype(np.float32)
-        d = rng.normal(size=(per_queue, 3)).astype(np.float32)
-        d /= np.linalg.norm(d, axis=1, keepdims=True).astype(np.float32)
-        rays = S.make_rays(o, d, np.float32(0.0), np.float32(3.4028235e38))
-        queues.append((scenes[q % 8], N.DeviceBuffer.from_array(rays), N.DeviceBuffer(per_queue * 16), per_queue,
-                       None, False))
+        n = int(q_sizes[q])
+        v, i = geometry[q % 8]
+        rays = W.c4_rays(v, i, n, coherent=q % 2 == 0, seed=5000 + q)
+        queues.append((scenes[q % 8], N.DeviceBuffer.from_array(rays), N.DeviceBuffer(n * 16), n, None, False))
         if q < 2:
             queues[-1] += (rays,)
-    log("%d queues x %d rays uploaded" % (n_queues, per_queue))
+    per_queue = int(q_sizes.max())
+    log("%d queues, %d rays (%d..%d per queue) generated and uploaded in %.1f s" % (
+        n_queues, int(q_sizes.sum()), int(q_sizes.min()), int(q_sizes.max()), time.time() - t0))
     cfg = N.TraceConfig(args.block, args.stack, 0, args.blocks_per_cu, args.bvh_width) \
         if (args.block or args.blocks_per_cu or args.bvh_width) else None
     # visit counts per queue (untimed)
@@ -335,7 +343,7 @@ def run_c4(args):
 
     wall_b, gpu_b = timed(lambda: batch.launch(cfg, st))
     wall_q, gpu_q = timed(per_queue_launches)
-    rays_step = n_queues * per_queue
+    rays_step = int(q_sizes.sum())
     achieved = bytes_step * args.steps / (gpu_b * 1e-3) / 1e9
     res = {
         "metric": METRIC, "value": rays_step * args.steps / wall_b / 1e6, "unit": "Mrays/s", "n_gpus": 1,
@@ -343,7 +351,8 @@ def run_c4(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded random triangles and incoherent rays)",
         "config": {"workload": "c4-256-queues-%d-rays-8-scenes" % rays_step, "queues": n_queues,
-                   "rays_per_queue": per_queue, "scene_triangles": sizes,
+                   "rays_per_queue": "log-uniform in [1K, 1M]; half pinhole, half cosine-hemisphere from surfaces",
+                   "scene_triangles": sizes,
                    "trace": "one batched launch per step (imba_trace_batch_launch)"},
         "per_queue_launches": {"value": rays_step * args.steps / wall_q / 1e6, "ms_per_step": wall_q * 1e3 / args.steps,
                                "gpu_ms_per_step": gpu_q / args.steps},
@@ -363,7 +372,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="pt", choices=["pt", "c1", "c4"])
-    ap.add_argument("--c4-rays", type=int, default=40_000_000, help="c4: rays over all 256 queues")
+    ap.add_argument("--c4-rays", type=int, default=0,
+                    help="c4: rescale the 256 log-uniform queue sizes to this many rays (0: as drawn, about 37M)")
     ap.add_argument("--width", type=int, default=1280)
     ap.add_argument("--height", type=int, default=720)
     ap.add_argument("--side", type=int, default=32, help="spheres per side of the grid")

@@ -171,10 +171,14 @@ struct LaneStack {
 enum : int { kTop = 1, kInstLast = 2, kBad = 4, kDone = 8 };
 
 // Traversal state of one ray (kept in registers across the steps of the persistent kernel).
+struct WorldRay {
+    float ox, oy, oz, dx, dy, dz;
+};
+
 struct TraceState {
-    RaySpace world;  // world-space ray
+    WorldRay world;  // world-space ray (its reciprocals are recomputed when the ray returns to it)
     RaySpace cs;     // ray in the space of the current level (world or an instance's object space)
-    float tmin, tmax;
+    float tmin;      // the ray's tmax is best_t until a hit
     float best_t, best_u;
     int best_tri, best_inst;
     int node;        // next node / leaf link to process, kNoNode if none
@@ -184,13 +188,18 @@ struct TraceState {
     uint32_t cn, ct, ci;  // visit counts (stats variant)
 };
 
+// The current space becomes world space (reciprocals recomputed, the same values as at the start).
+__device__ __forceinline__ void to_world(TraceState& s) {
+    s.cs.ox = s.world.ox; s.cs.oy = s.world.oy; s.cs.oz = s.world.oz;
+    s.cs.dx = s.world.dx; s.cs.dy = s.world.dy; s.cs.dz = s.world.dz;
+    setup_space(s.cs);
+}
+
 __device__ __forceinline__ void state_init(TraceState& s, const float4& ro, const float4& rd, int root) {
     s.world.ox = ro.x; s.world.oy = ro.y; s.world.oz = ro.z;
     s.world.dx = rd.x; s.world.dy = rd.y; s.world.dz = rd.z;
-    setup_space(s.world);
-    s.cs = s.world;
+    to_world(s);
     s.tmin = ro.w;
-    s.tmax = rd.w;
     s.best_t = rd.w;
     s.best_u = 0.0f;
     s.best_tri = -1;
@@ -215,7 +224,7 @@ __device__ __forceinline__ int enter_instance(const TraceArgs& a, TraceState& s,
     const float4 m0 = a.inst[4 * i + 0], m1 = a.inst[4 * i + 1], m2 = a.inst[4 * i + 2];
     const int4 meta = reinterpret_cast<const int4*>(a.inst)[4 * i + 3];
     if (STATS) s.ci++;
-    const RaySpace& w = s.world;
+    const WorldRay& w = s.world;
     s.cs.ox = fmaf(m0.x, w.ox, fmaf(m0.y, w.oy, fmaf(m0.z, w.oz, m0.w)));
     s.cs.oy = fmaf(m1.x, w.ox, fmaf(m1.y, w.oy, fmaf(m1.z, w.oz, m1.w)));
     s.cs.oz = fmaf(m2.x, w.ox, fmaf(m2.y, w.oy, fmaf(m2.z, w.oz, m2.w)));
@@ -346,7 +355,7 @@ __device__ __forceinline__ bool advance(const TraceArgs& a, TraceState& s, LaneS
         while (!(s.flags & kTop) && st.sp == s.inst_sp) {
             if (s.flags & kInstLast) {
                 s.flags |= kTop;
-                s.cs = s.world;
+                to_world(s);
                 s.inst_sp = -1;
                 break;
             }
@@ -373,7 +382,7 @@ __device__ __forceinline__ bool advance(const TraceArgs& a, TraceState& s, LaneS
         if (tri_hit(s.cs, v0, v1, v2, th, uh, vh) && th >= s.tmin) {
             const int tid = __float_as_int(v1.w);
             if (ANY) {
-                if (th <= s.tmax) {
+                if (th <= s.best_t) {
                     int op = 1;
                     if (MASKED) {
                         op = texel_opaque(a, tid, uh, vh);
@@ -539,7 +548,7 @@ __global__ void k_batch_prefix(BatchItem* items, BatchGroups groups, int* group_
 }
 
 template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool WIDE>
-__global__ __launch_bounds__(BLOCK, (ANY || MASKED) ? 1 : 8) void k_trace_batch(const BatchItem* __restrict__ items, int first, int n_items,
+__global__ __launch_bounds__(BLOCK, (ANY || MASKED || WIDE) ? 1 : 8) void k_trace_batch(const BatchItem* __restrict__ items, int first, int n_items,
                                                        const int* __restrict__ total_chunks, int* counter, int* spill,
                                                        int spill_depth) {
     __shared__ int lds_stack[STACK_LDS * BLOCK];

@@ -59,6 +59,49 @@ def shadow_rays(primary: np.ndarray, hits: np.ndarray, light=C1_LIGHT, seed: int
     return S.make_rays(pts, dirs, off, (dist - off).astype(np.float32))
 
 
+C4_SCENE_TRIS = [10_000, 30_000, 100_000, 250_000, 500_000, 1_000_000, 2_000_000, 4_000_000]
+
+
+def c4_queue_sizes(n_queues: int = 256, lo: int = 1_000, hi: int = 1_000_000, seed: int = 404) -> np.ndarray:
+    """Queue sizes log-uniform in [lo, hi] (about 37M rays for 256 queues)."""
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), n_queues)).astype(np.int64)
+
+
+def c4_rays(vertices: np.ndarray, indices: np.ndarray, n: int, coherent: bool, seed: int) -> np.ndarray:
+    """One queue of the multi-queue workload over a random-triangle scene in the unit cube:
+    coherent = pinhole rays from a seeded camera orbiting the cube; otherwise cosine-weighted
+    hemisphere rays leaving random points of random triangles (tmin 1e-4)."""
+    rng = np.random.default_rng(seed)
+    if coherent:
+        ang = rng.uniform(0.0, 2.0 * np.pi)
+        pos = np.array([0.5 + 1.8 * np.sin(ang), rng.uniform(0.2, 0.8), 0.5 - 1.8 * np.cos(ang)], np.float32)
+        look = (np.array([0.5, 0.5, 0.5], np.float32) - pos).astype(np.float32)
+        w = max(1, int(np.ceil(np.sqrt(n))))
+        h = int(np.ceil(n / w))
+        return S.camera_rays(pos, look, (0.0, 1.0, 0.0), 50.0, w, h)[:n]
+    tri = rng.integers(0, len(indices), n)
+    p = vertices[:, :3][indices[tri, :3].astype(np.int64)].astype(np.float64)  # (n, 3, 3)
+    r1, r2 = rng.random(n), rng.random(n)
+    s = np.sqrt(r1)
+    bary = np.stack([1.0 - s, s * (1.0 - r2), s * r2], 1)
+    org = (bary[:, :, None] * p).sum(1)
+    nrm = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    nrm /= np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-30)
+    nrm *= np.where(rng.random(n) < 0.5, -1.0, 1.0)[:, None]
+    # cosine-weighted direction around the normal
+    u1, u2 = rng.random(n), rng.random(n)
+    rr, phi = np.sqrt(u1), 2.0 * np.pi * u2
+    local = np.stack([rr * np.cos(phi), rr * np.sin(phi), np.sqrt(np.maximum(0.0, 1.0 - u1))], 1)
+    a = np.where(np.abs(nrm[:, :1]) > 0.9, np.array([[0.0, 1.0, 0.0]]), np.array([[1.0, 0.0, 0.0]]))
+    t = np.cross(nrm, a)
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    b = np.cross(nrm, t)
+    d = local[:, :1] * t + local[:, 1:2] * b + local[:, 2:] * nrm
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return S.make_rays(org.astype(np.float32), d.astype(np.float32), np.float32(1e-4), np.float32(3.4028235e38))
+
+
 def mwc_uniform(seed: int, n: int) -> np.ndarray:
     """n draws of the MWC64X generator's random_float() (render/random.h:19-46)."""
     out = np.empty(n, np.float32)
