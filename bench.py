@@ -12,8 +12,10 @@ timed region).
 
 --workload c1 runs the CPU-runnable configuration instead: 65,536 random triangles
 (binned SAH), 1,048,576 primary closest-hit rays plus 1,048,576 occlusion rays per step.
---workload c4: 256 queues over 8 scenes, 40M incoherent closest-hit rays per step in one
-batched launch, against the same queues launched one at a time.
+--workload c4: 256 queues over 8 scenes, about 35M closest-hit rays per step in one batched
+launch, against the same queues launched one at a time.
+--workload c5: serving; Poisson request arrivals over 8 resident scenes, the request loop with
+cross-session batching against the plain per-request path (p50/p99 latency and Mrays/s).
 
 Also reported: the roofline of the traversal kernel (algorithmic bytes per ray
 32 + 16 + 64 N_node + 48 N_tri + 64 N_inst with visit counts measured by the stats variant of
@@ -67,6 +69,47 @@ def measured_traffic(workload: str, kernel: str):
     return None, None
 
 
+def fixed_config(args):
+    """The launch configuration given on the command line, or None."""
+    from imbatracer_amd import native as N
+
+    if args.block or args.persistent or args.blocks_per_cu or args.bvh_width or args.quantized:
+        return N.TraceConfig(args.block, args.stack, args.persistent, args.blocks_per_cu, args.bvh_width or
+                             (4 if args.quantized else 0), int(args.quantized))
+    return None
+
+
+def cfg_dict(cfg):
+    if cfg is None:
+        return "default"
+    return {"block": cfg.block_size or 256, "stack_lds": cfg.stack_lds or 16, "persistent": cfg.persistent,
+            "bvh_width": cfg.bvh_width or 2, "quantized": cfg.quantized}
+
+
+def node_bytes_of(cfg):
+    """Bytes of one node of the tree a configuration walks: BVH2 64, 4-wide 128, quantized 64."""
+    return 128 if cfg is not None and cfg.bvh_width == 4 and not cfg.quantized else 64
+
+
+def kernel_label(cfg, any_hit=False):
+    suffix = ""
+    if cfg is not None and cfg.bvh_width == 4:
+        suffix = "_w4q" if cfg.quantized else "_w4"
+    return ("k_trace_occluded" if any_hit else "k_trace_closest") + suffix
+
+
+def autotune(scene, rays, any_hit=False):
+    """The library's autotuner on one device queue (every shape, persistent or not, both tree
+    widths): the fastest configuration and its time per launch."""
+    from imbatracer_amd import native as N
+
+    rb = N.DeviceBuffer.from_array(rays)
+    hb = N.DeviceBuffer(len(rays) * 16)
+    cfg, ms = scene.autotune(rb, hb, len(rays), any_hit=any_hit, reps=3)
+    log("autotune (%s, %d rays): %s, %.3f ms" % ("occluded" if any_hit else "closest", len(rays), cfg_dict(cfg), ms))
+    return cfg
+
+
 def cpu_baseline(build, builder, threads_build, ray_sets, any_flags, budget_s=12.0):
     """Times the reference's CPU path restated (oracle/cpu_bvh4.c: BVH4 nodes + 4-triangle
     packets of core/cpu_adapter.cpp, 4-wide SSE/FMA box and triangle tests, one ray per host
@@ -117,14 +160,18 @@ def run_pt(args):
     threads = max(1, min(16, os.cpu_count() or 1))
     scene = W.SphereGridScene(side=args.side, level=5, builder=N.BUILDER_SBVH, threads=threads)
     log("scene: %d triangles, %d nodes, built in %.1f s" % (scene.tri_count, scene.desc.node_count, time.time() - t0))
-    cfg = N.TraceConfig(args.block, args.stack, args.persistent, args.blocks_per_cu, args.bvh_width) \
-        if (args.block or args.persistent or args.blocks_per_cu or args.bvh_width) else None
-    node_bytes = 128 if args.bvh_width == 4 else 64
+    cfg = fixed_config(args)
     pt = W.PathTracer(scene, args.width, args.height, spp=1, max_path_len=8, rr_start=3, trace_cfg=cfg)
     log("uploaded %.2f GB" % (pt.gpu_scene.footprint() / 1e9))
+    # the queues of a frame (closest hits, hence the queues, do not depend on the configuration)
+    _, _, queues = pt.frame(keep_queues=2)
+    if cfg is None and not args.no_autotune:
+        cfg = autotune(pt.gpu_scene, queues[1])
+        pt.cfg = cfg
+    node_bytes = node_bytes_of(cfg)
 
     # untimed: visit counts of every bounce's queue (same seeds -> same rays as the timed frames)
-    counts, visits, queues = pt.frame(stats=True, keep_queues=2)
+    counts, visits, _ = pt.frame(stats=True)
     stat_bytes = [ray_bytes(v, n, node_bytes) for v, n in zip(visits, counts)]
     log("rays per bounce %s" % counts)
 
@@ -172,7 +219,7 @@ def run_pt(args):
         raise RuntimeError("traversal reported a malformed scene")
     achieved = bytes_total / (trace_ms * 1e-3) / 1e9
     workload = "pt-%dx%d-1spp-depth8-rr3-sphere-grid-%dtri-sbvh" % (args.width, args.height, scene.tri_count)
-    traffic, traffic_src = measured_traffic(workload, "k_trace_closest") if cfg is None else (None, None)
+    traffic, traffic_src = measured_traffic(workload, kernel_label(cfg)) if args.block == 0 else (None, None)
     res = {
         "metric": METRIC, "value": rays_total / wall / 1e6, "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
@@ -185,8 +232,7 @@ def run_pt(args):
                    "frame": "one graph launch per frame (queue sizes on the device)" if args.mode == "graph"
                             else "eager launches, host reads each bounce's queue size",
                    "trace_kernel": args.trace_kernel,
-                   "bvh_width": args.bvh_width or 2,
-                   "trace_shape": [args.block, args.stack] if args.block else "default"},
+                   "trace_config": cfg_dict(cfg), "autotuned": cfg is not None and not args.block},
         "eager_frames": {"value": rays_eager / wall_eager / 1e6, "ms_per_step": wall_eager * 1e3 / args.steps,
                          "frame_latency_ms": {"p50": float(np.percentile(eager_ms, 50)),
                                               "p99": float(np.percentile(eager_ms, 99))}},
@@ -218,21 +264,23 @@ def run_c1(args):
     rb = N.DeviceBuffer.from_array(prim)
     hb = N.DeviceBuffer(n * 16)
     cb = N.DeviceBuffer(n * 12)
-    cfg = N.TraceConfig(args.block, args.stack, args.persistent, args.blocks_per_cu, args.bvh_width) \
-        if (args.block or args.persistent or args.blocks_per_cu or args.bvh_width) else None
-    node_bytes = 128 if args.bvh_width == 4 else 64
-    sc.trace_stats(rb, hb, cb, n, cfg=cfg)
-    hits = hb.download(N.HIT_DTYPE, n)
-    vis_p = cb.download(np.uint32, 3 * n).reshape(-1, 3).astype(np.int64).sum(0)
+    cfg = cfg_any = fixed_config(args)
+    hits = sc.trace_host(prim)
     shadow = T.shadow_rays(prim, hits)
+    if cfg is None and not args.no_autotune:
+        cfg = autotune(sc, prim)
+        cfg_any = autotune(sc, shadow, any_hit=True)
+    node_bytes = node_bytes_of(cfg)
+    sc.trace_stats(rb, hb, cb, n, cfg=cfg)
+    vis_p = cb.download(np.uint32, 3 * n).reshape(-1, 3).astype(np.int64).sum(0)
     sb = N.DeviceBuffer.from_array(shadow)
     ob = N.DeviceBuffer(n * 16)
-    sc.trace_stats(sb, ob, cb, n, any_hit=True, cfg=cfg)
+    sc.trace_stats(sb, ob, cb, n, any_hit=True, cfg=cfg_any)
     vis_s = cb.download(np.uint32, 3 * n).reshape(-1, 3).astype(np.int64).sum(0)
     st = N.Stream()
     for _ in range(args.warmup):
         sc.intersect(rb, hb, n, cfg, st.handle)
-        sc.occluded(sb, ob, n, cfg, st.handle)
+        sc.occluded(sb, ob, n, cfg_any, st.handle)
     st.sync()
     ev = []
     t0 = time.perf_counter()
@@ -241,7 +289,7 @@ def run_c1(args):
         e0.record(st)
         sc.intersect(rb, hb, n, cfg, st.handle)
         e1.record(st)
-        sc.occluded(sb, ob, n, cfg, st.handle)
+        sc.occluded(sb, ob, n, cfg_any, st.handle)
         e2.record(st)
         ev.append((e0, e1, e2))
     st.sync()
@@ -249,13 +297,13 @@ def run_c1(args):
     t_closest = sum(a.elapsed_ms(b) for a, b, c in ev)
     achieved = ray_bytes(vis_p, n, node_bytes) * args.steps / (t_closest * 1e-3) / 1e9
     workload = "c1-65536-random-tris-binned-sah-1M-primary-1M-occlusion"
-    traffic, traffic_src = measured_traffic(workload, "k_trace_closest") if cfg is None else (None, None)
+    traffic, traffic_src = measured_traffic(workload, kernel_label(cfg)) if args.block == 0 else (None, None)
     res = {
         "metric": METRIC, "value": 2 * n * args.steps / wall / 1e6, "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded random triangles, pinhole camera, point light)",
-        "config": {"workload": workload, "bvh_width": args.bvh_width or 2},
+        "config": {"workload": workload, "trace_config": cfg_dict(cfg), "occluded_config": cfg_dict(cfg_any)},
         "closest_mrays_s": n * args.steps / (t_closest * 1e-3) / 1e6,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
@@ -300,17 +348,33 @@ def run_c4(args):
     per_queue = int(q_sizes.max())
     log("%d queues, %d rays (%d..%d per queue) generated and uploaded in %.1f s" % (
         n_queues, int(q_sizes.sum()), int(q_sizes.min()), int(q_sizes.max()), time.time() - t0))
-    cfg = N.TraceConfig(args.block, args.stack, 0, args.blocks_per_cu, args.bvh_width) \
-        if (args.block or args.blocks_per_cu or args.bvh_width) else None
+    cfg = fixed_config(args)
+    batch = S.TraceBatch([q[:6] for q in queues])
+    st = N.Stream()
+    if cfg is None and not args.no_autotune:
+        # the batched launch over every queue, timed for each tree width and a few shapes
+        best = None
+        for width, quant in ((2, 0), (4, 0), (4, 1)):
+            for block, stack in ((256, 16), (256, 8), (128, 32)):
+                c = N.TraceConfig(block, stack, 0, 0, width, quant)
+                batch.launch(c, st)
+                e0, e1 = N.Event(), N.Event()
+                e0.record(st)
+                batch.launch(c, st)
+                e1.record(st)
+                st.sync()
+                ms = e0.elapsed_ms(e1)
+                if best is None or ms < best[0]:
+                    best = (ms, c)
+        cfg = best[1]
+        log("batch autotune: %s, %.2f ms" % (cfg_dict(cfg), best[0]))
     # visit counts per queue (untimed)
     cb = N.DeviceBuffer(per_queue * 12)
     bytes_step = 0.0
-    node_bytes = 128 if args.bvh_width == 4 else 64
+    node_bytes = node_bytes_of(cfg)
     for sc, rb, hb, n, _, _ in [q[:6] for q in queues]:
         sc.trace_stats(rb, hb, cb, n, cfg=cfg)
         bytes_step += ray_bytes(cb.download(np.uint32, 3 * n).reshape(-1, 3).astype(np.int64).sum(0), n, node_bytes)
-    batch = S.TraceBatch([q[:6] for q in queues])
-    st = N.Stream()
 
     def per_queue_launches():
         for sc, rb, hb, n, _, _ in [q[:6] for q in queues]:
@@ -353,7 +417,7 @@ def run_c4(args):
         "config": {"workload": "c4-256-queues-%d-rays-8-scenes" % rays_step, "queues": n_queues,
                    "rays_per_queue": "log-uniform in [1K, 1M]; half pinhole, half cosine-hemisphere from surfaces",
                    "scene_triangles": sizes,
-                   "trace": "one batched launch per step (imba_trace_batch_launch)"},
+                   "trace": "one batched launch per step (imba_trace_batch_launch)", "trace_config": cfg_dict(cfg)},
         "per_queue_launches": {"value": rays_step * args.steps / wall_q / 1e6, "ms_per_step": wall_q * 1e3 / args.steps,
                                "gpu_ms_per_step": gpu_q / args.steps},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -366,12 +430,70 @@ def run_c4(args):
     return res
 
 
+def run_c5(args):
+    """Serving: Poisson arrivals from 16 sessions over 8 resident sphere-grid scenes (20K to 21M
+    triangles), random cameras, resolutions {256^2, 640x480, 1280x720, 1920x1080}, 1-4 spp, PT;
+    the request loop with cross-session batching against the plain per-request path on the
+    same arrivals."""
+    from imbatracer_amd import native as N
+    from imbatracer_amd import serve as V
+    from imbatracer_amd import workloads as W
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    grids = [(1, 5), (2, 5), (4, 5), (6, 5), (8, 5), (12, 5), (20, 5), (32, 5)]
+    if args.c5_small:
+        grids = [(1, 3), (2, 3), (2, 4), (3, 4), (4, 4), (4, 5), (6, 5), (8, 5)]
+    t0 = time.time()
+    scenes, centres, tris = [], [], []
+    for side, level in grids:
+        sg = W.SphereGridScene(side=side, level=level, threads=threads)
+        scenes.append(V.ServedScene(sg))
+        centres.append((side * 0.5, 0.4, side * 0.5, max(3.0, 1.6 * side)))
+        tris.append(sg.tri_count)
+    log("8 scenes (%s triangles) built in %.1f s" % (tris, time.time() - t0))
+    reqs = V.poisson_requests(args.c5_requests, args.c5_rate, 16, len(scenes), centres, seed=11)
+    cfg = fixed_config(args)
+    arena = args.c5_arena_gb << 30
+    results = {}
+    for name, batched in (("batched", True), ("plain", False)):
+        server = V.Server(scenes, arena, max_active=32, batched=batched, cfg=cfg)
+        # warm-up on a few requests (kernels loaded, batch tables sized)
+        server.run(V.poisson_requests(4, 1000.0, 1, len(scenes), centres, seed=99))
+        server = V.Server(scenes, arena, max_active=32, batched=batched, cfg=cfg)
+        t = time.perf_counter()
+        done = server.run([V.Request(**{**r.__dict__}) for r in reqs], progress=log)
+        wall = time.perf_counter() - t
+        st = V.latency_stats(done)
+        st["steps"] = server.steps
+        st["wall_s"] = wall
+        results[name] = st
+        log("%s: %s" % (name, st))
+    b, p = results["batched"], results["plain"]
+    res = {
+        "metric": METRIC, "value": b["mrays_s"], "unit": "Mrays/s", "n_gpus": 1,
+        "steps": len(reqs), "warmup": 4, "ms_per_step": b["wall_s"] * 1e3 / len(reqs),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded procedural scenes, Poisson arrivals, random cameras)",
+        "config": {"workload": "c5-serving-%d-requests-%.0f-per-s-16-sessions-8-scenes" % (len(reqs), args.c5_rate),
+                   "scene_triangles": tris, "integrator": "pt", "arena_gb": args.c5_arena_gb,
+                   "resolutions": ["256x256", "640x480", "1280x720", "1920x1080"], "spp": "1-4"},
+        "served": b, "plain_per_request": p,
+        "served_vs_plain": {"throughput": b["mrays_s"] / p["mrays_s"], "p99": b["p99_ms"] / p["p99_ms"]},
+        "frame_latency_ms": {"p50": b["p50_ms"], "p99": b["p99_ms"]},
+    }
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pt", choices=["pt", "c1", "c4"])
+    ap.add_argument("--workload", default="pt", choices=["pt", "c1", "c4", "c5"])
+    ap.add_argument("--c5-requests", type=int, default=300, help="c5: requests in the run")
+    ap.add_argument("--c5-rate", type=float, default=200.0, help="c5: mean arrivals per second")
+    ap.add_argument("--c5-arena-gb", type=int, default=48, help="c5: device memory for request buffers")
+    ap.add_argument("--c5-small", action="store_true", help="c5: eight small scenes (quick runs)")
     ap.add_argument("--c4-rays", type=int, default=0,
                     help="c4: rescale the 256 log-uniform queue sizes to this many rays (0: as drawn, about 37M)")
     ap.add_argument("--width", type=int, default=1280)
@@ -387,6 +509,8 @@ def main():
                     help="pt graph frames: k_trace with a device count, or the batched queue kernel")
     ap.add_argument("--bvh-width", type=int, default=0, choices=[0, 2, 4],
                     help="tree the traversal walks: 2 = the scene's BVH2, 4 = the derived 4-wide tree (0 = default)")
+    ap.add_argument("--quantized", action="store_true", help="walk the quantized 4-wide tree")
+    ap.add_argument("--no-autotune", action="store_true", help="use the library's default launch configuration")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -407,7 +531,7 @@ def main():
         N.check(N.lib.imba_memcpy_d2d(b.ptr, a.ptr, nbytes, None), "copy")
         print(json.dumps({"calibrate_copy_bytes": nbytes}))
         return
-    res = {"pt": run_pt, "c1": run_c1, "c4": run_c4}[args.workload](args)
+    res = {"pt": run_pt, "c1": run_c1, "c4": run_c4, "c5": run_c5}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

@@ -75,25 +75,32 @@ int grow_buffer(Scratch& s, size_t bytes) {
 // ---------------------------------------------------------------------------------------
 // The 4-wide tree of a scene (derived on the host, uploaded next to the BVH2).
 int upload_wide(const ImbaSceneDesc& host, const ImbaSceneDesc& dev, const SceneInfo& info, ImbaSceneDesc* dev4,
-                SceneInfo* info4, size_t* bytes) {
+                ImbaSceneDesc* dev4q, SceneInfo* info4, size_t* bytes) {
     WideScene w;
     int r = collapse_scene4(host, w);
     if (r) return r;
     *dev4 = dev;
     dev4->nodes = nullptr;
     dev4->instances = nullptr;
+    *dev4q = *dev4;
     void* pn = nullptr;
     void* pi = nullptr;
+    void* pq = nullptr;
     const size_t nb = sizeof(ImbaNode4) * w.nodes.size(), ib = sizeof(ImbaInstanceNode) * w.instances.size();
-    IMBA_HIP_TRY(hipMalloc(&pn, nb));
-    if (hipMalloc(&pi, ib) != hipSuccess) {
-        hipFree(pn);
-        return fail(IMBA_ERR_OUT_OF_MEMORY, "4-wide instance records");
+    const size_t qb = sizeof(QNode4) * w.qnodes.size();
+    auto cleanup = [&]() {
+        if (pn) hipFree(pn);
+        if (pi) hipFree(pi);
+        if (pq) hipFree(pq);
+    };
+    if (hipMalloc(&pn, nb) != hipSuccess || hipMalloc(&pi, ib) != hipSuccess || (qb && hipMalloc(&pq, qb) != hipSuccess)) {
+        cleanup();
+        return fail(IMBA_ERR_OUT_OF_MEMORY, "4-wide tree");
     }
     if (hipMemcpy(pn, w.nodes.data(), nb, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(pi, w.instances.data(), ib, hipMemcpyHostToDevice) != hipSuccess) {
-        hipFree(pn);
-        hipFree(pi);
+        hipMemcpy(pi, w.instances.data(), ib, hipMemcpyHostToDevice) != hipSuccess ||
+        (qb && hipMemcpy(pq, w.qnodes.data(), qb, hipMemcpyHostToDevice) != hipSuccess)) {
+        cleanup();
         return fail(IMBA_ERR_HIP, "upload of the 4-wide tree failed");
     }
     dev4->nodes = (const ImbaNode*)pn;
@@ -101,17 +108,22 @@ int upload_wide(const ImbaSceneDesc& host, const ImbaSceneDesc& dev, const Scene
     dev4->instances = (const ImbaInstanceNode*)pi;
     dev4->instance_count = (int)w.instances.size();
     dev4->root = w.root;
+    *dev4q = *dev4;
+    dev4q->nodes = (const ImbaNode*)pq;  // NULL when the boxes could not be quantized
     *info4 = info;
     info4->max_stack = w.max_stack;
-    *bytes = nb + ib;
+    *bytes = nb + ib + qb;
     return IMBA_OK;
 }
 
-void free_wide(ImbaSceneDesc& dev4) {
+void free_wide(ImbaSceneDesc& dev4, ImbaSceneDesc& dev4q) {
     if (dev4.nodes) (void)hipFree(const_cast<ImbaNode*>(dev4.nodes));
     if (dev4.instances) (void)hipFree(const_cast<ImbaInstanceNode*>(dev4.instances));
+    if (dev4q.nodes) (void)hipFree(const_cast<ImbaNode*>(dev4q.nodes));
     dev4.nodes = nullptr;
     dev4.instances = nullptr;
+    dev4q.nodes = nullptr;
+    dev4q.instances = nullptr;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -298,6 +310,16 @@ int imba_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
     return copy_impl(dst, src, bytes, hipMemcpyDeviceToDevice, stream);
 }
 
+int imba_memset(void* dst, int value, size_t bytes, void* stream) {
+    if (bytes == 0) return IMBA_OK;
+    IMBA_CHECK_ARG(dst, "NULL pointer");
+    bool sync;
+    hipStream_t s = resolve_stream(stream, &sync);
+    IMBA_HIP_TRY(hipMemsetAsync(dst, value, bytes, s));
+    if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
+    return IMBA_OK;
+}
+
 int imba_stream_create(void** stream) {
     IMBA_CHECK_ARG(stream, "stream is NULL");
     hipStream_t s;
@@ -386,7 +408,7 @@ int imba_scene_upload(const ImbaSceneDesc* h, ImbaScene** out) {
     dd.mask_buffer = (const uint8_t*)p;
     {
         size_t wb = 0;
-        if ((r = upload_wide(*h, dd, info, &s->dev4, &s->info4, &wb))) goto err;
+        if ((r = upload_wide(*h, dd, info, &s->dev4, &s->dev4q, &s->info4, &wb))) goto err;
         s->footprint += wb;
     }
     if (hipMalloc((void**)&s->d_error, sizeof(int)) != hipSuccess) { r = fail(IMBA_ERR_OUT_OF_MEMORY, "error counter"); goto err; }
@@ -405,7 +427,7 @@ int imba_scene_release(ImbaScene* s) {
                            s->dev.masks, s->dev.mask_buffer, s->d_error};
     for (const void* p : owned)
         if (p) (void)hipFree(const_cast<void*>(p));
-    free_wide(s->dev4);
+    free_wide(s->dev4, s->dev4q);
     delete s;
     return IMBA_OK;
 }

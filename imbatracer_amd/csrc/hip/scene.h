@@ -18,8 +18,8 @@ int validate_scene(const ImbaSceneDesc& d, SceneInfo& info);
 // *dev4 = dev with nodes, node_count, instances and root replaced; info4 = info with the
 // 4-wide tree's stack bound.  *bytes receives the device bytes allocated.
 int upload_wide(const ImbaSceneDesc& host, const ImbaSceneDesc& dev, const SceneInfo& info, ImbaSceneDesc* dev4,
-                SceneInfo* info4, size_t* bytes);
-void free_wide(ImbaSceneDesc& dev4);
+                ImbaSceneDesc* dev4q, SceneInfo* info4, size_t* bytes);
+void free_wide(ImbaSceneDesc& dev4, ImbaSceneDesc& dev4q);
 
 }  // namespace imba
 
@@ -32,6 +32,7 @@ struct ImbaScene {
     // own; the other arrays are those of `dev`
     ImbaSceneDesc dev4;
     imba::SceneInfo info4;
+    ImbaSceneDesc dev4q;   // the same tree with quantized nodes (nodes NULL if unavailable)
     int* d_error = nullptr;
     size_t footprint = 0;
 };

@@ -345,10 +345,65 @@ __device__ __forceinline__ void node_step4(const TraceArgs& a, TraceState& s, La
     resolve_leaf<STATS>(a, s, st.sp);
 }
 
+// Child box j of a quantized 4-wide node, one axis: [fma(q_lo, 2^e, o), fma(q_hi, 2^e, o)].
+__device__ __forceinline__ float dequant(uint32_t word, int j, float scale, float origin) {
+    return fmaf((float)((word >> (8 * j)) & 255u), scale, origin);
+}
+
+// One node of the quantized 4-wide tree (64 B: origin, exponents, children, 8-bit child
+// bounds per axis); otherwise as node_step4.  The dequantized boxes contain the exact ones.
+template <int BLOCK, int STACK_LDS, bool STATS>
+__device__ __forceinline__ void node_step4q(const TraceArgs& a, TraceState& s, LaneStack<BLOCK, STACK_LDS>& st) {
+    const int n = s.node;
+    if (n >= a.n_nodes) {
+        s.flags |= kBad;
+        s.node = kNoNode;
+        return;
+    }
+    const uint4* p = reinterpret_cast<const uint4*>(a.nodes) + 4 * (size_t)n;
+    const uint4 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3];
+    if (STATS) s.cn++;
+    const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
+    const float sx = __uint_as_float((uint32_t)((int)(int8_t)(w0.w & 255u) + 127) << 23);
+    const float sy = __uint_as_float((uint32_t)((int)(int8_t)((w0.w >> 8) & 255u) + 127) << 23);
+    const float sz = __uint_as_float((uint32_t)((int)(int8_t)((w0.w >> 16) & 255u) + 127) << 23);
+    const int4 ch = make_int4((int)w1.x, (int)w1.y, (int)w1.z, (int)w1.w);
+    float d[4];
+    bool h[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int c = j == 0 ? ch.x : j == 1 ? ch.y : j == 2 ? ch.z : ch.w;
+        h[j] = c != 0 && box_hit(s.cs, dequant(w2.x, j, sx, ox), dequant(w2.y, j, sx, ox), dequant(w2.z, j, sy, oy),
+                                 dequant(w2.w, j, sy, oy), dequant(w3.x, j, sz, oz), dequant(w3.y, j, sz, oz), s.tmin,
+                                 s.best_t, d[j]);
+        if (h[j] && c > 0 && c <= n) { h[j] = false; s.flags |= kBad; }
+    }
+    const int cnt = (int)h[0] + (int)h[1] + (int)h[2] + (int)h[3];
+    const float inf = __int_as_float(0x7f800000);
+    float e0 = h[0] ? d[0] : inf, e1 = h[1] ? d[1] : inf, e2 = h[2] ? d[2] : inf, e3 = h[3] ? d[3] : inf;
+    int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+    order2(e0, c0, e1, c1);
+    order2(e2, c2, e3, c3);
+    order2(e0, c0, e2, c2);
+    order2(e1, c1, e3, c3);
+    order2(e1, c1, e2, c2);
+    if (cnt == 0) {
+        s.node = kNoNode;
+    } else {
+        bool ok = true;
+        if (cnt >= 4) ok &= st.push(c3);
+        if (cnt >= 3) ok &= st.push(c2);
+        if (cnt >= 2) ok &= st.push(c1);
+        if (!ok) s.flags |= kBad;
+        s.node = c0;
+    }
+    resolve_leaf<STATS>(a, s, st.sp);
+}
+
 // Advances one ray by one step: bookkeeping (next instance, back to world space, pop) if
 // nothing is pending, then one triangle test or one inner node.  Returns true when the ray
 // is finished.
-template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS, bool WIDE>
+template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS, int TREE>
 __device__ __forceinline__ bool advance(const TraceArgs& a, TraceState& s, LaneStack<BLOCK, STACK_LDS>& st) {
     if (s.tri < 0 && s.node == kNoNode) {
         // the current instance's tree is exhausted: next instance of the leaf or world space
@@ -412,7 +467,8 @@ __device__ __forceinline__ bool advance(const TraceArgs& a, TraceState& s, LaneS
         return false;
     }
     if (s.node >= 0) {
-        if (WIDE) node_step4<BLOCK, STACK_LDS, STATS>(a, s, st);
+        if (TREE == 2) node_step4q<BLOCK, STACK_LDS, STATS>(a, s, st);
+        else if (TREE == 1) node_step4<BLOCK, STACK_LDS, STATS>(a, s, st);
         else node_step2<BLOCK, STACK_LDS, STATS>(a, s, st);
     }
     return false;
@@ -441,7 +497,7 @@ __device__ __forceinline__ void write_result(const TraceArgs& a, const TraceStat
 }
 
 // One ray per thread, rays strided over a grid sized to the resident waves.
-template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS, bool WIDE>
+template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS, int TREE>
 __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     __shared__ int lds_stack[STACK_LDS * BLOCK];
     LaneStack<BLOCK, STACK_LDS> st;
@@ -452,7 +508,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
         TraceState s;
         state_init(s, a.rays[2 * (size_t)id], a.rays[2 * (size_t)id + 1], a.root);
         st.sp = 0;
-        while (!advance<BLOCK, STACK_LDS, ANY, MASKED, STATS, WIDE>(a, s, st)) {
+        while (!advance<BLOCK, STACK_LDS, ANY, MASKED, STATS, TREE>(a, s, st)) {
         }
         write_result<STATS>(a, s, id);
     }
@@ -463,7 +519,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
 // on a global counter (ballot + popcount + mbcnt).  Each iteration advances every active lane
 // by one step of `advance`, so the per-ray results are those of k_trace.
 // counter[0] = next ray, counter[1] = finished waves; the last wave to finish resets both.
-template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS, bool WIDE>
+template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS, int TREE>
 __global__ __launch_bounds__(BLOCK) void k_trace_dyn(TraceArgs a, int refill) {
     __shared__ int lds_stack[STACK_LDS * BLOCK];
     LaneStack<BLOCK, STACK_LDS> st;
@@ -497,7 +553,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dyn(TraceArgs a, int refill) {
             if (exhausted) break;
             continue;
         }
-        if (id >= 0 && advance<BLOCK, STACK_LDS, ANY, MASKED, STATS, WIDE>(a, s, st)) {
+        if (id >= 0 && advance<BLOCK, STACK_LDS, ANY, MASKED, STATS, TREE>(a, s, st)) {
             write_result<STATS>(a, s, id);
             id = -1;
         }
@@ -547,8 +603,8 @@ __global__ void k_batch_prefix(BatchItem* items, BatchGroups groups, int* group_
     }
 }
 
-template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool WIDE>
-__global__ __launch_bounds__(BLOCK, (ANY || MASKED || WIDE) ? 1 : 8) void k_trace_batch(const BatchItem* __restrict__ items, int first, int n_items,
+template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, int TREE>
+__global__ __launch_bounds__(BLOCK, (ANY || MASKED || TREE != 0) ? 1 : 8) void k_trace_batch(const BatchItem* __restrict__ items, int first, int n_items,
                                                        const int* __restrict__ total_chunks, int* counter, int* spill,
                                                        int spill_depth) {
     __shared__ int lds_stack[STACK_LDS * BLOCK];
@@ -582,7 +638,7 @@ __global__ __launch_bounds__(BLOCK, (ANY || MASKED || WIDE) ? 1 : 8) void k_trac
             TraceState s;
             state_init(s, a.rays[2 * (size_t)id], a.rays[2 * (size_t)id + 1], a.root);
             st.sp = 0;
-            while (!advance<BLOCK, STACK_LDS, ANY, MASKED, false, WIDE>(a, s, st)) {
+            while (!advance<BLOCK, STACK_LDS, ANY, MASKED, false, TREE>(a, s, st)) {
             }
             write_result<false>(a, s, id);
         }

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -53,14 +54,14 @@ int not_capturing(hipStream_t s, const char* what) {
     return IMBA_OK;
 }
 
-template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS, bool WIDE>
+template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS, int TREE>
 int launch_one(const TraceArgs& args_in, int max_stack, int blocks_per_cu, int refill, hipStream_t s) {
     TraceArgs a = args_in;
     const DeviceProps* p = device_props();
     if (!p) return fail(IMBA_ERR_NO_DEVICE, "no HIP device");
     const bool dyn = refill > 0;
-    void (*kern)(TraceArgs) = k_trace<BLOCK, STACK_LDS, ANY, MASKED, STATS, WIDE>;
-    void (*kern_dyn)(TraceArgs, int) = k_trace_dyn<BLOCK, STACK_LDS, ANY, MASKED, STATS, WIDE>;
+    void (*kern)(TraceArgs) = k_trace<BLOCK, STACK_LDS, ANY, MASKED, STATS, TREE>;
+    void (*kern_dyn)(TraceArgs, int) = k_trace_dyn<BLOCK, STACK_LDS, ANY, MASKED, STATS, TREE>;
     int resident = 0;
     if (dyn)
         IMBA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern_dyn, BLOCK, 0));
@@ -107,23 +108,30 @@ constexpr int kDefaultBlock = 256;
 constexpr int kDefaultStack = 16;
 constexpr int kDefaultWidth = 2;
 
-// Tree width a configuration asks for (2 or 4); 0 for an invalid value.
-int width_of(const ImbaTraceConfig* cfg) {
+// Tree a configuration asks for: 0 the BVH2, 1 the 4-wide tree, 2 its quantized form; -1 if
+// the configuration is invalid.
+int tree_code(const ImbaTraceConfig* cfg) {
     const int w = cfg && cfg->bvh_width ? cfg->bvh_width : kDefaultWidth;
-    return (w == 2 || w == 4) ? w : 0;
+    if (w == 2) return (cfg && cfg->quantized) ? -1 : 0;
+    if (w == 4) return (cfg && cfg->quantized) ? 2 : 1;
+    return -1;
+}
+int width_of(const ImbaTraceConfig* cfg) {
+    const int t = tree_code(cfg);
+    return t < 0 ? 0 : (t == 0 ? 2 : 4);
 }
 
-template <bool ANY, bool MASKED, bool STATS, bool WIDE>
+template <bool ANY, bool MASKED, bool STATS, int TREE>
 int dispatch_shape(const TraceArgs& a, int max_stack, const ImbaTraceConfig* cfg, hipStream_t s) {
     const int block = cfg && cfg->block_size ? cfg->block_size : kDefaultBlock;
     const int stack = cfg && cfg->stack_lds ? cfg->stack_lds : kDefaultStack;
     const int bpc = cfg ? cfg->blocks_per_cu : 0;
     const int refill = cfg ? cfg->persistent : 0;
     if (STATS) {  // visit counts do not depend on the launch shape
-        return launch_one<kDefaultBlock, kDefaultStack, ANY, MASKED, STATS, WIDE>(a, max_stack, 0, 0, s);
+        return launch_one<kDefaultBlock, kDefaultStack, ANY, MASKED, STATS, TREE>(a, max_stack, 0, 0, s);
     } else {
 #define IMBA_SHAPE(B, S) \
-    if (block == B && stack == S) return launch_one<B, S, ANY, MASKED, STATS, WIDE>(a, max_stack, bpc, refill, s);
+    if (block == B && stack == S) return launch_one<B, S, ANY, MASKED, STATS, TREE>(a, max_stack, bpc, refill, s);
         IMBA_SHAPE(256, 16)
         IMBA_SHAPE(256, 8)
         IMBA_SHAPE(256, 32)
@@ -137,20 +145,21 @@ int dispatch_shape(const TraceArgs& a, int max_stack, const ImbaTraceConfig* cfg
                 "(supported: 256/16, 256/8, 256/32, 128/16, 128/32, 64/32, 64/64)", block, stack);
 }
 
-template <bool STATS, bool WIDE>
+template <bool STATS, int TREE>
 int dispatch_flags(const TraceArgs& a, int max_stack, bool any, bool masked, const ImbaTraceConfig* cfg,
                    hipStream_t s) {
-    return any ? (masked ? dispatch_shape<true, true, STATS, WIDE>(a, max_stack, cfg, s)
-                         : dispatch_shape<true, false, STATS, WIDE>(a, max_stack, cfg, s))
-               : (masked ? dispatch_shape<false, true, STATS, WIDE>(a, max_stack, cfg, s)
-                         : dispatch_shape<false, false, STATS, WIDE>(a, max_stack, cfg, s));
+    return any ? (masked ? dispatch_shape<true, true, STATS, TREE>(a, max_stack, cfg, s)
+                         : dispatch_shape<true, false, STATS, TREE>(a, max_stack, cfg, s))
+               : (masked ? dispatch_shape<false, true, STATS, TREE>(a, max_stack, cfg, s)
+                         : dispatch_shape<false, false, STATS, TREE>(a, max_stack, cfg, s));
 }
 
-// Scene arrays of one tree width: the BVH2 as given or the derived 4-wide tree.
+// Scene arrays of one tree: the BVH2 as given (0), the derived 4-wide tree (1) or its quantized
+// form (2).
 struct TreeView {
     const ImbaSceneDesc* d;
     const SceneInfo* info;
-    bool wide;
+    int tree;
 };
 
 int launch_trace(const TreeView& t, int* d_err, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts, int count,
@@ -179,11 +188,13 @@ int launch_trace(const TreeView& t, int* d_err, const ImbaRay* rays, ImbaHit* hi
     const int ms = t.info->max_stack;
     int r;
     if (counts) {
-        r = t.wide ? dispatch_flags<true, true>(a, ms, any, masked, nullptr, s)
-                   : dispatch_flags<true, false>(a, ms, any, masked, nullptr, s);
+        r = t.tree == 2 ? dispatch_flags<true, 2>(a, ms, any, masked, nullptr, s)
+          : t.tree == 1 ? dispatch_flags<true, 1>(a, ms, any, masked, nullptr, s)
+                        : dispatch_flags<true, 0>(a, ms, any, masked, nullptr, s);
     } else {
-        r = t.wide ? dispatch_flags<false, true>(a, ms, any, masked, cfg, s)
-                   : dispatch_flags<false, false>(a, ms, any, masked, cfg, s);
+        r = t.tree == 2 ? dispatch_flags<false, 2>(a, ms, any, masked, cfg, s)
+          : t.tree == 1 ? dispatch_flags<false, 1>(a, ms, any, masked, cfg, s)
+                        : dispatch_flags<false, 0>(a, ms, any, masked, cfg, s);
     }
     if (r) return r;
     if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
@@ -191,11 +202,17 @@ int launch_trace(const TreeView& t, int* d_err, const ImbaRay* rays, ImbaHit* hi
 }
 
 // The tree of `s` that `cfg` asks for.
-int tree_of(const ImbaScene* s, const ImbaTraceConfig* cfg, TreeView* t) {
-    const int w = width_of(cfg);
-    if (!w) return fail(IMBA_ERR_INVALID_ARG, "bvh_width must be 0, 2 or 4");
-    *t = w == 4 ? TreeView{&s->dev4, &s->info4, true} : TreeView{&s->dev, &s->info, false};
+int pick_tree(const ImbaTraceConfig* cfg, const ImbaSceneDesc* d2, const SceneInfo* i2, const ImbaSceneDesc* d4,
+              const ImbaSceneDesc* d4q, const SceneInfo* i4, TreeView* t) {
+    const int code = tree_code(cfg);
+    if (code < 0) return fail(IMBA_ERR_INVALID_ARG, "bvh_width must be 0, 2 or 4 (quantized only with 4)");
+    if (code == 2 && !d4q->nodes) return fail(IMBA_ERR_INVALID_ARG, "the scene has no quantized 4-wide tree");
+    *t = code == 0 ? TreeView{d2, i2, 0} : code == 1 ? TreeView{d4, i4, 1} : TreeView{d4q, i4, 2};
     return IMBA_OK;
+}
+
+int tree_of(const ImbaScene* s, const ImbaTraceConfig* cfg, TreeView* t) {
+    return pick_tree(cfg, &s->dev, &s->info, &s->dev4, &s->dev4q, &s->info4, t);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -217,15 +234,13 @@ struct FfiScene {
     ImbaSceneDesc dev;
     SceneInfo info;
     ImbaSceneDesc dev4;  // derived 4-wide tree (nodes and instance records owned)
+    ImbaSceneDesc dev4q;
     SceneInfo info4;
     int* d_err = nullptr;
 };
 
 int ffi_tree(FfiScene* fs, const ImbaTraceConfig* cfg, TreeView* t) {
-    const int w = width_of(cfg);
-    if (!w) return fail(IMBA_ERR_INVALID_ARG, "bvh_width must be 0, 2 or 4");
-    *t = w == 4 ? TreeView{&fs->dev4, &fs->info4, true} : TreeView{&fs->dev, &fs->info, false};
-    return IMBA_OK;
+    return pick_tree(cfg, &fs->dev, &fs->info, &fs->dev4, &fs->dev4q, &fs->info4, t);
 }
 std::mutex g_ffi_mutex;
 std::map<FfiKey, FfiScene> g_ffi_scenes;
@@ -297,7 +312,7 @@ int ffi_scene(const ImbaSceneDesc& given, FfiScene** out) {
     fs.dev.masks = (const ImbaTransparencyMask*)masks;
     fs.dev.mask_buffer = (const uint8_t*)mask_buffer;
     size_t wide_bytes = 0;
-    r = upload_wide(h, fs.dev, fs.info, &fs.dev4, &fs.info4, &wide_bytes);
+    r = upload_wide(h, fs.dev, fs.info, &fs.dev4, &fs.dev4q, &fs.info4, &wide_bytes);
     if (r) return r;
     IMBA_HIP_TRY(hipMalloc((void**)&fs.d_err, sizeof(int)));
     IMBA_HIP_TRY(hipMemset(fs.d_err, 0, sizeof(int)));
@@ -353,7 +368,12 @@ struct ImbaTraceBatch {
     imba::BatchGroups groups{};
     int max_stack[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};  // per tree width (2, 4) and group
     long long capacity_chunks[4] = {0, 0, 0, 0};
-    imba::BatchItem* d_items[2] = {nullptr, nullptr};      // queue tables for the BVH2 and the 4-wide tree
+    imba::BatchItem* d_items[3] = {nullptr, nullptr, nullptr};  // queue tables per tree (BVH2, 4-wide, quantized)
+    bool quantized_ok = true;                              // every queue's scene has the quantized tree
+    int cap = 0;                                           // queues the tables hold room for
+    imba::BatchItem* staging[2] = {nullptr, nullptr};      // pinned host copies of both tables
+    hipEvent_t staged[2] = {nullptr, nullptr};             // last copy out of each staging buffer
+    int next_staging = 0;
     int* d_group_chunks = nullptr;  // 4
     int* d_counters = nullptr;      // 2 per group
     imba::Scratch spill[4];
@@ -362,12 +382,12 @@ struct ImbaTraceBatch {
 namespace imba {
 namespace {
 
-template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool WIDE>
+template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, int TREE>
 int launch_batch_group(ImbaTraceBatch* b, int g, int blocks_per_cu, hipStream_t s) {
     const DeviceProps* p = device_props();
     if (!p) return fail(IMBA_ERR_NO_DEVICE, "no HIP device");
-    auto kern = k_trace_batch<BLOCK, STACK_LDS, ANY, MASKED, WIDE>;
-    const int max_stack = b->max_stack[WIDE ? 1 : 0][g];
+    auto kern = k_trace_batch<BLOCK, STACK_LDS, ANY, MASKED, TREE>;
+    const int max_stack = b->max_stack[TREE ? 1 : 0][g];
     int resident = 0;
     IMBA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern, BLOCK, 0));
     if (resident <= 0) resident = 1;
@@ -387,20 +407,20 @@ int launch_batch_group(ImbaTraceBatch* b, int g, int blocks_per_cu, hipStream_t 
         }
         spill = (int*)b->spill[g].ptr;
     }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, s, b->d_items[WIDE ? 1 : 0], b->groups.first[g],
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, s, b->d_items[TREE], b->groups.first[g],
                        b->groups.first[g + 1] - b->groups.first[g], b->d_group_chunks + g, b->d_counters + 2 * g,
                        spill, spill_depth);
     IMBA_HIP_TRY(hipGetLastError());
     return IMBA_OK;
 }
 
-template <bool ANY, bool MASKED, bool WIDE>
+template <bool ANY, bool MASKED, int TREE>
 int dispatch_batch(ImbaTraceBatch* b, int g, const ImbaTraceConfig* cfg, hipStream_t s) {
     const int block = cfg && cfg->block_size ? cfg->block_size : kDefaultBlock;
     const int stack = cfg && cfg->stack_lds ? cfg->stack_lds : kDefaultStack;
     const int bpc = cfg ? cfg->blocks_per_cu : 0;
 #define IMBA_SHAPE(B, S) \
-    if (block == B && stack == S) return launch_batch_group<B, S, ANY, MASKED, WIDE>(b, g, bpc, s);
+    if (block == B && stack == S) return launch_batch_group<B, S, ANY, MASKED, TREE>(b, g, bpc, s);
     IMBA_SHAPE(256, 16)
     IMBA_SHAPE(256, 8)
     IMBA_SHAPE(256, 32)
@@ -415,12 +435,111 @@ int dispatch_batch(ImbaTraceBatch* b, int g, const ImbaTraceConfig* cfg, hipStre
 
 void free_batch(ImbaTraceBatch* b) {
     if (!b) return;
-    hipFree(b->d_items[0]);
-    hipFree(b->d_items[1]);
+    for (auto* t : b->d_items) hipFree(t);
     hipFree(b->d_group_chunks);
     hipFree(b->d_counters);
     for (auto& sp : b->spill) hipFree(sp.ptr);
+    for (int k = 0; k < 2; k++) {
+        if (b->staging[k]) hipHostFree(b->staging[k]);
+        if (b->staged[k]) hipEventDestroy(b->staged[k]);
+    }
     delete b;
+}
+
+// Builds the group-contiguous queue tables (one per tree width) and puts them in device memory:
+// with a stream, through pinned staging buffers and copies on that stream (two staging buffers
+// used in turn, each reused only after its previous copy has completed); without, synchronously.
+int set_tables(ImbaTraceBatch* b, const ImbaTraceItem* items, int n_items, hipStream_t stream) {
+    std::vector<BatchItem> table, table4, table4q;
+    table.reserve(n_items);
+    table4.reserve(n_items);
+    table4q.reserve(n_items);
+    bool quantized_ok = true;
+    BatchGroups groups{};
+    int max_stack[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    long long chunks[4] = {0, 0, 0, 0};
+    for (int g = 0; g < 4; g++) {
+        groups.first[g] = (int)table.size();
+        for (int i = 0; i < n_items; i++) {
+            const ImbaTraceItem& it = items[i];
+            if (!it.scene || it.count < 0 || (it.count > 0 && (!it.rays || !it.hits)))
+                return fail(IMBA_ERR_INVALID_ARG, "queue %d: scene NULL, count < 0 or NULL rays/hits", i);
+            if (it.scene->device != b->device)
+                return fail(IMBA_ERR_INVALID_ARG, "queue %d: scene lives on device %d, not the batch's device %d", i,
+                            it.scene->device, b->device);
+            const int masked = !it.scene->info.all_opaque;
+            if (2 * (it.any_hit != 0) + masked != g) continue;
+            BatchItem e{};
+            e.a = scene_args(it.scene->dev, it.scene->d_error);
+            e.a.rays = (const float4*)it.rays;
+            e.a.hits = (int4*)it.hits;
+            e.a.count = it.count;
+            e.count_dev = it.count_dev;
+            e.capacity = it.count;
+            e.group = g;
+            max_stack[0][g] = std::max(max_stack[0][g], it.scene->info.max_stack);
+            max_stack[1][g] = std::max(max_stack[1][g], it.scene->info4.max_stack);
+            chunks[g] += ceil_div(it.count, 64);
+            table.push_back(e);
+            const TraceArgs a4 = scene_args(it.scene->dev4, it.scene->d_error);
+            e.a.nodes = a4.nodes;
+            e.a.n_nodes = a4.n_nodes;
+            e.a.inst = a4.inst;
+            e.a.n_inst = a4.n_inst;
+            e.a.root = a4.root;
+            table4.push_back(e);
+            if (it.scene->dev4q.nodes) e.a.nodes = (const float4*)it.scene->dev4q.nodes;
+            else quantized_ok = false;
+            table4q.push_back(e);
+        }
+    }
+    groups.first[4] = (int)table.size();
+    const int n = (int)table.size();
+    const size_t bytes = sizeof(BatchItem) * (size_t)n;
+    if (n > b->cap) {
+        // grow: device tables and staging buffers (synchronous)
+        IMBA_HIP_TRY(hipDeviceSynchronize());
+        for (int k = 0; k < 3; k++) {
+            if (b->d_items[k]) IMBA_HIP_TRY(hipFree(b->d_items[k]));
+            b->d_items[k] = nullptr;
+        }
+        for (int k = 0; k < 2; k++) {
+            if (b->staging[k]) IMBA_HIP_TRY(hipHostFree(b->staging[k]));
+            b->staging[k] = nullptr;
+        }
+        const int cap = std::max(n, 2 * b->cap);
+        for (int k = 0; k < 3; k++) IMBA_HIP_TRY(hipMalloc((void**)&b->d_items[k], sizeof(BatchItem) * (size_t)cap));
+        for (int k = 0; k < 2; k++) {
+            IMBA_HIP_TRY(hipHostMalloc((void**)&b->staging[k], 3 * sizeof(BatchItem) * (size_t)cap));
+            if (!b->staged[k]) IMBA_HIP_TRY(hipEventCreateWithFlags(&b->staged[k], hipEventDisableTiming));
+        }
+        b->cap = cap;
+    }
+    if (n > 0) {
+        if (stream) {
+            const int k = b->next_staging;
+            b->next_staging ^= 1;
+            IMBA_HIP_TRY(hipEventSynchronize(b->staged[k]));
+            BatchItem* st = b->staging[k];
+            std::memcpy(st, table.data(), bytes);
+            std::memcpy(st + n, table4.data(), bytes);
+            std::memcpy(st + 2 * n, table4q.data(), bytes);
+            IMBA_HIP_TRY(hipMemcpyAsync(b->d_items[0], st, bytes, hipMemcpyHostToDevice, stream));
+            IMBA_HIP_TRY(hipMemcpyAsync(b->d_items[1], st + n, bytes, hipMemcpyHostToDevice, stream));
+            IMBA_HIP_TRY(hipMemcpyAsync(b->d_items[2], st + 2 * n, bytes, hipMemcpyHostToDevice, stream));
+            IMBA_HIP_TRY(hipEventRecord(b->staged[k], stream));
+        } else {
+            IMBA_HIP_TRY(hipMemcpy(b->d_items[0], table.data(), bytes, hipMemcpyHostToDevice));
+            IMBA_HIP_TRY(hipMemcpy(b->d_items[1], table4.data(), bytes, hipMemcpyHostToDevice));
+            IMBA_HIP_TRY(hipMemcpy(b->d_items[2], table4q.data(), bytes, hipMemcpyHostToDevice));
+        }
+    }
+    b->groups = groups;
+    b->n = n;
+    b->quantized_ok = quantized_ok;
+    std::memcpy(b->max_stack, max_stack, sizeof(max_stack));
+    std::memcpy(b->capacity_chunks, chunks, sizeof(chunks));
+    return IMBA_OK;
 }
 
 }  // namespace
@@ -434,68 +553,38 @@ int imba_trace_batch_create(const ImbaTraceItem* items, int n_items, ImbaTraceBa
     IMBA_CHECK_ARG(out && (items || n_items == 0), "NULL argument");
     IMBA_CHECK_ARG(n_items >= 0, "n_items < 0");
     *out = nullptr;
-    std::vector<BatchItem> table, table4;
-    table.reserve(n_items);
-    table4.reserve(n_items);
     auto* b = new ImbaTraceBatch();
-    IMBA_HIP_TRY(hipGetDevice(&b->device));
-    // group-contiguous table, queues in the caller's order within a group
-    for (int g = 0; g < 4; g++) {
-        b->groups.first[g] = (int)table.size();
-        for (int i = 0; i < n_items; i++) {
-            const ImbaTraceItem& it = items[i];
-            if (!it.scene || it.count < 0 || (it.count > 0 && (!it.rays || !it.hits))) {
-                delete b;
-                return fail(IMBA_ERR_INVALID_ARG, "queue %d: scene NULL, count < 0 or NULL rays/hits", i);
-            }
-            if (it.scene->device != b->device) {
-                delete b;
-                return fail(IMBA_ERR_INVALID_ARG, "queue %d: scene lives on device %d, not the current device %d", i,
-                            it.scene->device, b->device);
-            }
-            const int masked = !it.scene->info.all_opaque;
-            if (2 * (it.any_hit != 0) + masked != g) continue;
-            BatchItem e{};
-            e.a = scene_args(it.scene->dev, it.scene->d_error);
-            e.a.rays = (const float4*)it.rays;
-            e.a.hits = (int4*)it.hits;
-            e.a.count = it.count;
-            e.count_dev = it.count_dev;
-            e.capacity = it.count;
-            e.group = g;
-            b->max_stack[0][g] = std::max(b->max_stack[0][g], it.scene->info.max_stack);
-            b->max_stack[1][g] = std::max(b->max_stack[1][g], it.scene->info4.max_stack);
-            b->capacity_chunks[g] += ceil_div(it.count, 64);
-            table.push_back(e);
-            const TraceArgs a4 = scene_args(it.scene->dev4, it.scene->d_error);
-            e.a.nodes = a4.nodes;
-            e.a.n_nodes = a4.n_nodes;
-            e.a.inst = a4.inst;
-            e.a.n_inst = a4.n_inst;
-            e.a.root = a4.root;
-            table4.push_back(e);
-        }
-    }
-    b->groups.first[4] = (int)table.size();
-    b->n = (int)table.size();
-    hipError_t e = hipSuccess;
-    if (!table.empty()) e = hipMalloc((void**)&b->d_items[0], sizeof(BatchItem) * table.size());
-    if (e == hipSuccess && !table.empty()) e = hipMalloc((void**)&b->d_items[1], sizeof(BatchItem) * table.size());
+    hipError_t e = hipGetDevice(&b->device);
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_group_chunks, sizeof(int) * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_counters, sizeof(int) * 8);
-    if (e == hipSuccess && !table.empty())
-        e = hipMemcpy(b->d_items[0], table.data(), sizeof(BatchItem) * table.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess && !table.empty())
-        e = hipMemcpy(b->d_items[1], table4.data(), sizeof(BatchItem) * table4.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(b->d_counters, 0, sizeof(int) * 8);
     if (e == hipSuccess) e = hipMemset(b->d_group_chunks, 0, sizeof(int) * 4);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         free_batch(b);
-        return fail(e == hipErrorOutOfMemory ? IMBA_ERR_OUT_OF_MEMORY : IMBA_ERR_HIP, "batch upload failed: %s",
+        return fail(e == hipErrorOutOfMemory ? IMBA_ERR_OUT_OF_MEMORY : IMBA_ERR_HIP, "batch setup failed: %s",
                     hipGetErrorString(e));
     }
+    int r = set_tables(b, items, n_items, nullptr);
+    if (r) {
+        free_batch(b);
+        return r;
+    }
     *out = b;
+    return IMBA_OK;
+}
+
+int imba_trace_batch_update(ImbaTraceBatch* b, const ImbaTraceItem* items, int n_items, void* stream) {
+    IMBA_CHECK_ARG(b && (items || n_items == 0), "NULL argument");
+    IMBA_CHECK_ARG(n_items >= 0, "n_items < 0");
+    int dev = 0;
+    IMBA_HIP_TRY(hipGetDevice(&dev));
+    if (dev != b->device) return fail(IMBA_ERR_INVALID_ARG, "batch belongs to device %d, current is %d", b->device, dev);
+    bool sync;
+    hipStream_t s = resolve_stream(stream, &sync);
+    int r = set_tables(b, items, n_items, s);
+    if (r) return r;
+    if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
     return IMBA_OK;
 }
 
@@ -505,26 +594,29 @@ int imba_trace_batch_launch(ImbaTraceBatch* b, const ImbaTraceConfig* cfg, void*
     IMBA_HIP_TRY(hipGetDevice(&dev));
     if (dev != b->device) return fail(IMBA_ERR_INVALID_ARG, "batch belongs to device %d, current is %d", b->device, dev);
     if (b->n == 0) return IMBA_OK;
-    const int width = width_of(cfg);
-    if (!width) return fail(IMBA_ERR_INVALID_ARG, "bvh_width must be 0, 2 or 4");
-    const bool wide = width == 4;
+    const int tree = tree_code(cfg);
+    if (tree < 0) return fail(IMBA_ERR_INVALID_ARG, "bvh_width must be 0, 2 or 4 (quantized only with 4)");
+    if (tree == 2 && !b->quantized_ok) return fail(IMBA_ERR_INVALID_ARG, "a queue's scene has no quantized tree");
     bool sync;
     hipStream_t s = resolve_stream(stream, &sync);
-    hipLaunchKernelGGL(k_batch_prefix, dim3(1), dim3(64), 0, s, b->d_items[wide ? 1 : 0], b->groups,
-                       b->d_group_chunks);
+    hipLaunchKernelGGL(k_batch_prefix, dim3(1), dim3(64), 0, s, b->d_items[tree], b->groups, b->d_group_chunks);
     IMBA_HIP_TRY(hipGetLastError());
     for (int g = 0; g < 4; g++) {
         if (b->groups.first[g + 1] == b->groups.first[g] || b->capacity_chunks[g] == 0) continue;
         int r;
-        switch (g + 4 * wide) {
-            case 0: r = dispatch_batch<false, false, false>(b, g, cfg, s); break;
-            case 1: r = dispatch_batch<false, true, false>(b, g, cfg, s); break;
-            case 2: r = dispatch_batch<true, false, false>(b, g, cfg, s); break;
-            case 3: r = dispatch_batch<true, true, false>(b, g, cfg, s); break;
-            case 4: r = dispatch_batch<false, false, true>(b, g, cfg, s); break;
-            case 5: r = dispatch_batch<false, true, true>(b, g, cfg, s); break;
-            case 6: r = dispatch_batch<true, false, true>(b, g, cfg, s); break;
-            default: r = dispatch_batch<true, true, true>(b, g, cfg, s); break;
+        switch (g + 4 * tree) {
+            case 0: r = dispatch_batch<false, false, 0>(b, g, cfg, s); break;
+            case 1: r = dispatch_batch<false, true, 0>(b, g, cfg, s); break;
+            case 2: r = dispatch_batch<true, false, 0>(b, g, cfg, s); break;
+            case 3: r = dispatch_batch<true, true, 0>(b, g, cfg, s); break;
+            case 4: r = dispatch_batch<false, false, 1>(b, g, cfg, s); break;
+            case 5: r = dispatch_batch<false, true, 1>(b, g, cfg, s); break;
+            case 6: r = dispatch_batch<true, false, 1>(b, g, cfg, s); break;
+            case 7: r = dispatch_batch<true, true, 1>(b, g, cfg, s); break;
+            case 8: r = dispatch_batch<false, false, 2>(b, g, cfg, s); break;
+            case 9: r = dispatch_batch<false, true, 2>(b, g, cfg, s); break;
+            case 10: r = dispatch_batch<true, false, 2>(b, g, cfg, s); break;
+            default: r = dispatch_batch<true, true, 2>(b, g, cfg, s); break;
         }
         if (r) return r;
     }
@@ -538,7 +630,7 @@ int imba_autotune(const ImbaScene* sc, const ImbaRay* rays, ImbaHit* hits, int c
     IMBA_CHECK_ARG(count > 0 && reps > 0, "count and reps must be positive");
     static const int kShapes[][2] = {{256, 16}, {256, 8}, {256, 32}, {128, 16}, {128, 32}, {64, 32}, {64, 64}};
     static const int kRefill[] = {0, 32};
-    static const int kWidth[] = {2, 4};
+    static const int kTree[][2] = {{2, 0}, {4, 0}, {4, 1}};  // bvh_width, quantized
     bool sync;
     hipStream_t s = resolve_stream(stream, &sync);
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -546,10 +638,11 @@ int imba_autotune(const ImbaScene* sc, const ImbaRay* rays, ImbaHit* hits, int c
     IMBA_HIP_TRY(hipEventCreate(&e1));
     float best_t = -1.0f;
     int r = IMBA_OK;
-    for (int width : kWidth)
+    for (const auto& tree : kTree)
     for (const auto& shape : kShapes) {
         for (int refill : kRefill) {
-            ImbaTraceConfig cfg{shape[0], shape[1], refill, 0, width};
+            ImbaTraceConfig cfg{shape[0], shape[1], refill, 0, tree[0], tree[1]};
+            if (tree[1] && !sc->dev4q.nodes) continue;
             TreeView t{};
             r = tree_of(sc, &cfg, &t);
             if (!r)
@@ -624,14 +717,14 @@ int imba_occluded_dev(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, in
 int imba_intersect_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts, int count,
                          void* stream) {
     IMBA_CHECK_ARG(counts, "counts is NULL");
-    const ImbaTraceConfig bvh2{0, 0, 0, 0, 2};
+    const ImbaTraceConfig bvh2{0, 0, 0, 0, 2, 0};
     return trace_entry(s, rays, hits, counts, count, false, &bvh2, stream, nullptr);
 }
 
 int imba_occluded_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts, int count,
                         void* stream) {
     IMBA_CHECK_ARG(counts, "counts is NULL");
-    const ImbaTraceConfig bvh2{0, 0, 0, 0, 2};
+    const ImbaTraceConfig bvh2{0, 0, 0, 0, 2, 0};
     return trace_entry(s, rays, hits, counts, count, true, &bvh2, stream, nullptr);
 }
 

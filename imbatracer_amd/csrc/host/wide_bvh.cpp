@@ -11,6 +11,7 @@
 // follows its parent); mesh trees come first (in increasing BVH2 root order), then the top
 // level.  Instance records are copied with `next` pointing at the mesh's 4-wide root.
 #include <algorithm>
+#include <cmath>
 #include <cfloat>
 #include <cstring>
 #include <map>
@@ -141,7 +142,78 @@ int collapse_scene4(const ImbaSceneDesc& d, WideScene& out) {
     for (int i = 0; i < d.instance_count; i++) out.instances[i].next = used[i] ? root_map[d.instances[i].next] : -1;
     // stack entries of the top level stay while a ray is inside a mesh tree
     out.max_stack = top_pushes + mesh_pushes + 2;
+    quantize_nodes(out);
     return IMBA_OK;
+}
+
+namespace {
+
+float pow2(int e) { return std::ldexp(1.0f, e); }
+
+// Smallest exponent e >= -100 with 255 * 2^e >= extent, and quantized bounds of every child
+// such that the dequantized box (computed as the kernel does) contains the exact box.
+bool quantize_axis(const float* lo, const float* hi, int k, float origin, float extent, int8_t* exp_out, uint8_t* qlo,
+                   uint8_t* qhi) {
+    if (!std::isfinite(origin) || !std::isfinite(extent) || extent < 0.0f) return false;
+    int e = -100;
+    if (extent > 0.0f) {
+        int ex = 0;
+        std::frexp(extent / 255.0f, &ex);  // extent/255 < 2^ex
+        e = std::max(-100, ex);
+    }
+    for (; e <= 127; e++) {
+        const float s = pow2(e);
+        bool ok = true;
+        for (int j = 0; j < k && ok; j++) {
+            if (!std::isfinite(lo[j]) || !std::isfinite(hi[j])) return false;
+            long ql = (long)std::floor((double)(lo[j] - origin) / s);
+            long qh = (long)std::ceil((double)(hi[j] - origin) / s);
+            ql = std::max(0L, std::min(255L, ql));
+            qh = std::max(0L, std::min(255L, qh));
+            while (ql > 0 && std::fma((float)ql, s, origin) > lo[j]) ql--;
+            while (qh < 255 && std::fma((float)qh, s, origin) < hi[j]) qh++;
+            if (std::fma((float)ql, s, origin) > lo[j] || std::fma((float)qh, s, origin) < hi[j]) ok = false;
+            qlo[j] = (uint8_t)ql;
+            qhi[j] = (uint8_t)qh;
+        }
+        if (ok) {
+            *exp_out = (int8_t)e;
+            return true;
+        }
+    }
+    return false;
+}
+
+}  // namespace
+
+bool quantize_nodes(WideScene& w) {
+    w.qnodes.assign(w.nodes.size(), QNode4{});
+    for (size_t i = 0; i < w.nodes.size(); i++) {
+        const ImbaNode4& n = w.nodes[i];
+        QNode4& q = w.qnodes[i];
+        std::memset(&q, 0, sizeof(q));
+        int k = 0;
+        while (k < 4 && n.children[k] != 0) k++;
+        q.count = (uint8_t)k;
+        for (int j = 0; j < 4; j++) q.children[j] = n.children[j];
+        const float* lo[3] = {n.min_x, n.min_y, n.min_z};
+        const float* hi[3] = {n.max_x, n.max_y, n.max_z};
+        uint8_t* qlo[3] = {q.lo_x, q.lo_y, q.lo_z};
+        uint8_t* qhi[3] = {q.hi_x, q.hi_y, q.hi_z};
+        for (int a = 0; a < 3; a++) {
+            float mn = lo[a][0], mx = hi[a][0];
+            for (int j = 1; j < k; j++) {
+                mn = std::min(mn, lo[a][j]);
+                mx = std::max(mx, hi[a][j]);
+            }
+            q.origin[a] = mn;
+            if (!quantize_axis(lo[a], hi[a], k, mn, mx - mn, &q.exp[a], qlo[a], qhi[a])) {
+                w.qnodes.clear();
+                return false;
+            }
+        }
+    }
+    return true;
 }
 
 }  // namespace imba
@@ -176,6 +248,13 @@ int imba_wide_get(const ImbaWideBvh* h, const ImbaNode4** nodes, int* node_count
     *instance_count = (int)h->w.instances.size();
     *root = h->w.root;
     *max_stack = h->w.max_stack;
+    return IMBA_OK;
+}
+
+int imba_wide_get_quantized(const ImbaWideBvh* h, const void** nodes, int* node_count) {
+    IMBA_CHECK_ARG(h && nodes && node_count, "NULL argument");
+    *nodes = h->w.qnodes.data();
+    *node_count = (int)h->w.qnodes.size();
     return IMBA_OK;
 }
 

@@ -84,7 +84,7 @@ class MeshDesc(C.Structure):
 
 class TraceConfig(C.Structure):
     _fields_ = [("block_size", C.c_int), ("stack_lds", C.c_int), ("persistent", C.c_int), ("blocks_per_cu", C.c_int),
-                ("bvh_width", C.c_int)]
+                ("bvh_width", C.c_int), ("quantized", C.c_int)]
 
 
 class TraceItem(C.Structure):
@@ -121,6 +121,7 @@ _sig("imba_host_free", i32, vp)
 _sig("imba_memcpy_h2d", i32, vp, vp, sz, vp)
 _sig("imba_memcpy_d2h", i32, vp, vp, sz, vp)
 _sig("imba_memcpy_d2d", i32, vp, vp, sz, vp)
+_sig("imba_memset", i32, vp, i32, sz, vp)
 _sig("imba_stream_create", i32, C.POINTER(vp))
 _sig("imba_stream_destroy", i32, vp)
 _sig("imba_stream_sync", i32, vp)
@@ -161,9 +162,15 @@ _sig("imba_trace_error_count", i32, vp, ip)
 _sig("imba_trace_stats", i32, vp, vp, vp, vp, i32, i32, C.POINTER(TraceConfig), vp)
 _sig("imba_wide_create", i32, C.POINTER(SceneDesc), C.POINTER(vp))
 _sig("imba_wide_get", i32, vp, C.POINTER(vp), ip, C.POINTER(vp), ip, ip, ip)
+_sig("imba_wide_get_quantized", i32, vp, C.POINTER(vp), ip)
 _sig("imba_wide_release", i32, vp)
+QNODE4_DTYPE = np.dtype([("origin", "<f4", (3,)), ("exp", "i1", (3,)), ("count", "u1"), ("children", "<i4", (4,)),
+                         ("lo_x", "u1", (4,)), ("hi_x", "u1", (4,)), ("lo_y", "u1", (4,)), ("hi_y", "u1", (4,)),
+                         ("lo_z", "u1", (4,)), ("hi_z", "u1", (4,)), ("pad", "<u4", (2,))])
+assert QNODE4_DTYPE.itemsize == 64
 _sig("imba_trace_batch_create", i32, C.POINTER(TraceItem), i32, C.POINTER(vp))
 _sig("imba_trace_batch_launch", i32, vp, C.POINTER(TraceConfig), vp)
+_sig("imba_trace_batch_update", i32, vp, C.POINTER(TraceItem), i32, vp)
 _sig("imba_trace_batch_release", i32, vp)
 _sig("imba_autotune", i32, vp, vp, vp, i32, i32, i32, C.POINTER(TraceConfig), C.POINTER(f32), vp)
 _sig("imba_graph_begin", i32, vp)

@@ -217,7 +217,11 @@ def wide_tree(arrays: dict) -> dict:
               "imba_wide_get")
         nodes = np.frombuffer((C.c_char * (nn.value * 128)).from_address(pn.value), N.NODE4_DTYPE).copy()
         inst = np.frombuffer((C.c_char * (ni.value * 64)).from_address(pi.value), INSTANCE_DTYPE).copy()
-        return {"nodes": nodes, "instances": inst, "root": root.value, "max_stack": ms.value}
+        pq, nq = C.c_void_p(), C.c_int()
+        check(N.lib.imba_wide_get_quantized(h, C.byref(pq), C.byref(nq)), "imba_wide_get_quantized")
+        qnodes = np.frombuffer((C.c_char * (nq.value * 64)).from_address(pq.value), N.QNODE4_DTYPE).copy() \
+            if nq.value else np.zeros(0, N.QNODE4_DTYPE)
+        return {"nodes": nodes, "qnodes": qnodes, "instances": inst, "root": root.value, "max_stack": ms.value}
     finally:
         N.lib.imba_wide_release(h)
 
@@ -314,6 +318,11 @@ class TraceBatch:
     """Many queues (scene, rays, hits, count[, device count], any_hit) traversed together."""
 
     def __init__(self, items):
+        arr = self._items(items)
+        self.h = C.c_void_p()
+        check(N.lib.imba_trace_batch_create(arr, len(items), C.byref(self.h)), "imba_trace_batch_create")
+
+    def _items(self, items):
         arr = (N.TraceItem * max(1, len(items)))()
         self._keep = []
         for k, it in enumerate(items):
@@ -321,10 +330,16 @@ class TraceBatch:
             count_dev = it[4] if len(it) > 4 else None
             any_hit = it[5] if len(it) > 5 else False
             cdev = count_dev.ptr if isinstance(count_dev, DeviceBuffer) else count_dev
-            arr[k] = N.TraceItem(scene.h, rays.ptr, hits.ptr, count, cdev, int(any_hit))
+            rp = rays.ptr if isinstance(rays, DeviceBuffer) else rays
+            hp = hits.ptr if isinstance(hits, DeviceBuffer) else hits
+            arr[k] = N.TraceItem(scene.h, rp, hp, count, cdev, int(any_hit))
             self._keep.append(scene)
-        self.h = C.c_void_p()
-        check(N.lib.imba_trace_batch_create(arr, len(items), C.byref(self.h)), "imba_trace_batch_create")
+        return arr
+
+    def update(self, items, stream=None):
+        """Replaces the queues; the new table is copied on `stream` (launch on the same stream)."""
+        arr = self._items(items)
+        check(N.lib.imba_trace_batch_update(self.h, arr, len(items), N._s(stream)), "imba_trace_batch_update")
 
     def launch(self, cfg: N.TraceConfig | None = None, stream=None):
         check(N.lib.imba_trace_batch_launch(self.h, C.byref(cfg) if cfg else None, N._s(stream)),

@@ -1,0 +1,301 @@
+"""A request loop for path-traced frames on one GPU, with continuous cross-session batching.
+
+Requests (scene, camera, resolution, spp, seed) arrive over time.  A request is admitted when
+its buffers fit in a fixed device arena (admission by HBM footprint, first come first served)
+and a counter slot is free; its queues then live in the arena until it completes.  Every step
+of the loop:
+  * admitted requests get their camera rays (the reference's per-pixel seeding);
+  * the current queue of every active request, whatever its scene and bounce, goes into one
+    batched traversal launch (imba_trace_batch_update + imba_trace_batch_launch), so requests
+    that arrived at different times share launches;
+  * each request then runs its Lambertian bounce and compaction with its queue size kept on
+    the device, and one read-back of all the requests' queue sizes ends the step.
+A request completes when its queue is empty or its paths reach the maximum length; its latency
+is completion minus arrival.  The plain per-request path (`batched=False`) runs the same
+kernels for one request at a time, in arrival order.
+
+A request's image does not depend on the other requests it shares launches with: its rays,
+seeds and per-ray results are its own (the image is a sum of float atomics, so equal up to the
+order of additions).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import native as N
+from . import scene as S
+
+STATE_BYTES = 32
+ALIGN = 256
+
+
+class DeviceArena:
+    """First-fit allocator of byte ranges inside one device allocation."""
+
+    def __init__(self, nbytes: int, buf=None):
+        self.buf = buf if buf is not None else N.DeviceBuffer(nbytes)
+        self.size = nbytes
+        self.free = [(0, nbytes)]  # sorted (offset, length)
+
+    def alloc(self, nbytes: int):
+        nbytes = (nbytes + ALIGN - 1) // ALIGN * ALIGN
+        for k, (off, ln) in enumerate(self.free):
+            if ln >= nbytes:
+                if ln == nbytes:
+                    self.free.pop(k)
+                else:
+                    self.free[k] = (off + nbytes, ln - nbytes)
+                return off, nbytes
+        return None
+
+    def release(self, off: int, nbytes: int):
+        self.free.append((off, nbytes))
+        self.free.sort()
+        merged = []
+        for o, ln in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == o:
+                merged[-1] = (merged[-1][0], merged[-1][1] + ln)
+            else:
+                merged.append((o, ln))
+        self.free = merged
+
+    def ptr(self, off: int):
+        return C.c_void_p(self.buf.ptr.value + off)
+
+
+class ServedScene:
+    """A resident scene with the shading data of the PT bounce (per-triangle normals and
+    materials, per-material albedo and emission)."""
+
+    def __init__(self, scene, cfg: N.TraceConfig | None = None):
+        self.s = scene
+        self.gpu = S.Scene(scene.desc)
+        self.tri_n = N.DeviceBuffer.from_array(scene.tri_normals)
+        self.tri_m = N.DeviceBuffer.from_array(scene.tri_materials)
+        self.albedo = N.DeviceBuffer.from_array(scene.albedo)
+        self.emission = N.DeviceBuffer.from_array(scene.emission)
+        self.cfg = cfg
+
+
+@dataclass
+class Request:
+    rid: int
+    session: int
+    scene: int
+    width: int
+    height: int
+    spp: int
+    cam_pos: tuple
+    cam_dir: tuple
+    seed: int
+    arrival: float  # seconds after the start of the run
+    # filled in by the server
+    admitted: float = -1.0
+    done: float = -1.0
+    rays: int = 0
+    counts: list = field(default_factory=list)
+
+    @property
+    def samples(self) -> int:
+        return self.width * self.height * self.spp
+
+
+def footprint(req: Request) -> int:
+    """Device bytes a request holds while active: two ray and state buffers, hits, image and
+    the compaction workspace."""
+    n = req.samples
+    ws = int(N.lib.imba_compact_workspace_size(n))
+    return 2 * 32 * n + 2 * STATE_BYTES * n + 16 * n + 12 * req.width * req.height + ws + 8 * ALIGN
+
+
+class _Job:
+    def __init__(self, req: Request, slot: int, block, scene: ServedScene, arena: DeviceArena):
+        self.req, self.slot, self.block, self.sc = req, slot, block, scene
+        n = req.samples
+        self.n = n
+        off = block[0]
+
+        def take(nbytes):
+            nonlocal off
+            p = arena.ptr(off)
+            off += (nbytes + ALIGN - 1) // ALIGN * ALIGN
+            return p
+
+        self.rays = [take(32 * n), take(32 * n)]
+        self.states = [take(STATE_BYTES * n), take(STATE_BYTES * n)]
+        self.hits = take(16 * n)
+        self.image = take(12 * req.width * req.height)
+        self.workspace = take(int(N.lib.imba_compact_workspace_size(n)))
+        self.bounce = 0
+        self.cur = 0
+        self.r2w = np.zeros(16, np.float32)
+        self.pos = np.asarray(req.cam_pos, np.float32)
+        d = np.asarray(req.cam_dir, np.float32)
+        up = np.asarray((0.0, 1.0, 0.0), np.float32)
+        N.check(N.lib.imba_gen_camera_matrix(N.ptr(self.pos), N.ptr(d), N.ptr(up), 60.0, req.width, req.height,
+                                             N.ptr(self.r2w)), "camera matrix")
+
+
+class Server:
+    def __init__(self, scenes: list, arena_bytes: int, max_active: int = 32, batched: bool = True,
+                 max_path_len: int = 8, rr_start: int = 3, cfg: N.TraceConfig | None = None):
+        self.scenes = scenes
+        self.arena = DeviceArena(arena_bytes)
+        self.max_active = max_active if batched else 1
+        self.batched = batched
+        self.max_path_len, self.rr_start = max_path_len, rr_start
+        self.cfg = cfg
+        self.stream = N.Stream()
+        self.counts = N.DeviceBuffer(4 * (max_path_len + 1) * self.max_active)
+        self.free_slots = list(range(self.max_active))
+        self.pending: list[Request] = []
+        self.active: list[_Job] = []
+        self.done: list[Request] = []
+        self.batch = S.TraceBatch([])
+        self.steps = 0
+        self.t0 = None
+
+    def counts_ptr(self, slot: int, bounce: int):
+        return self.counts.at(4 * ((self.max_path_len + 1) * slot + bounce))
+
+    def now(self) -> float:
+        return time.perf_counter() - self.t0
+
+    def submit(self, req: Request):
+        if footprint(req) > self.arena.size:
+            raise ValueError("request %d needs %d bytes, more than the arena" % (req.rid, footprint(req)))
+        self.pending.append(req)
+
+    def _admit(self):
+        st = self.stream.handle
+        while self.pending and self.free_slots:
+            req = self.pending[0]
+            block = self.arena.alloc(footprint(req))
+            if block is None:
+                break  # first come first served: wait for memory
+            self.pending.pop(0)
+            job = _Job(req, self.free_slots.pop(0), block, self.scenes[req.scene], self.arena)
+            req.admitted = self.now()
+            n = job.n
+            c0 = np.zeros(self.max_path_len + 1, np.int32)
+            c0[0] = n
+            job.c0 = c0  # kept until the copy has run (the step ends with a stream synchronisation)
+            N.check(N.lib.imba_memcpy_h2d(self.counts_ptr(job.slot, 0), N.ptr(c0), c0.nbytes, st), "counts")
+            N.check(N.lib.imba_memset(job.image, 0, 12 * req.width * req.height, st), "image")
+            N.check(N.lib.imba_pt_camera(N.ptr(job.r2w), N.ptr(job.pos), req.width, req.height, req.spp, req.seed, 0,
+                                         n, job.rays[0], job.states[0], st), "camera")
+            self.active.append(job)
+
+    def step(self):
+        """One step: admit, one batched traversal over every active queue, bounces and
+        compactions, one read-back of the queue sizes."""
+        self._admit()
+        if not self.active:
+            return
+        st = self.stream.handle
+        items = [(j.sc.gpu, j.rays[j.cur], j.hits, j.n, self.counts_ptr(j.slot, j.bounce), False) for j in self.active]
+        if self.batched:
+            self.batch.update(items, self.stream)
+            self.batch.launch(self.cfg, self.stream)
+        else:
+            for j in self.active:
+                N.check(N.lib.imba_intersect_dev(j.sc.gpu.h, j.rays[j.cur], j.hits, j.n, self.counts_ptr(j.slot, j.bounce),
+                                                 C.byref(self.cfg) if self.cfg else None, st), "intersect_dev")
+        for j in self.active:
+            sc = j.sc
+            cnt = self.counts_ptr(j.slot, j.bounce)
+            N.check(N.lib.imba_pt_bounce_dev(j.rays[j.cur], j.hits, j.states[j.cur], j.n, cnt, sc.tri_n.ptr, sc.tri_m.ptr,
+                                             sc.s.tri_count, sc.albedo.ptr, sc.emission.ptr, len(sc.s.albedo),
+                                             self.max_path_len, self.rr_start, sc.s.offset, j.image, j.rays[j.cur], st),
+                    "bounce_dev")
+            N.check(N.lib.imba_compact_rays_dev(j.rays[j.cur], j.states[j.cur], STATE_BYTES, j.n, cnt,
+                                                j.rays[1 - j.cur], j.states[1 - j.cur],
+                                                self.counts_ptr(j.slot, j.bounce + 1), j.workspace, st), "compact_dev")
+            j.bounce += 1
+            j.cur = 1 - j.cur
+        allc = self.counts.download(np.int32, (self.max_path_len + 1) * self.max_active, self.stream)
+        self.stream.sync()
+        allc = allc.reshape(self.max_active, self.max_path_len + 1)
+        self.steps += 1
+        still = []
+        for j in self.active:
+            c = allc[j.slot]
+            if c[j.bounce] == 0 or j.bounce >= self.max_path_len:
+                req = j.req
+                req.done = self.now()
+                req.counts = [int(x) for x in c[:j.bounce]]
+                req.rays = int(sum(req.counts))
+                self.done.append(req)
+                self.arena.release(*j.block)
+                self.free_slots.append(j.slot)
+            else:
+                still.append(j)
+        self.active = still
+
+    def run(self, requests: list, keep_images: bool = False, progress=None):
+        """Runs the requests with their arrival times (seconds after start); returns the
+        completed requests (and {rid: image} with keep_images)."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        images = {}
+        self.t0 = time.perf_counter()
+        k = 0
+        last_report = 0.0
+        while k < len(reqs) or self.pending or self.active:
+            now = self.now()
+            while k < len(reqs) and reqs[k].arrival <= now:
+                self.submit(reqs[k])
+                k += 1
+            if not self.pending and not self.active:
+                if k < len(reqs):
+                    time.sleep(max(0.0, min(0.002, reqs[k].arrival - now)))
+                continue
+            before = {j.req.rid: j for j in self.active}
+            self.step()
+            if keep_images:
+                for rid, j in before.items():
+                    if j.req.done >= 0 and rid not in images:
+                        images[rid] = self._image_now(j)
+            if progress and self.now() - last_report > 5.0:
+                last_report = self.now()
+                progress("%d/%d requests done, %d active, %d pending" % (len(self.done), len(reqs), len(self.active),
+                                                                        len(self.pending)))
+        return (self.done, images) if keep_images else self.done
+
+    def _image_now(self, job: _Job) -> np.ndarray:
+        out = np.empty(job.req.width * job.req.height * 3, np.float32)
+        N.check(N.lib.imba_memcpy_d2h(N.ptr(out), job.image, out.nbytes, self.stream.handle), "image")
+        self.stream.sync()
+        return out
+
+
+def poisson_requests(n: int, rate: float, n_sessions: int, n_scenes: int, scene_centres: list, seed: int = 7,
+                     resolutions=((256, 256), (640, 480), (1280, 720), (1920, 1080)), spp=(1, 4)) -> list:
+    """n requests with Poisson arrivals at `rate` per second from n_sessions sessions, each for a
+    uniformly drawn scene, resolution and spp and a random camera looking at the scene centre."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for rid in range(n):
+        t += rng.exponential(1.0 / rate)
+        sc = int(rng.integers(0, n_scenes))
+        w, h = resolutions[int(rng.integers(0, len(resolutions)))]
+        cx, cy, cz, radius = scene_centres[sc]
+        ang = rng.uniform(0, 2 * np.pi)
+        pos = (cx + radius * np.sin(ang), cy + rng.uniform(0.3, 0.9) * radius, cz - radius * np.cos(ang))
+        d = (cx - pos[0], cy * 0.3 - pos[1], cz - pos[2])
+        out.append(Request(rid, int(rng.integers(0, n_sessions)), sc, w, h, int(rng.integers(spp[0], spp[1] + 1)),
+                           pos, d, int(rng.integers(1, 2 ** 31 - 1)), t))
+    return out
+
+
+def latency_stats(done: list) -> dict:
+    lat = np.array([(r.done - r.arrival) * 1e3 for r in done])
+    rays = sum(r.rays for r in done)
+    span = max(r.done for r in done) - min(r.arrival for r in done)
+    return {"requests": len(done), "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "rays": int(rays), "mrays_s": rays / span / 1e6,
+            "requests_per_s": len(done) / span, "span_s": span}

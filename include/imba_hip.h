@@ -99,6 +99,8 @@ int imba_host_free(void* ptr);
 int imba_memcpy_h2d(void* dst_device, const void* src_host, size_t bytes, void* stream);
 int imba_memcpy_d2h(void* dst_host, const void* src_device, size_t bytes, void* stream);
 int imba_memcpy_d2d(void* dst_device, const void* src_device, size_t bytes, void* stream);
+/* Byte fill of device memory (hipMemsetAsync: not for code captured into a graph). */
+int imba_memset(void* dst_device, int value, size_t bytes, void* stream);
 int imba_stream_create(void** stream);
 int imba_stream_destroy(void* stream);
 int imba_stream_sync(void* stream);
@@ -240,6 +242,11 @@ typedef struct ImbaWideBvh ImbaWideBvh;
 int imba_wide_create(const ImbaSceneDesc* host, ImbaWideBvh** out);
 int imba_wide_get(const ImbaWideBvh* w, const ImbaNode4** nodes, int* node_count,
                   const ImbaInstanceNode** instances, int* instance_count, int* root, int* max_stack);
+/* The same tree with quantized 64-byte nodes (node_count 0 if a box is not finite): origin[3]
+ * (float), exponent[3] (int8) and child count (uint8), children[4] (int), then uint8
+ * lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4] and 8 bytes of padding; child box j on
+ * an axis is [fma(lo[j], 2^e, origin), fma(hi[j], 2^e, origin)], which contains the exact box. */
+int imba_wide_get_quantized(const ImbaWideBvh* w, const void** nodes, int* node_count);
 int imba_wide_release(ImbaWideBvh* w);
 
 /* ------------------------------ device-resident scenes -----------------------------------
@@ -273,6 +280,8 @@ typedef struct ImbaTraceConfig {
     int blocks_per_cu;     /* persistent grid size per CU (0 = default)                    */
     int bvh_width;         /* 2: the scene's BVH2 as given; 4: the 4-wide tree derived from
                               it at upload (same leaves); 0 = default                      */
+    int quantized;         /* with bvh_width 4: 64-byte nodes whose child boxes are 8-bit
+                              offsets from the node box, rounded outward                  */
 } ImbaTraceConfig;
 
 int imba_intersect(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int count,
@@ -321,6 +330,10 @@ typedef struct ImbaTraceItem {
 } ImbaTraceItem;
 typedef struct ImbaTraceBatch ImbaTraceBatch;
 int imba_trace_batch_create(const ImbaTraceItem* items, int n_items, ImbaTraceBatch** out);
+/* Replaces the batch's queues (same rules as create).  The new table is copied on `stream`
+ * (pinned staging, no device-wide synchronisation unless the batch must grow): launch on the
+ * same stream after it.  For request loops whose set of queues changes every step. */
+int imba_trace_batch_update(ImbaTraceBatch* b, const ImbaTraceItem* items, int n_items, void* stream);
 /* cfg: block_size / stack_lds / blocks_per_cu as for imba_intersect; every launch is persistent. */
 int imba_trace_batch_launch(ImbaTraceBatch* b, const ImbaTraceConfig* cfg, void* stream);
 int imba_trace_batch_release(ImbaTraceBatch* b);

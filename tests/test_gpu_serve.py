@@ -1,0 +1,61 @@
+"""The request loop (imbatracer_amd.serve): a request's result is the same alone and in a mix
+of concurrent requests sharing batched launches (GPU)."""
+import numpy as np
+import pytest
+
+from imbatracer_amd import serve as V
+from imbatracer_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def scenes():
+    out = []
+    for side, level in ((2, 2), (3, 3)):
+        out.append(V.ServedScene(W.SphereGridScene(side=side, level=level, threads=4)))
+    return out
+
+
+def _requests():
+    res = [(64, 48), (32, 32), (80, 60), (48, 48), (64, 64), (40, 30)]
+    reqs = []
+    for k, (w, h) in enumerate(res):
+        sc = k % 2
+        side = 2 + sc
+        c = (side * 0.5, 0.0, side * 0.5)
+        pos = (c[0] + 0.3 * k, 3.0 + 0.2 * k, c[2] - 6.0)
+        reqs.append(V.Request(k, k % 3, sc, w, h, 1 + k % 2, pos, (c[0] - pos[0], -pos[1], c[2] - pos[2]), 1000 + 17 * k,
+                              0.0))
+    return reqs
+
+
+def test_alone_and_in_a_mix_give_the_same_frames(gpu, scenes):
+    mix = V.Server(scenes, 256 << 20, max_active=8, batched=True)
+    done_mix, img_mix = mix.run(_requests(), keep_images=True)
+    alone = V.Server(scenes, 256 << 20, batched=False)
+    done_alone, img_alone = alone.run(_requests(), keep_images=True)
+    assert len(done_mix) == len(done_alone) == 6
+    by_mix = {r.rid: r for r in done_mix}
+    for r in done_alone:
+        m = by_mix[r.rid]
+        assert m.counts == r.counts and m.rays == r.rays and r.counts[0] == r.samples
+        a, b = img_alone[r.rid], img_mix[r.rid]
+        assert np.allclose(a, b, rtol=1e-5, atol=1e-6)
+        lit = np.abs(a) > 1e-6
+        if lit.any():
+            assert np.mean(np.abs(a[lit] - b[lit]) / np.abs(a[lit])) <= 1e-4
+    # the requests really shared steps: fewer steps than one request at a time
+    assert mix.steps < alone.steps
+    assert mix.arena.free == [(0, mix.arena.size)] and alone.arena.free == [(0, alone.arena.size)]
+
+
+def test_admission_waits_for_memory(gpu, scenes):
+    reqs = _requests()
+    small = V.Server(scenes, max(V.footprint(r) for r in reqs) + 1024, max_active=8, batched=True)
+    done = small.run(reqs)
+    assert len(done) == 6
+    # with room for about one request at a time, admissions never overlapped in memory
+    spans = sorted((r.admitted, r.done) for r in done)
+    for (a0, d0), (a1, d1) in zip(spans, spans[1:]):
+        assert a1 >= d0 - 1e-9 or V.footprint(reqs[0]) * 2 <= small.arena.size

@@ -18,6 +18,8 @@ from tests import scenes as T
 pytestmark = pytest.mark.gpu
 
 WIDE = N.TraceConfig(0, 0, 0, 0, 4)
+TREES = [N.TraceConfig(0, 0, 0, 0, 4, 0), N.TraceConfig(0, 0, 0, 0, 4, 1)]   # 4-wide, 4-wide quantized
+TREE_IDS = ["wide", "quantized"]
 
 
 def _same(a, b):
@@ -31,8 +33,10 @@ def _check_occlusion(got, want, rays):
     assert np.all(t >= rays["org"][hit, 3]) and np.all(t <= rays["dir"][hit, 3])
 
 
+@pytest.mark.parametrize("tree", TREES, ids=TREE_IDS)
 @pytest.mark.parametrize("builder", [N.BUILDER_BINNED_SAH, N.BUILDER_SBVH])
-def test_closest_and_occluded(gpu, builder):
+def test_closest_and_occluded(gpu, builder, tree):
+    WIDE = tree
     b, _, _ = T.random_tri_scene(65536, builder)
     a = b.arrays()
     sc = S.Scene(b.desc())
@@ -48,18 +52,21 @@ def test_closest_and_occluded(gpu, builder):
     assert sc.error_count() == 0
 
 
+@pytest.mark.parametrize("quantized", [0, 1])
 @pytest.mark.parametrize("shape", [(256, 16, 0), (256, 8, 0), (128, 32, 0), (64, 64, 0), (256, 16, 32), (128, 32, 8)])
-def test_launch_shapes(gpu, shape):
+def test_launch_shapes(gpu, shape, quantized):
     b, _, _ = T.random_tri_scene(20000, N.BUILDER_SBVH)
     sc = S.Scene(b.desc())
     rays = T.random_rays(30000, seed=8)
     want, _ = oracle.trace(b.arrays(), rays, threads=8)
-    cfg = N.TraceConfig(shape[0], shape[1], shape[2], 0, 4)
+    cfg = N.TraceConfig(shape[0], shape[1], shape[2], 0, 4, quantized)
     for _ in range(2):
         assert _same(sc.trace_host(rays, cfg=cfg), want)
 
 
-def test_instanced_and_masked(gpu):
+@pytest.mark.parametrize("tree", TREES, ids=TREE_IDS)
+def test_instanced_and_masked(gpu, tree):
+    WIDE = tree
     b = T.instanced_scene()
     sc = S.Scene(b.desc())
     rays = T.random_rays(40000, seed=9, box=(-2.0, 2.0))
@@ -76,7 +83,9 @@ def test_instanced_and_masked(gpu):
     _check_occlusion(msc.trace_host(prim, any_hit=True, cfg=WIDE), want_o, prim)
 
 
-def test_single_triangle(gpu):
+@pytest.mark.parametrize("tree", TREES, ids=TREE_IDS)
+def test_single_triangle(gpu, tree):
+    WIDE = tree
     v = np.array([[-1, -1, 1, 1], [1, -1, 1, 1], [0, 1, 1, 1]], np.float32)
     i = np.array([[0, 1, 2, 0]], np.uint32)
     b = S.SceneBuild()
@@ -110,8 +119,15 @@ def test_visit_counts_and_batch(gpu):
     # the BVH2 counts are the oracle's
     assert np.array_equal(c2.download(np.uint32, 3 * len(rays)).reshape(-1, 3),
                           oracle.trace(b.arrays(), rays, counts=True)[1])
-    # the batched kernel on the 4-wide tree
+    # the batched kernel on the 4-wide trees
     out = N.DeviceBuffer(len(rays) * 16)
     batch = S.TraceBatch([(sc, rb, out, len(rays), None, False)])
-    batch.launch(WIDE)
-    assert _same(out.download(N.HIT_DTYPE, len(rays)), h2)
+    for tree in TREES:
+        batch.launch(tree)
+        assert _same(out.download(N.HIT_DTYPE, len(rays)), h2)
+    # the quantized tree: same hits, about as many node visits as the exact 4-wide tree
+    cq = N.DeviceBuffer(len(rays) * 12)
+    sc.trace_stats(rb, hb, cq, len(rays), cfg=TREES[1])
+    assert _same(hb.download(N.HIT_DTYPE, len(rays)), h2)
+    nq = cq.download(np.uint32, 3 * len(rays)).reshape(-1, 3).sum(0)
+    assert 0.9 * n4[0] < nq[0] < 1.5 * n4[0]

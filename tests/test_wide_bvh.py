@@ -129,3 +129,33 @@ def test_largest_child_is_opened_first():
         assert n4["min_x"][j] == bb["lo_x"] and n4["max_z"][j] == bb["hi_z"]
         if c < 0:
             assert n4["children"][j] == c
+
+
+def _dequant(q, axis, j):
+    """The kernel's child box on one axis, fma(q, 2^e, origin) rounded once to float32 (the
+    float64 sum of these operands is exact)."""
+    e = float(2.0 ** int(q["exp"][axis]))
+    o = np.float64(q["origin"][axis])
+    lo = np.float32(q["lo_" + "xyz"[axis]][j] * e + o)
+    hi = np.float32(q["hi_" + "xyz"[axis]][j] * e + o)
+    return lo, hi
+
+
+@pytest.mark.parametrize("name,build", list(_scenes()), ids=lambda x: x if isinstance(x, str) else "")
+def test_quantized_boxes_contain_the_exact_boxes(name, build):
+    w = S.wide_tree(build.arrays())
+    nodes, qn = w["nodes"], w["qnodes"]
+    assert len(qn) == len(nodes)
+    for n, q in zip(nodes, qn):
+        assert list(q["children"]) == list(n["children"])
+        k = int((n["children"] != 0).sum())
+        assert q["count"] == k
+        for j in range(k):
+            for axis, (mn, mx) in enumerate((("min_x", "max_x"), ("min_y", "max_y"), ("min_z", "max_z"))):
+                lo, hi = _dequant(q, axis, j)
+                assert lo <= n[mn][j] and hi >= n[mx][j]
+                # at most about one quantization step larger on each side
+                parent = float(max(n[mx][:k]) - min(n[mn][:k]))
+                step = 2.0 ** int(q["exp"][axis])
+                assert n[mn][j] - lo <= step * 1.01 + 1e-6 and hi - n[mx][j] <= step * 1.01 + 1e-6
+                assert step <= max(parent / 255.0 * 2.0, 2.0 ** -100)
