@@ -12,6 +12,8 @@ timed region).
 
 --workload c1 runs the CPU-runnable configuration instead: 65,536 random triangles
 (binned SAH), 1,048,576 primary closest-hit rays plus 1,048,576 occlusion rays per step.
+--workload c3: photon mapping (the PPM mode of VCM) at 1280x720 with 921,600 light paths: light
+and camera traversal, photon compaction, hash-grid build and k-nearest queries per frame.
 --workload c4: 256 queues over 8 scenes, about 35M closest-hit rays per step in one batched
 launch, against the same queues launched one at a time.
 --workload c5: serving; Poisson request arrivals over 8 resident scenes, the request loop with
@@ -484,16 +486,170 @@ def run_c5(args):
     return res
 
 
+def grid_candidates(info, qpos):
+    """Photons visited per query by HashGrid::query (rangesearch.h:77-129): the sizes of the 8
+    probed buckets (a bucket probed twice counts twice); 0 for queries outside the bounds."""
+    ends = np.empty(info.table_size, np.int32)
+    from imbatracer_amd import native as N
+
+    N.check(N.lib.imba_memcpy_d2h(N.ptr(ends), C.c_void_p(info.cell_ends), ends.nbytes, None), "d2h")
+    size = np.diff(np.concatenate([[0], ends])).astype(np.int64)
+    bmin = np.array(info.bbox_min, np.float32)
+    bmax = np.array(info.bbox_max, np.float32)
+    q = np.ascontiguousarray(qpos, np.float32)
+    inside = np.all((q >= bmin) & (q <= bmax), axis=1)
+    q = q[inside]
+    c = np.float32(info.inv_cell_size) * (q - bmin)
+    f = np.floor(c)
+    p = f.astype(np.int64)
+    o = p + np.where((c - f) < np.float32(0.5), -1, 1)
+    total = np.zeros(len(q), np.int64)
+    for j in range(8):
+        x = o[:, 0] if j & 4 else p[:, 0]
+        y = o[:, 1] if j & 2 else p[:, 1]
+        z = o[:, 2] if j & 1 else p[:, 2]
+        h = ((x.astype(np.uint32) * np.uint32(73856093)) ^ (y.astype(np.uint32) * np.uint32(19349663))
+             ^ (z.astype(np.uint32) * np.uint32(83492791))) % np.uint32(info.table_size)
+        total += size[h.astype(np.int64)]
+    out = np.zeros(len(qpos), np.int64)
+    out[inside] = total
+    return out
+
+
+def run_c3(args):
+    """Photon mapping (the PPM mode of the VCM integrator) at 1280x720 over the sphere-grid scene
+    with 30% mirror and glass spheres: 921,600 light paths of up to 6 hits deposit photons, a
+    hash grid is built over them (r = 0.005 x the scene diagonal), one merge query per pixel
+    finds its 10 nearest photons.  One step = one frame (light pass, photon compaction, grid
+    build, camera pass, queries, merging); value = closest-hit rays traversed per second of
+    frame time."""
+    import oracle
+    from imbatracer_amd import native as N
+    from imbatracer_amd import workloads as W
+
+    t0 = time.time()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    scene = W.SphereGridScene(side=args.side, level=5, builder=N.BUILDER_SBVH, threads=threads,
+                              specular_fraction=0.3)
+    log("scene: %d triangles, built in %.1f s" % (scene.tri_count, time.time() - t0))
+    cfg = fixed_config(args)
+    pm = W.PhotonMapper(scene, args.width, args.height, light_paths=args.c3_paths, max_depth=6, rr_start=3,
+                        radius_factor=0.005, k=10, trace_cfg=cfg)
+    if cfg is None and not args.no_autotune:
+        lr, ls = N.DeviceBuffer(pm.P * 32), N.DeviceBuffer(pm.P * 32)
+        N.check(N.lib.imba_ppm_light(N.ptr(pm.light_tri), N.ptr(pm.light_int), pm.light_seed, 0, pm.P, lr.ptr,
+                                     ls.ptr, None), "ppm_light")
+        hb = N.DeviceBuffer(pm.P * 16)
+        cfg, ms = pm.gpu_scene.autotune(lr, hb, pm.P, reps=3)
+        log("autotune (light rays, %d): %s, %.3f ms" % (pm.P, cfg_dict(cfg), ms))
+        pm.cfg = cfg
+        del lr, ls, hb
+    light, cam, n_ph = pm.frame()
+    log("light queues %s, camera queues %s, photons %d" % (light, cam, n_ph))
+    for _ in range(args.warmup):
+        pm.frame()
+    phases, frame_ms, rays_total = [], [], 0
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        f0 = time.perf_counter()
+        l, c, n = pm.frame(timed=True)
+        frame_ms.append((time.perf_counter() - f0) * 1e3)
+        if (l, c, n) != (light, cam, n_ph):
+            raise RuntimeError("frame queue sizes changed between frames")
+        rays_total += sum(l) + sum(c)
+        phases.append(pm.phase_ms())
+    wall = time.perf_counter() - t_start
+    if pm.gpu_scene.error_count():
+        raise RuntimeError("traversal reported a malformed scene")
+    phase = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+    log("phases (ms): %s" % {k: round(v, 3) for k, v in phase.items()})
+
+    # parity sample: the grid over the GPU's photons and a sample of the queries, against the oracle
+    ph_rays, _ = pm.photons_host()
+    pos = np.ascontiguousarray(ph_rays["pos"])
+    q = pm.queries_host()
+    qpos = np.ascontiguousarray(q["pos"])
+    idx, dist, cnt = pm.results_host()
+    og = oracle.Grid()
+    tb = time.perf_counter()
+    og.build(pos, pm.radius)
+    cpu_build_s = time.perf_counter() - tb
+    valid = np.nonzero(q["pixel"] >= 0)[0]
+    rng = np.random.default_rng(3)
+    sample = rng.choice(valid, size=min(len(valid), args.c3_check), replace=False)
+    tq = time.perf_counter()
+    bad = 0
+    for j in sample:
+        oi, od = og.query(qpos[j], pm.k)
+        if cnt[j] != len(oi) or not np.array_equal(idx[j, :cnt[j]], oi) or \
+                not np.array_equal(dist[j, :cnt[j]].view(np.int32), od.view(np.int32)):
+            bad += 1
+    cpu_query_s = time.perf_counter() - tq
+    if bad:
+        raise RuntimeError("%d of %d sampled photon queries differ from the oracle grid" % (bad, len(sample)))
+    info = pm.grid.info()
+    cand = grid_candidates(info, qpos)
+    found = cnt.astype(np.int64)
+    q_bytes = float(np.sum(np.where(np.isfinite(qpos[:, 0]), 12 + 8 * 8, 12) + cand * 12 + found * 8 + 4))
+    q_ms = phase["query"]
+    achieved = q_bytes / (q_ms * 1e-3) / 1e9
+    n_q = pm.n_cam
+    # build: bounds 12 B, count 12 + 4 B, scan 2 x 4 B per bucket, scatter 16 + 4 + 4 B per photon
+    b_bytes = float(n_ph * 12 + n_ph * 16 + 2 * info.table_size * 4 + n_ph * 24)
+    workload = "c3-ppm-%dx%d-%d-light-paths-depth6-sphere-grid-%dtri-30pct-specular" % (
+        args.width, args.height, pm.P, scene.tri_count)
+    res = {
+        "metric": METRIC, "value": rays_total / wall / 1e6, "unit": "Mrays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded procedural scene, light and camera)",
+        "config": {"workload": workload, "width": args.width, "height": args.height, "light_paths": pm.P,
+                   "max_depth": 6, "rr_start": 3, "k": pm.k, "radius": pm.radius, "photons": n_ph,
+                   "light_rays_per_bounce": light, "camera_rays_per_bounce": cam,
+                   "trace_config": cfg_dict(cfg),
+                   "frame": "eager launches with device-side queue sizes; one host read (photon count) per frame"},
+        "phase_ms": phase,
+        "frame_latency_ms": {"p50": float(np.percentile(frame_ms, 50)), "p99": float(np.percentile(frame_ms, 99))},
+        "grid": {"photons": n_ph, "table_size": info.table_size, "build_ms": phase["grid_build"],
+                 "queries": n_q, "query_ms": q_ms, "mqueries_s": n_q / (q_ms * 1e-3) / 1e6,
+                 "photons_visited_per_query": float(cand.mean()),
+                 "results_per_query": float(found.mean()),
+                 "parity": {"queries_checked": int(len(sample)), "mismatches": bad,
+                            "rule": "photon index lists and squared distances bit-exact against the oracle grid "
+                                    "built over the same photons"},
+                 "query_roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                    "frac": achieved / HBM_PEAK_GBS, "kernel": "k_query",
+                                    "algorithmic_bytes_per_launch": q_bytes,
+                                    "bytes_rule": "12 + 8x8 per query inside the bounds, 12 per photon visited "
+                                                  "(its position), 8 per result, 4 per count"},
+                 "build_algorithmic_bytes": b_bytes},
+        "cpu_grid": {"build_ms": cpu_build_s * 1e3, "mqueries_s": len(sample) / cpu_query_s / 1e6, "cores": 1,
+                     "kind": "port", "sample": "oracle/hashgrid.c (serial restatement of HashGrid) building over "
+                                               "the frame's %d photons and answering %d sampled queries "
+                                               "(one Python call per query)" % (n_ph, len(sample))},
+    }
+    if not args.no_cpu:
+        lr, ls = N.DeviceBuffer(pm.P * 32), N.DeviceBuffer(pm.P * 32)
+        N.check(N.lib.imba_ppm_light(N.ptr(pm.light_tri), N.ptr(pm.light_int), pm.light_seed, 0, pm.P, lr.ptr,
+                                     ls.ptr, None), "ppm_light")
+        light_rays = lr.download(N.RAY_DTYPE, pm.P)
+        res["cpu_baseline"] = cpu_baseline(scene.build, N.BUILDER_SBVH, threads, [light_rays], [False],
+                                           budget_s=args.cpu_seconds)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pt", choices=["pt", "c1", "c4", "c5"])
+    ap.add_argument("--workload", default="pt", choices=["pt", "c1", "c3", "c4", "c5"])
     ap.add_argument("--c5-requests", type=int, default=300, help="c5: requests in the run")
     ap.add_argument("--c5-rate", type=float, default=200.0, help="c5: mean arrivals per second")
     ap.add_argument("--c5-arena-gb", type=int, default=48, help="c5: device memory for request buffers")
     ap.add_argument("--c5-small", action="store_true", help="c5: eight small scenes (quick runs)")
+    ap.add_argument("--c3-paths", type=int, default=921_600, help="c3: light paths per frame")
+    ap.add_argument("--c3-check", type=int, default=20_000, help="c3: queries checked against the oracle grid")
     ap.add_argument("--c4-rays", type=int, default=0,
                     help="c4: rescale the 256 log-uniform queue sizes to this many rays (0: as drawn, about 37M)")
     ap.add_argument("--width", type=int, default=1280)
@@ -531,7 +687,7 @@ def main():
         N.check(N.lib.imba_memcpy_d2d(b.ptr, a.ptr, nbytes, None), "copy")
         print(json.dumps({"calibrate_copy_bytes": nbytes}))
         return
-    res = {"pt": run_pt, "c1": run_c1, "c4": run_c4, "c5": run_c5}[args.workload](args)
+    res = {"pt": run_pt, "c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

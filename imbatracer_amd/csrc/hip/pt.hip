@@ -13,39 +13,10 @@
 #include <hip/hip_runtime.h>
 
 #include "common.h"
+#include "sampling.h"
 
 namespace imba {
 namespace {
-
-struct Rng {
-    uint64_t s;
-    __device__ uint32_t next() {
-        const uint32_t c = (uint32_t)(s >> 32);
-        const uint32_t x = (uint32_t)(s & 0xFFFFFFFFull);
-        s = (uint64_t)x * 4294883355ull + c;
-        return x ^ c;
-    }
-    __device__ float uniform() { return (float)next() / 4294967296.0f; }
-};
-
-__device__ __forceinline__ Rng seeded(uint32_t seed_base, int index, int pixel) {
-    int seed = (int)seed_base;
-    for (int k = 0; k < 4; k++) seed = (int)((uint32_t)(33u * (uint32_t)seed) ^ (uint32_t)index);
-    Rng r;
-    r.s = (uint64_t)(int64_t)seed;
-    const int n = (seed % 5) + 16 + pixel % 5;
-    for (int k = 0; k < n; k++) r.next();
-    return r;
-}
-
-struct PtState {  // 32 bytes; pixel_id first so compact_rays can test it
-    int pixel_id;
-    int sample_id;
-    uint64_t rng;
-    float tp[3];
-    int bounces;
-};
-static_assert(sizeof(PtState) == 32, "PtState layout");
 
 struct CamArgs {
     float m[16];  // raster -> world (row major), applied to (raster y, raster x, 0, 1)
@@ -54,6 +25,7 @@ struct CamArgs {
     uint32_t seed_base;
     int first;  // first sample index of this launch
     int count;
+    float4* queries;  // photon-mapping merge queries (4 float4 per sample) to mark empty, or null
 };
 
 __global__ __launch_bounds__(256) void k_camera(CamArgs c, float4* rays, PtState* states) {
@@ -80,6 +52,9 @@ __global__ __launch_bounds__(256) void k_camera(CamArgs c, float4* rays, PtState
     s.tp[0] = s.tp[1] = s.tp[2] = 1.0f;
     s.bounces = 0;
     states[j] = s;
+    if (c.queries)
+        c.queries[4 * (size_t)i] = make_float4(__int_as_float(0x7f800000), __int_as_float(0x7f800000),
+                                               __int_as_float(0x7f800000), __int_as_float(-1));
 }
 
 struct BounceArgs {
@@ -151,25 +126,14 @@ __global__ __launch_bounds__(256) void k_bounce(BounceArgs a) {
     const float px = o.x + t * d.x, py = o.y + t * d.y, pz = o.z + t * d.z;
     float nx = a.tri_normals[3 * (size_t)h.x], ny = a.tri_normals[3 * (size_t)h.x + 1], nz = a.tri_normals[3 * (size_t)h.x + 2];
     if (nx * d.x + ny * d.y + nz * d.z > 0.0f) { nx = -nx; ny = -ny; nz = -nz; }
-    // orthonormal frame around n
-    const bool xbig = fabsf(nx) > fabsf(ny);
-    const float n0 = xbig ? nx : ny;
-    const float sig = xbig ? -1.0f : 1.0f;
-    const float il = 1.0f / sqrtf(n0 * n0 + nz * nz);
-    float tx, ty, tz;
-    if (xbig) { tx = nz * sig * il; ty = 0.0f; } else { ty = nz * sig * il; tx = 0.0f; }
-    tz = n0 * -1.0f * sig * il;
-    float bx = ny * tz - nz * ty, by = nz * tx - nx * tz, bz = nx * ty - ny * tx;
-    const float bl = 1.0f / sqrtf(bx * bx + by * by + bz * bz);
-    bx *= bl; by *= bl; bz *= bl;
+    Vec3 tg, bn;
+    const Vec3 nv = v3(nx, ny, nz);
+    frame(nv, tg, bn);
     const float u1 = rng.uniform(), u2 = rng.uniform();
-    const float sin_t = sqrtf(1.0f - u2), cos_t = sqrtf(u2);
-    float sp, cp;
-    sincosf(2.0f * 3.14159265359f * u1, &sp, &cp);
-    const float lx = sin_t * cp, ly = sin_t * sp, lz = cos_t;
-    const float wx = tx * lx + bx * ly + nx * lz;
-    const float wy = ty * lx + by * ly + ny * lz;
-    const float wz = tz * lx + bz * ly + nz * lz;
+    const Vec3 l = cos_hemisphere(u1, u2);
+    const float wx = tg.x * l.x + bn.x * l.y + nx * l.z;
+    const float wy = tg.y * l.x + bn.y * l.y + ny * l.z;
+    const float wz = tg.z * l.x + bn.z * l.y + nz * l.z;
     s.tp[0] *= a.albedo[3 * mat];
     s.tp[1] *= a.albedo[3 * mat + 1];
     s.tp[2] *= a.albedo[3 * mat + 2];
@@ -187,8 +151,8 @@ using namespace imba;
 
 extern "C" {
 
-int imba_pt_camera(const float* raster_to_world16, const float* cam_pos3, int width, int height, int spp,
-                   uint32_t seed_base, int first, int count, ImbaRay* rays, void* states, void* stream) {
+static int camera(const float* raster_to_world16, const float* cam_pos3, int width, int height, int spp,
+                  uint32_t seed_base, int first, int count, ImbaRay* rays, void* states, void* queries, void* stream) {
     IMBA_CHECK_ARG(raster_to_world16 && cam_pos3 && rays && states, "NULL argument");
     IMBA_CHECK_ARG(width > 0 && height > 0 && spp > 0 && first >= 0 && count >= 0, "bad sizes");
     IMBA_CHECK_ARG((int64_t)first + count <= (int64_t)width * height * spp, "samples out of range");
@@ -202,12 +166,27 @@ int imba_pt_camera(const float* raster_to_world16, const float* cam_pos3, int wi
     c.seed_base = seed_base;
     c.first = first;
     c.count = count;
+    c.queries = (float4*)queries;
     bool sync;
     hipStream_t s = resolve_stream(stream, &sync);
     hipLaunchKernelGGL(k_camera, dim3(ceil_div(count, 256)), dim3(256), 0, s, c, (float4*)rays, (PtState*)states);
     IMBA_HIP_TRY(hipGetLastError());
     if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
     return IMBA_OK;
+}
+
+int imba_pt_camera(const float* raster_to_world16, const float* cam_pos3, int width, int height, int spp,
+                   uint32_t seed_base, int first, int count, ImbaRay* rays, void* states, void* stream) {
+    return camera(raster_to_world16, cam_pos3, width, height, spp, seed_base, first, count, rays, states, nullptr,
+                  stream);
+}
+
+int imba_ppm_camera(const float* raster_to_world16, const float* cam_pos3, int width, int height, int spp,
+                    uint32_t seed_base, int first, int count, ImbaRay* rays, void* states, void* queries,
+                    void* stream) {
+    IMBA_CHECK_ARG(queries, "NULL argument");
+    return camera(raster_to_world16, cam_pos3, width, height, spp, seed_base, first, count, rays, states, queries,
+                  stream);
 }
 
 static int pt_bounce(const ImbaRay* rays, const ImbaHit* hits, void* states, int count, const int* count_dev,

@@ -196,6 +196,11 @@ _sig("imba_gen_camera_matrix", i32, vp, vp, vp, f32, i32, i32, vp)
 _sig("imba_pt_camera", i32, vp, vp, i32, i32, i32, C.c_uint32, i32, i32, vp, vp, vp)
 _sig("imba_pt_bounce", i32, vp, vp, vp, i32, vp, vp, i32, vp, vp, i32, i32, i32, f32, vp, vp, vp)
 _sig("imba_pt_bounce_dev", i32, vp, vp, vp, i32, vp, vp, vp, i32, vp, vp, i32, i32, i32, f32, vp, vp, vp)
+_sig("imba_ppm_light", i32, vp, vp, C.c_uint32, i32, i32, vp, vp, vp)
+_sig("imba_ppm_light_bounce", i32, vp, vp, vp, i32, vp, vp, vp, i32, vp, vp, i32, i32, i32, vp, vp, vp, vp)
+_sig("imba_ppm_camera", i32, vp, vp, i32, i32, i32, C.c_uint32, i32, i32, vp, vp, vp, vp)
+_sig("imba_ppm_camera_bounce", i32, vp, vp, vp, i32, vp, vp, vp, i32, vp, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp)
+_sig("imba_ppm_merge", i32, vp, i32, i32, vp, vp, vp, vp, vp, f32, i32, vp, vp)
 _sig("intersect_gpu_masked_instanced", None, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32)
 _sig("occluded_gpu_masked_instanced", None, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32)
 

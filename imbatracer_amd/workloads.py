@@ -140,7 +140,7 @@ def open_box(lo, hi):
 
 class SphereGridScene:
     def __init__(self, side: int = 32, level: int = 5, radius: float = 0.4, spacing: float = 1.0,
-                 builder: int = N.BUILDER_SBVH, threads: int = 8, seed: int = 0x5EED):
+                 builder: int = N.BUILDER_SBVH, threads: int = 8, seed: int = 0x5EED, specular_fraction: float = 0.0):
         self.side = side
         ext = side * spacing
         self.bounds = ((-1.0, 0.0, -12.0), (ext + 1.0, 8.0, ext + 1.0))
@@ -154,6 +154,14 @@ class SphereGridScene:
         self.albedo[0] = 0.6
         draws = mwc_uniform(seed, 3 * side * side)
         self.albedo[1:1 + side * side] = (0.2 + 0.6 * draws).reshape(-1, 3)
+        # material kinds (photon mapping): 0 Lambertian, 1 mirror, 2 glass; a fraction of the
+        # spheres, half mirrors and half glass, chosen by a second draw per sphere
+        self.kinds = np.zeros(n_mat, np.int32)
+        if specular_fraction > 0.0:
+            u = mwc_uniform(seed ^ 0x5BEC, side * side)
+            sph = self.kinds[1:1 + side * side]
+            sph[u < 0.5 * specular_fraction] = 1
+            sph[(u >= 0.5 * specular_fraction) & (u < specular_fraction)] = 2
         for k in range(side * side):
             gx, gz = k % side, k // side
             centre = ((gx + 0.5) * spacing, radius, (gz + 0.5) * spacing)
@@ -164,6 +172,8 @@ class SphereGridScene:
                        [ext * 0.5, 7.9, ext * 0.65, 1]], np.float32)
         self.build.add_mesh(lv, np.array([[0, 2, 1, light]], np.uint32))
         self.emission[light] = 40.0
+        # the light's vertices in the order whose normal, cross(v1 - v0, v2 - v0), faces the scene
+        self.light_tri = np.ascontiguousarray(lv[:, :3])
         self.build.run(builder, threads)
         self.desc = self.build.desc()
         self.tri_normals, self.tri_materials = self.build.tri_attrs()
@@ -316,3 +326,217 @@ class PathTracer:
                 break
             out.append(int(v))
         return out
+
+
+PHOTON_RAY_DTYPE = np.dtype([("pos", "<f4", 3), ("length", "<i4"), ("out_dir", "<f4", 3), ("pad", "<f4")])
+PHOTON_STATE_DTYPE = np.dtype([("path", "<i4"), ("tp", "<f4", 3)])
+QUERY_DTYPE = np.dtype([("pos", "<f4", 3), ("pixel", "<i4"), ("normal", "<f4", 3), ("pad0", "<f4"),
+                        ("out_dir", "<f4", 3), ("pad1", "<f4"), ("weight", "<f4", 3), ("pad2", "<f4")])
+
+
+class PhotonMapper:
+    """Device-resident photon-mapping frame (the PPM mode of the VCM integrator,
+    render/integrators/vcm.cpp with ALGO_PPM) over a SphereGridScene:
+
+      light pass   light_paths paths from the triangle light, up to max_depth hits each; every
+                   non-specular hit deposits a photon into slot (bounce, queue entry)
+      photons      the occupied slots, stable-compacted (by bounce, then by path)
+      grid         HashGrid over the photon positions with radius = radius_factor * the scene
+                   diagonal (LightVertices::build, light_vertices.h:70-76)
+      camera pass  one path per pixel and sample, continued through mirrors and glass; the first
+                   non-specular hit becomes a merge query
+      query        the k nearest photons within the radius of each query (HashGrid::query)
+      merge        vertex_merging into the image
+
+    Every traversal, producer and compaction runs on the device with queue sizes kept on the
+    device; the host reads one number per frame (the photon count the grid build needs)."""
+
+    def __init__(self, scene: SphereGridScene, width: int, height: int, light_paths: int, max_depth: int = 6,
+                 rr_start: int = 3, radius_factor: float = 0.005, k: int = 10, spp: int = 1,
+                 seed_base: int = 0x1234567, light_seed: int = 0x7654321, trace_cfg: N.TraceConfig | None = None):
+        from .scene import HashGrid
+
+        self.s = scene
+        self.gpu_scene = S.Scene(scene.desc)
+        self.w, self.h, self.spp = width, height, spp
+        self.n_cam = width * height * spp
+        self.P = light_paths
+        self.L = max_depth
+        self.rr_start, self.k = rr_start, k
+        self.seed_base, self.light_seed = seed_base, light_seed
+        self.cfg = trace_cfg
+        lo, hi = np.asarray(scene.bounds[0], np.float64), np.asarray(scene.bounds[1], np.float64)
+        self.radius = float(np.float32(radius_factor * np.linalg.norm(hi - lo)))
+        pos, d, up, fov = scene.camera
+        self.cam_pos = np.asarray(pos, np.float32)
+        self.r2w = np.zeros(16, np.float32)
+        N.check(N.lib.imba_gen_camera_matrix(N.ptr(self.cam_pos), N.ptr(np.asarray(d, np.float32)),
+                                             N.ptr(np.asarray(up, np.float32)), fov, width, height, N.ptr(self.r2w)),
+                "camera matrix")
+        cap = max(self.P, self.n_cam)
+        self.cap = cap
+        self.rays = [N.DeviceBuffer(cap * 32), N.DeviceBuffer(cap * 32)]
+        self.states = [N.DeviceBuffer(cap * STATE_BYTES), N.DeviceBuffer(cap * STATE_BYTES)]
+        self.hits = N.DeviceBuffer(cap * 16)
+        slots = self.L * self.P
+        self.slots = slots
+        self.slot_rays = N.DeviceBuffer(slots * 32)
+        self.slot_states = N.DeviceBuffer(slots * 16)
+        self.photon_rays = N.DeviceBuffer(slots * 32)
+        self.photon_states = N.DeviceBuffer(slots * 16)
+        self.queries = N.DeviceBuffer(self.n_cam * 64)
+        self.out_idx = N.DeviceBuffer(self.n_cam * k * 4)
+        self.out_dist = N.DeviceBuffer(self.n_cam * k * 4)
+        self.out_count = N.DeviceBuffer(self.n_cam * 4)
+        self.image = N.DeviceBuffer(width * height * 12)
+        # device counts: light queue sizes [0, L], camera queue sizes [L+1, 2L+1], slot count,
+        # photon count
+        self.counts = N.DeviceBuffer(4 * (2 * self.L + 4))
+        self.workspace = N.DeviceBuffer(int(N.lib.imba_compact_workspace_size(max(slots, cap))))
+        self.tri_n = N.DeviceBuffer.from_array(scene.tri_normals)
+        self.tri_m = N.DeviceBuffer.from_array(scene.tri_materials)
+        self.albedo = N.DeviceBuffer.from_array(scene.albedo)
+        self.emission = N.DeviceBuffer.from_array(scene.emission)
+        self.kinds = N.DeviceBuffer.from_array(scene.kinds)
+        light = len(scene.albedo) - 1
+        self.light_tri = np.ascontiguousarray(scene.light_tri, np.float32).reshape(-1)
+        self.light_int = np.ascontiguousarray(scene.emission[light], np.float32)
+        self.grid = HashGrid()
+        self.stream = N.Stream()
+        self.photon_count = 0
+        self.events = {}
+
+    def _cnt(self, j: int):
+        return self.counts.at(4 * j)
+
+    def _cfg(self):
+        return C.byref(self.cfg) if self.cfg else None
+
+    def _reset_counts(self):
+        c = np.zeros(2 * self.L + 4, np.int32)
+        c[0] = self.P
+        c[self.L + 1] = self.n_cam
+        c[2 * self.L + 2] = self.slots
+        self._counts0 = c  # the host side of the copy stays alive until the stream has run it
+        self.counts.upload(c, self.stream)
+
+    def light_pass(self):
+        st = self.stream.handle
+        P, L = self.P, self.L
+        N.check(N.lib.imba_ppm_light(N.ptr(self.light_tri), N.ptr(self.light_int), self.light_seed, 0, P,
+                                     self.rays[0].ptr, self.states[0].ptr, st), "ppm_light")
+        cur = 0
+        for b in range(L):
+            cnt = self._cnt(b)
+            N.check(N.lib.imba_intersect_dev(self.gpu_scene.h, self.rays[cur].ptr, self.hits.ptr, P, cnt, self._cfg(),
+                                             st), "intersect_dev")
+            N.check(N.lib.imba_ppm_light_bounce(self.rays[cur].ptr, self.hits.ptr, self.states[cur].ptr, P, cnt,
+                                                self.tri_n.ptr, self.tri_m.ptr, self.s.tri_count, self.albedo.ptr,
+                                                self.kinds.ptr, len(self.s.albedo), L, self.rr_start,
+                                                self.slot_rays.at(32 * b * P), self.slot_states.at(16 * b * P),
+                                                self.rays[cur].ptr, st), "ppm_light_bounce")
+            N.check(N.lib.imba_compact_rays_dev(self.rays[cur].ptr, self.states[cur].ptr, STATE_BYTES, P, cnt,
+                                                self.rays[1 - cur].ptr, self.states[1 - cur].ptr, self._cnt(b + 1),
+                                                self.workspace.ptr, st), "compact_rays_dev")
+            cur = 1 - cur
+
+    def compact_photons(self):
+        N.check(N.lib.imba_compact_rays_dev(self.slot_rays.ptr, self.slot_states.ptr, 16, self.slots,
+                                            self._cnt(2 * self.L + 2), self.photon_rays.ptr, self.photon_states.ptr,
+                                            self._cnt(2 * self.L + 3), self.workspace.ptr, self.stream.handle),
+                "compact photons")
+
+    def read_photon_count(self) -> int:
+        c = self.counts.download(np.int32, 1, self.stream, offset=4 * (2 * self.L + 3))
+        self.stream.sync()
+        self.photon_count = int(c[0])
+        return self.photon_count
+
+    def build_grid(self):
+        self.grid.build(self.photon_rays, 32, 0, self.photon_count, self.radius, self.stream)
+
+    def camera_pass(self):
+        st = self.stream.handle
+        n, L = self.n_cam, self.L
+        N.check(N.lib.imba_ppm_camera(N.ptr(self.r2w), N.ptr(self.cam_pos), self.w, self.h, self.spp, self.seed_base,
+                                      0, n, self.rays[0].ptr, self.states[0].ptr, self.queries.ptr, st), "ppm_camera")
+        cur = 0
+        for b in range(L):
+            cnt = self._cnt(L + 1 + b)
+            N.check(N.lib.imba_intersect_dev(self.gpu_scene.h, self.rays[cur].ptr, self.hits.ptr, n, cnt, self._cfg(),
+                                             st), "intersect_dev")
+            N.check(N.lib.imba_ppm_camera_bounce(self.rays[cur].ptr, self.hits.ptr, self.states[cur].ptr, n, cnt,
+                                                 self.tri_n.ptr, self.tri_m.ptr, self.s.tri_count, self.albedo.ptr,
+                                                 self.emission.ptr, self.kinds.ptr, len(self.s.albedo), L,
+                                                 self.rr_start, self.spp, self.queries.ptr, self.image.ptr,
+                                                 self.rays[cur].ptr, st), "ppm_camera_bounce")
+            N.check(N.lib.imba_compact_rays_dev(self.rays[cur].ptr, self.states[cur].ptr, STATE_BYTES, n, cnt,
+                                                self.rays[1 - cur].ptr, self.states[1 - cur].ptr,
+                                                self._cnt(L + 2 + b), self.workspace.ptr, st), "compact_rays_dev")
+            cur = 1 - cur
+
+    def query(self):
+        self.grid.query(self.queries, 64, 0, self.n_cam, self.k, self.out_idx, self.out_count, self.out_dist,
+                        self.stream)
+
+    def merge(self):
+        N.check(N.lib.imba_ppm_merge(self.queries.ptr, self.n_cam, self.k, self.out_idx.ptr, self.out_dist.ptr,
+                                     self.out_count.ptr, self.photon_rays.ptr, self.photon_states.ptr,
+                                     self.radius, self.P, self.image.ptr, self.stream.handle), "ppm_merge")
+
+    def clear_image(self):
+        N.check(N.lib.imba_memset(self.image.ptr, 0, self.w * self.h * 12, self.stream.handle), "memset")
+
+    def frame(self, timed: bool = False):
+        """One frame; returns (light queue sizes, camera queue sizes, photon count).  With
+        timed=True the phases are bracketed by events (see phase_ms)."""
+        marks = []
+
+        def mark(name):
+            if timed:
+                e = N.Event()
+                e.record(self.stream)
+                marks.append((name, e))
+
+        self._reset_counts()
+        mark("start")
+        self.light_pass()
+        mark("light")
+        self.compact_photons()
+        mark("photons")
+        self.read_photon_count()
+        mark("count")
+        self.build_grid()
+        mark("grid_build")
+        self.camera_pass()
+        mark("camera")
+        self.query()
+        mark("query")
+        self.merge()
+        mark("merge")
+        c = self.counts.download(np.int32, 2 * self.L + 4, self.stream)
+        self.stream.sync()
+        self.events = marks
+        light = [int(v) for v in c[:self.L] if v > 0]
+        cam = [int(v) for v in c[self.L + 1:2 * self.L + 1] if v > 0]
+        return light, cam, self.photon_count
+
+    def phase_ms(self) -> dict:
+        out = {}
+        for (_, e0), (name, e1) in zip(self.events, self.events[1:]):
+            out[name] = e0.elapsed_ms(e1)
+        return out
+
+    def photons_host(self):
+        n = self.photon_count
+        return (self.photon_rays.download(PHOTON_RAY_DTYPE, n, self.stream),
+                self.photon_states.download(PHOTON_STATE_DTYPE, n, self.stream))
+
+    def queries_host(self):
+        return self.queries.download(QUERY_DTYPE, self.n_cam, self.stream)
+
+    def results_host(self):
+        n, k = self.n_cam, self.k
+        return (self.out_idx.download(np.int32, n * k, self.stream).reshape(n, k),
+                self.out_dist.download(np.float32, n * k, self.stream).reshape(n, k),
+                self.out_count.download(np.int32, n, self.stream))

@@ -446,6 +446,48 @@ int imba_pt_bounce_dev(const ImbaRay* rays, const ImbaHit* hits, void* states, i
                        const float* emission, int material_count, int max_path_len, int rr_start, float offset,
                        float* image, ImbaRay* rays_out, void* stream);
 
+/* ------------------------------ photon-mapping ray producers ------------------------------
+ * The PPM mode of the VCM integrator (render/integrators/vcm.cpp, algo == ALGO_PPM) on the
+ * device: light paths deposit photons that HashGrid indexes, camera paths end in merge queries
+ * answered by imba_hashgrid_query.  States are the 32-byte path-tracing states above
+ * (pixel_id = light path index for light paths).  Material kinds: 0 Lambertian, 1 mirror,
+ * 2 glass (index of refraction 1.5).
+ * imba_ppm_light: light paths first..first+count-1 from the triangle light_tri9 (3 vertices;
+ *   it emits around cross(v1 - v0, v2 - v0)), seeded as LightRayGen
+ *   (render/ray_gen/ray_gen.h:122-152), sampled as TriangleLight::sample_emit
+ *   (render/light.h:109-141); throughput = intensity * area * pi.
+ * imba_ppm_light_bounce: one hit of every light path of the queue (process_light_rays,
+ *   vcm.cpp:196-256): a non-specular hit stores a photon in slot i of photon_rays (2 float4:
+ *   position, path length | direction back along the ray) and photon_states (1 float4: path
+ *   index, throughput); every other slot of the count entries (or of the capacity, with
+ *   count_dev) gets path index -1.  Paths end after max_depth hits, by Russian roulette from
+ *   bounce rr_start on, or on a miss; the next ray starts at the hit with tmin = 1e-4 t.
+ *   Photon slots are stable-compacted with imba_compact_rays (state stride 16).
+ * imba_ppm_camera: imba_pt_camera, and marks the merge query of each sample empty.
+ * imba_ppm_camera_bounce: one hit of every camera path (process_camera_rays, vcm.cpp:372-441,
+ *   PPM): an emitter adds throughput * emission to image and ends the path; a non-specular hit
+ *   writes the sample's merge query (4 float4: position, pixel | normal | direction to the
+ *   camera | throughput * albedo / pi) and ends the path; mirror and glass continue it.
+ * imba_ppm_merge: vertex_merging (vcm.cpp:572-611, mis weight 1) from imba_hashgrid_query's
+ *   output over the queries: Epanechnikov-weighted photon throughput into image.
+ * count_dev (nullable): the queue size read on the device when the launch runs. */
+int imba_ppm_light(const float* light_tri9, const float* intensity3, uint32_t seed_base, int first, int count,
+                   ImbaRay* rays, void* states, void* stream);
+int imba_ppm_light_bounce(const ImbaRay* rays, const ImbaHit* hits, void* states, int count, const int* count_dev,
+                          const float* tri_normals, const int* tri_materials, int tri_count, const float* albedo,
+                          const int* kinds, int material_count, int max_depth, int rr_start, void* photon_rays,
+                          void* photon_states, ImbaRay* rays_out, void* stream);
+int imba_ppm_camera(const float* raster_to_world16, const float* cam_pos3, int width, int height, int spp,
+                    uint32_t seed_base, int first, int count, ImbaRay* rays, void* states, void* queries,
+                    void* stream);
+int imba_ppm_camera_bounce(const ImbaRay* rays, const ImbaHit* hits, void* states, int count, const int* count_dev,
+                           const float* tri_normals, const int* tri_materials, int tri_count, const float* albedo,
+                           const float* emission, const int* kinds, int material_count, int max_depth, int rr_start,
+                           int spp, void* queries, float* image, ImbaRay* rays_out, void* stream);
+int imba_ppm_merge(const void* queries, int nq, int k, const int* out_idx, const float* out_dist, const int* out_count,
+                   const void* photon_rays, const void* photon_states, float radius, int light_paths, float* image,
+                   void* stream);
+
 /* ------------------------------ synthetic workloads ---------------------------------------
  * Seeded generators for the measured configurations (host memory outputs). */
 /* Random triangles in the unit cube: per triangle a centre ~ U[0,1)^3, then three vertices
