@@ -456,21 +456,36 @@ def run_c5(args):
     reqs = V.poisson_requests(args.c5_requests, args.c5_rate, 16, len(scenes), centres, seed=11)
     cfg = fixed_config(args)
     arena = args.c5_arena_gb << 30
-    results = {}
-    for name, batched in (("batched", True), ("plain", False)):
-        server = V.Server(scenes, arena, max_active=32, batched=batched, cfg=cfg)
-        # warm-up on a few requests (kernels loaded, batch tables sized)
-        server.run(V.poisson_requests(4, 1000.0, 1, len(scenes), centres, seed=99))
-        server = V.Server(scenes, arena, max_active=32, batched=batched, cfg=cfg)
-        t = time.perf_counter()
-        done = server.run([V.Request(**{**r.__dict__}) for r in reqs], progress=log)
-        wall = time.perf_counter() - t
-        st = V.latency_stats(done)
-        st["steps"] = server.steps
-        st["wall_s"] = wall
-        results[name] = st
-        log("%s: %s" % (name, st))
+
+    def compare(request_list, label):
+        out = {}
+        for name, batched in (("batched", True), ("plain", False)):
+            server = V.Server(scenes, arena, max_active=32, batched=batched, cfg=cfg)
+            # warm-up on a few requests (kernels loaded, batch tables sized)
+            server.run(V.poisson_requests(4, 1000.0, 1, len(scenes), centres, seed=99))
+            server = V.Server(scenes, arena, max_active=32, batched=batched, cfg=cfg)
+            t = time.perf_counter()
+            done = server.run([V.Request(**{**r.__dict__}) for r in request_list], progress=log)
+            wall = time.perf_counter() - t
+            st = V.latency_stats(done)
+            st["steps"] = server.steps
+            st["wall_s"] = wall
+            out[name] = st
+            log("%s %s: %s" % (label, name, st))
+        return out
+
+    results = compare(reqs, "poisson")
     b, p = results["batched"], results["plain"]
+    # capacity: the same requests all waiting at the start (the loop is never idle), and a mix
+    # of small requests (256x256, 1 spp) that leave the GPU underfilled when run one at a time
+    sat = compare([V.Request(**{**r.__dict__, "arrival": 0.0}) for r in reqs], "saturated")
+    small = [V.Request(**{**r.__dict__, "width": 256, "height": 256, "spp": 1, "arrival": 0.0}) for r in reqs]
+    sm = compare(small, "small")
+
+    def ratio(d):
+        return {"throughput": d["batched"]["mrays_s"] / d["plain"]["mrays_s"],
+                "requests_per_s": d["batched"]["requests_per_s"] / d["plain"]["requests_per_s"],
+                "p99": d["batched"]["p99_ms"] / d["plain"]["p99_ms"]}
     res = {
         "metric": METRIC, "value": b["mrays_s"], "unit": "Mrays/s", "n_gpus": 1,
         "steps": len(reqs), "warmup": 4, "ms_per_step": b["wall_s"] * 1e3 / len(reqs),
@@ -481,6 +496,10 @@ def run_c5(args):
                    "resolutions": ["256x256", "640x480", "1280x720", "1920x1080"], "spp": "1-4"},
         "served": b, "plain_per_request": p,
         "served_vs_plain": {"throughput": b["mrays_s"] / p["mrays_s"], "p99": b["p99_ms"] / p["p99_ms"]},
+        "saturated": {"served": sat["batched"], "plain_per_request": sat["plain"], "served_vs_plain": ratio(sat),
+                      "arrivals": "all %d requests waiting at the start" % len(reqs)},
+        "small_requests": {"served": sm["batched"], "plain_per_request": sm["plain"], "served_vs_plain": ratio(sm),
+                           "arrivals": "the same %d requests at 256x256, 1 spp, all waiting at the start" % len(reqs)},
         "frame_latency_ms": {"p50": b["p50_ms"], "p99": b["p99_ms"]},
     }
     return res

@@ -370,9 +370,10 @@ class PhotonMapper:
         pos, d, up, fov = scene.camera
         self.cam_pos = np.asarray(pos, np.float32)
         self.r2w = np.zeros(16, np.float32)
-        N.check(N.lib.imba_gen_camera_matrix(N.ptr(self.cam_pos), N.ptr(np.asarray(d, np.float32)),
-                                             N.ptr(np.asarray(up, np.float32)), fov, width, height, N.ptr(self.r2w)),
-                "camera matrix")
+        dd, uu = np.asarray(d, np.float32), np.asarray(up, np.float32)  # alive across the call
+        N.check(N.lib.imba_gen_camera_matrix(N.ptr(self.cam_pos), N.ptr(dd), N.ptr(uu), fov, width, height,
+                                             N.ptr(self.r2w)), "camera matrix")
+        assert np.all(np.isfinite(self.r2w))
         cap = max(self.P, self.n_cam)
         self.cap = cap
         self.rays = [N.DeviceBuffer(cap * 32), N.DeviceBuffer(cap * 32)]

@@ -140,7 +140,9 @@ def test_first_hit_queries(pm):
     ended = ~hit | (hit & emissive)
     assert np.all(q["pixel"][ended] == -1)
     assert np.all(np.ascontiguousarray(q["pos"][ended]).view(np.int32) == F32_INF_BITS)
+    assert np.all(np.isfinite(rays["dir"][:, :3]))
     i = np.nonzero(diffuse)[0]
+    assert len(i) > 0.5 * len(rays)
     assert np.array_equal(q["pixel"][i], i)
     assert np.array_equal(q["pos"][i], rays["org"][i, :3] + hits["tmax"][i, None] * d[i])
     assert np.array_equal(q["normal"][i], nrm[i])
