@@ -460,10 +460,10 @@ def run_c5(args):
     def compare(request_list, label):
         out = {}
         for name, batched in (("batched", True), ("plain", False)):
-            server = V.Server(scenes, arena, max_active=32, batched=batched, cfg=cfg)
+            server = V.Server(scenes, arena, max_active=args.c5_max_active, batched=batched, cfg=cfg)
             # warm-up on a few requests (kernels loaded, batch tables sized)
             server.run(V.poisson_requests(4, 1000.0, 1, len(scenes), centres, seed=99))
-            server = V.Server(scenes, arena, max_active=32, batched=batched, cfg=cfg)
+            server = V.Server(scenes, arena, max_active=args.c5_max_active, batched=batched, cfg=cfg)
             t = time.perf_counter()
             done = server.run([V.Request(**{**r.__dict__}) for r in request_list], progress=log)
             wall = time.perf_counter() - t
@@ -493,6 +493,7 @@ def run_c5(args):
         "data": "synthetic (seeded procedural scenes, Poisson arrivals, random cameras)",
         "config": {"workload": "c5-serving-%d-requests-%.0f-per-s-16-sessions-8-scenes" % (len(reqs), args.c5_rate),
                    "scene_triangles": tris, "integrator": "pt", "arena_gb": args.c5_arena_gb,
+                   "max_active": args.c5_max_active, "own_launch_rays": V.Server.OWN_LAUNCH_RAYS,
                    "resolutions": ["256x256", "640x480", "1280x720", "1920x1080"], "spp": "1-4"},
         "served": b, "plain_per_request": p,
         "served_vs_plain": {"throughput": b["mrays_s"] / p["mrays_s"], "p99": b["p99_ms"] / p["p99_ms"]},
@@ -667,6 +668,7 @@ def main():
     ap.add_argument("--c5-rate", type=float, default=200.0, help="c5: mean arrivals per second")
     ap.add_argument("--c5-arena-gb", type=int, default=48, help="c5: device memory for request buffers")
     ap.add_argument("--c5-small", action="store_true", help="c5: eight small scenes (quick runs)")
+    ap.add_argument("--c5-max-active", type=int, default=64, help="c5: requests in flight at once (batched loop)")
     ap.add_argument("--c3-paths", type=int, default=921_600, help="c3: light paths per frame")
     ap.add_argument("--c3-check", type=int, default=20_000, help="c3: queries checked against the oracle grid")
     ap.add_argument("--c4-rays", type=int, default=0,

@@ -5,9 +5,10 @@ its buffers fit in a fixed device arena (admission by HBM footprint, first come 
 and a counter slot is free; its queues then live in the arena until it completes.  Every step
 of the loop:
   * admitted requests get their camera rays (the reference's per-pixel seeding);
-  * the current queue of every active request, whatever its scene and bounce, goes into one
-    batched traversal launch (imba_trace_batch_update + imba_trace_batch_launch), so requests
-    that arrived at different times share launches;
+  * the current queue of every active request, whatever its scene and bounce, is traversed:
+    the short ones together in one batched launch (imba_trace_batch_update +
+    imba_trace_batch_launch), so requests that arrived at different times share launches, and
+    each queue long enough to fill the GPU alone in a launch of its own;
   * each request then runs its Lambertian bounce and compaction with its queue size kept on
     the device, and one read-back of all the requests' queue sizes ends the step.
 A request completes when its queue is empty or its paths reach the maximum length; its latency
@@ -132,6 +133,7 @@ class _Job:
         self.workspace = take(int(N.lib.imba_compact_workspace_size(n)))
         self.bounce = 0
         self.cur = 0
+        self.queued = n  # rays in the current queue (as of the last read-back)
         self.r2w = np.zeros(16, np.float32)
         self.pos = np.asarray(req.cam_pos, np.float32)
         d = np.asarray(req.cam_dir, np.float32)
@@ -141,6 +143,10 @@ class _Job:
 
 
 class Server:
+    # A queue at least this long fills the GPU by itself: it gets its own k_trace launch in a
+    # batched step, and the shorter queues share one batched launch.
+    OWN_LAUNCH_RAYS = 131072
+
     def __init__(self, scenes: list, arena_bytes: int, max_active: int = 32, batched: bool = True,
                  max_path_len: int = 8, rr_start: int = 3, cfg: N.TraceConfig | None = None):
         self.scenes = scenes
@@ -197,14 +203,15 @@ class Server:
         if not self.active:
             return
         st = self.stream.handle
-        items = [(j.sc.gpu, j.rays[j.cur], j.hits, j.n, self.counts_ptr(j.slot, j.bounce), False) for j in self.active]
-        if self.batched:
+        own = [j for j in self.active if not self.batched or j.queued >= self.OWN_LAUNCH_RAYS]
+        shared = [j for j in self.active if self.batched and j.queued < self.OWN_LAUNCH_RAYS]
+        for j in own:
+            N.check(N.lib.imba_intersect_dev(j.sc.gpu.h, j.rays[j.cur], j.hits, j.n, self.counts_ptr(j.slot, j.bounce),
+                                             C.byref(self.cfg) if self.cfg else None, st), "intersect_dev")
+        if shared:
+            items = [(j.sc.gpu, j.rays[j.cur], j.hits, j.n, self.counts_ptr(j.slot, j.bounce), False) for j in shared]
             self.batch.update(items, self.stream)
             self.batch.launch(self.cfg, self.stream)
-        else:
-            for j in self.active:
-                N.check(N.lib.imba_intersect_dev(j.sc.gpu.h, j.rays[j.cur], j.hits, j.n, self.counts_ptr(j.slot, j.bounce),
-                                                 C.byref(self.cfg) if self.cfg else None, st), "intersect_dev")
         for j in self.active:
             sc = j.sc
             cnt = self.counts_ptr(j.slot, j.bounce)
@@ -224,6 +231,7 @@ class Server:
         still = []
         for j in self.active:
             c = allc[j.slot]
+            j.queued = int(c[j.bounce])
             if c[j.bounce] == 0 or j.bounce >= self.max_path_len:
                 req = j.req
                 req.done = self.now()
