@@ -221,7 +221,7 @@ def run_pt(args):
         raise RuntimeError("traversal reported a malformed scene")
     achieved = bytes_total / (trace_ms * 1e-3) / 1e9
     workload = "pt-%dx%d-1spp-depth8-rr3-sphere-grid-%dtri-sbvh" % (args.width, args.height, scene.tri_count)
-    traffic, traffic_src = measured_traffic(workload, kernel_label(cfg)) if args.block == 0 else (None, None)
+    traffic, traffic_src = measured_traffic(workload, kernel_label(cfg))
     res = {
         "metric": METRIC, "value": rays_total / wall / 1e6, "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
@@ -299,7 +299,7 @@ def run_c1(args):
     t_closest = sum(a.elapsed_ms(b) for a, b, c in ev)
     achieved = ray_bytes(vis_p, n, node_bytes) * args.steps / (t_closest * 1e-3) / 1e9
     workload = "c1-65536-random-tris-binned-sah-1M-primary-1M-occlusion"
-    traffic, traffic_src = measured_traffic(workload, kernel_label(cfg)) if args.block == 0 else (None, None)
+    traffic, traffic_src = measured_traffic(workload, kernel_label(cfg))
     res = {
         "metric": METRIC, "value": 2 * n * args.steps / wall / 1e6, "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
@@ -618,6 +618,7 @@ def run_c3(args):
     b_bytes = float(n_ph * 12 + n_ph * 16 + 2 * info.table_size * 4 + n_ph * 24)
     workload = "c3-ppm-%dx%d-%d-light-paths-depth6-sphere-grid-%dtri-30pct-specular" % (
         args.width, args.height, pm.P, scene.tri_count)
+    q_traffic, q_traffic_src = measured_traffic(workload, "k_query")
     res = {
         "metric": METRIC, "value": rays_total / wall / 1e6, "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
@@ -629,6 +630,7 @@ def run_c3(args):
                    "trace_config": cfg_dict(cfg),
                    "frame": "eager launches with device-side queue sizes; one host read (photon count) per frame"},
         "phase_ms": phase,
+        "roofline": None,
         "frame_latency_ms": {"p50": float(np.percentile(frame_ms, 50)), "p99": float(np.percentile(frame_ms, 99))},
         "grid": {"photons": n_ph, "table_size": info.table_size, "build_ms": phase["grid_build"],
                  "queries": n_q, "query_ms": q_ms, "mqueries_s": n_q / (q_ms * 1e-3) / 1e6,
@@ -639,6 +641,7 @@ def run_c3(args):
                                     "built over the same photons"},
                  "query_roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                     "frac": achieved / HBM_PEAK_GBS, "kernel": "k_query",
+                                    "traffic": q_traffic, "traffic_source": q_traffic_src,
                                     "algorithmic_bytes_per_launch": q_bytes,
                                     "bytes_rule": "12 + 8x8 per query inside the bounds, 12 per photon visited "
                                                   "(its position), 8 per result, 4 per count"},
@@ -648,6 +651,7 @@ def run_c3(args):
                                                "the frame's %d photons and answering %d sampled queries "
                                                "(one Python call per query)" % (n_ph, len(sample))},
     }
+    res["roofline"] = res["grid"]["query_roofline"]
     if not args.no_cpu:
         lr, ls = N.DeviceBuffer(pm.P * 32), N.DeviceBuffer(pm.P * 32)
         N.check(N.lib.imba_ppm_light(N.ptr(pm.light_tri), N.ptr(pm.light_int), pm.light_seed, 0, pm.P, lr.ptr,

@@ -3,11 +3,12 @@
 // HashGrid (rangesearch/rangesearch.h:21-158) builds on the host with TBB: bounds reduce,
 // atomic count per bucket, serial exclusive scan, atomic scatter.  Here:
 //   bounds   k_bounds (per-block min/max) + k_params (combine, grow by 0.1%, derived values)
-//   count    k_count_buckets: bucket per photon, atomicAdd ticket per photon
-//   scan     k_scan_tiles / k_scan_top / k_scan_apply: bucket ends (inclusive scan)
-//   scatter  k_scatter: slot = bucket start + ticket
-//   order    k_order_small / k_order_large: each bucket's slots sorted by photon index, so the
-//            layout is deterministic and equals a serial run of the reference's scatter
+//   keys     k_grid_keys: bucket of each photon, paired with the photon index
+//   sort     stable LSD radix sort of the pairs by bucket (radix.h): the slot order is by
+//            bucket, then by photon index, which is deterministic and equals a serial run of
+//            the reference's scatter
+//   ends     k_mark_ends writes slot + 1 at the last slot of each bucket, an inclusive max-scan
+//            (k_max_tiles / k_max_top / k_max_apply) carries it over the empty buckets
 //   gather   positions copied to slot order as float4 (x, y, z, index bits)
 // The query visits the same 8 buckets in the same order as the reference (duplicates
 // included) and keeps a k-capped, distance-sorted list in registers with the reference's
@@ -18,6 +19,7 @@
 #include <cfloat>
 
 #include "common.h"
+#include "radix.h"
 
 namespace imba {
 
@@ -36,17 +38,18 @@ struct ImbaHashGrid {
     int count = 0;
     int cap = 0;         // photon capacity of the per-photon arrays
     int* cell_ends = nullptr;
-    int* counts = nullptr;
     int* order = nullptr;
-    int* bucket = nullptr;
-    int* ticket = nullptr;
+    int* keys_a = nullptr;     // sort buffers (photon_count entries each)
+    int* vals_a = nullptr;
+    int* keys_b = nullptr;
+    int* vals_b = nullptr;
     float4* pos4 = nullptr;
     imba::GridParams* d_params = nullptr;
     float* partial = nullptr;  // 6 per bounds block
-    int* scan_tmp = nullptr;   // tile sums
+    int* scan_tmp = nullptr;   // tile maxima of the ends scan
     int scan_cap = 0;
-    int* large_list = nullptr; // buckets too large for one thread, then their count at [table]
-    int large_cap = 0;
+    int* digits = nullptr;     // radix digit counts
+    int digits_cap = 0;
 };
 
 namespace imba {
@@ -56,8 +59,7 @@ constexpr int kBoundsBlocks = 1024;
 constexpr int kBlock = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
-constexpr int kSmallBucket = 32;
-constexpr int kLdsSort = 4096;
+constexpr int kSortItems = 16;  // radix tiles of 4096 pairs
 
 __device__ __forceinline__ const float* photon_pos(const unsigned char* base, int stride, int off, int i) {
     return reinterpret_cast<const float*>(base + (size_t)i * stride + off);
@@ -90,15 +92,33 @@ __global__ __launch_bounds__(kBlock) void k_bounds(const unsigned char* ph, int 
     if (threadIdx.x < 6) partial[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ void k_params(const float* partial, int nparts, GridParams* gp, float radius, int table_size, int count) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One block: the per-block bounds reduced to the photon bounds (min / max are exact in any
+// order), then the grid parameters.
+__global__ __launch_bounds__(kBlock) void k_params(const float* partial, int nparts, GridParams* gp, float radius,
+                                                   int table_size, int count) {
     float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-    for (int b = 0; b < nparts; b++)
+    for (int b = threadIdx.x; b < nparts; b += kBlock)
         for (int k = 0; k < 3; k++) {
             const float a = partial[6 * b + k], c = partial[6 * b + 3 + k];
             lo[k] = lo[k] < a ? lo[k] : a;
             hi[k] = hi[k] > c ? hi[k] : c;
         }
+    __shared__ float red[6][kBlock];
+    for (int k = 0; k < 3; k++) { red[k][threadIdx.x] = lo[k]; red[3 + k][threadIdx.x] = hi[k]; }
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            for (int k = 0; k < 3; k++) {
+                const float a = red[k][threadIdx.x], b = red[k][threadIdx.x + s];
+                red[k][threadIdx.x] = a < b ? a : b;
+                const float c = red[3 + k][threadIdx.x], d = red[3 + k][threadIdx.x + s];
+                red[3 + k][threadIdx.x] = c > d ? c : d;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < 3; k++) { lo[k] = red[k][0]; hi[k] = red[3 + k][0]; }
     for (int k = 0; k < 3; k++) {
         const float e = hi[k] - lo[k];
         gp->bmax[k] = hi[k] + e * 0.001f;
@@ -121,8 +141,8 @@ __device__ __forceinline__ int bucket_of(uint32_t x, uint32_t y, uint32_t z, int
     return (int)(((x * 73856093u) ^ (y * 19349663u) ^ (z * 83492791u)) % (uint32_t)table);
 }
 
-__global__ __launch_bounds__(kBlock) void k_count_buckets(const unsigned char* ph, int stride, int off, int n,
-                                                          const GridParams* gp, int* bucket, int* ticket, int* counts) {
+__global__ __launch_bounds__(kBlock) void k_grid_keys(const unsigned char* ph, int stride, int off, int n,
+                                                      const GridParams* gp, int* keys, int* vals) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const float* p = photon_pos(ph, stride, off, i);
@@ -130,155 +150,84 @@ __global__ __launch_bounds__(kBlock) void k_count_buckets(const unsigned char* p
     const int cx = (int)floorf(inv * (p[0] - gp->bmin[0]));
     const int cy = (int)floorf(inv * (p[1] - gp->bmin[1]));
     const int cz = (int)floorf(inv * (p[2] - gp->bmin[2]));
-    const int b = bucket_of((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, gp->table_size);
-    bucket[i] = b;
-    ticket[i] = atomicAdd(&counts[b], 1);
+    keys[i] = bucket_of((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, gp->table_size);
+    vals[i] = i;
 }
 
-// Inclusive scan of `n` ints: per-tile sums, a one-block scan of the sums, then each tile is
-// scanned again with its offset.
-__global__ __launch_bounds__(kBlock) void k_scan_tiles(const int* in, int n, int* tile_sums) {
+// ends[b] = 1 + the last slot of bucket b, for the buckets that hold photons (the others
+// were zeroed).
+__global__ __launch_bounds__(kBlock) void k_mark_ends(const int* keys, int n, int* ends) {
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const int b = keys[s];
+    if (s == n - 1 || keys[s + 1] != b) ends[b] = s + 1;
+}
+
+// Inclusive max-scan of `n` ints in place: per-tile maxima, a one-block exclusive max-scan of
+// the maxima, then each tile is scanned again from its offset.  Values are >= 0.
+__global__ __launch_bounds__(kBlock) void k_max_tiles(const int* in, int n, int* tile_max) {
     __shared__ int ws[kBlock / 64];
     const int base = blockIdx.x * kScanTile;
-    int s = 0;
+    int m = 0;
     for (int it = 0; it < kScanItems; it++) {
         const int i = base + it * kBlock + threadIdx.x;
-        if (i < n) s += in[i];
+        if (i < n) m = max(m, in[i]);
     }
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_down(m, o, 64));
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
         int t = 0;
-        for (int w = 0; w < kBlock / 64; w++) t += ws[w];
-        tile_sums[blockIdx.x] = t;
+        for (int w = 0; w < kBlock / 64; w++) t = max(t, ws[w]);
+        tile_max[blockIdx.x] = t;
     }
 }
 
-__global__ __launch_bounds__(1024) void k_scan_top(int* sums, int nt) {
+__global__ __launch_bounds__(1024) void k_max_top(int* v, int nt) {
     __shared__ int part[1024];
     const int t = threadIdx.x;
     const int per = (nt + 1023) / 1024;
-    const int b0 = t * per, b1 = min(nt, b0 + per);
-    int s = 0;
-    for (int b = b0; b < b1; b++) s += sums[b];
-    part[t] = s;
+    const int b0 = min(nt, t * per), b1 = min(nt, b0 + per);
+    int m = 0;
+    for (int b = b0; b < b1; b++) m = max(m, v[b]);
+    part[t] = m;
     __syncthreads();
     for (int d = 1; d < 1024; d <<= 1) {
-        const int v = t >= d ? part[t - d] : 0;
+        const int x = t >= d ? part[t - d] : 0;
         __syncthreads();
-        part[t] += v;
+        part[t] = max(part[t], x);
         __syncthreads();
     }
-    int run = part[t] - s;
+    int run = t > 0 ? part[t - 1] : 0;
     for (int b = b0; b < b1; b++) {
-        const int c = sums[b];
-        sums[b] = run;
-        run += c;
+        const int c = v[b];
+        v[b] = run;
+        run = max(run, c);
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_apply(const int* in, int n, const int* tile_offsets, int* out) {
-    __shared__ int wsum[kBlock / 64];
+__global__ __launch_bounds__(kBlock) void k_max_apply(int* v, int n, const int* tile_offsets) {
+    __shared__ int wmax[kBlock / 64];
     const int base = blockIdx.x * kScanTile;
     int run = tile_offsets[blockIdx.x];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int it = 0; it < kScanItems; it++) {
         const int i = base + it * kBlock + threadIdx.x;
-        const int v = i < n ? in[i] : 0;
-        // inclusive wave scan
-        int x = v;
+        int x = i < n ? v[i] : 0;
         for (int o = 1; o < 64; o <<= 1) {
             const int y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
+            if (lane >= o) x = max(x, y);
         }
-        if (lane == 63) wsum[wave] = x;
+        if (lane == 63) wmax[wave] = x;
         __syncthreads();
-        int woff = 0, total = 0;
+        int wm = 0, total = 0;
         for (int w = 0; w < kBlock / 64; w++) {
-            woff += w < wave ? wsum[w] : 0;
-            total += wsum[w];
+            if (w < wave) wm = max(wm, wmax[w]);
+            total = max(total, wmax[w]);
         }
-        if (i < n) out[i] = run + woff + x;
-        run += total;
+        if (i < n) v[i] = max(run, max(wm, x));
+        run = max(run, total);
         __syncthreads();
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_scatter(int n, const int* bucket, const int* ticket, const int* counts,
-                                                    const int* ends, int* order) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const int b = bucket[i];
-    order[ends[b] - counts[b] + ticket[i]] = i;
-}
-
-// Buckets of up to kSmallBucket photons are ordered by one thread (insertion sort in place);
-// larger ones are listed for the block-wide pass.
-__global__ __launch_bounds__(kBlock) void k_order_small(int table, const int* counts, const int* ends, int* order,
-                                                        int* large_list, int* large_count) {
-    const int b = blockIdx.x * kBlock + threadIdx.x;
-    if (b >= table) return;
-    const int c = counts[b];
-    if (c <= 1) return;
-    if (c > kSmallBucket) {
-        large_list[atomicAdd(large_count, 1)] = b;
-        return;
-    }
-    int* seg = order + (ends[b] - c);
-    for (int i = 1; i < c; i++) {
-        const int x = seg[i];
-        int j = i - 1;
-        while (j >= 0 && seg[j] > x) {
-            seg[j + 1] = seg[j];
-            j--;
-        }
-        seg[j + 1] = x;
-    }
-}
-
-// One block per large bucket: bitonic sort in LDS when it fits, otherwise each element's
-// final position is its rank (number of smaller indices; indices are distinct).
-__global__ __launch_bounds__(1024) void k_order_large(const int* large_list, const int* large_count, const int* counts,
-                                                      const int* ends, int* order, int* tmp) {
-    __shared__ int buf[kLdsSort];
-    const int nl = *large_count;
-    for (int li = blockIdx.x; li < nl; li += gridDim.x) {
-        const int b = large_list[li];
-        const int c = counts[b];
-        int* seg = order + (ends[b] - c);
-        if (c <= kLdsSort) {
-            int p2 = 1;
-            while (p2 < c) p2 <<= 1;
-            for (int i = threadIdx.x; i < p2; i += blockDim.x) buf[i] = i < c ? seg[i] : INT32_MAX;
-            __syncthreads();
-            for (int k = 2; k <= p2; k <<= 1) {
-                for (int j = k >> 1; j > 0; j >>= 1) {
-                    for (int i = threadIdx.x; i < p2; i += blockDim.x) {
-                        const int l = i ^ j;
-                        if (l > i) {
-                            const int x = buf[i], y = buf[l];
-                            const bool up = (i & k) == 0;
-                            if ((x > y) == up) { buf[i] = y; buf[l] = x; }
-                        }
-                    }
-                    __syncthreads();
-                }
-            }
-            for (int i = threadIdx.x; i < c; i += blockDim.x) seg[i] = buf[i];
-            __syncthreads();
-        } else {
-            int* dst = tmp + (ends[b] - c);
-            for (int i = threadIdx.x; i < c; i += blockDim.x) {
-                const int x = seg[i];
-                int rank = 0;
-                for (int j = 0; j < c; j++) rank += seg[j] < x;
-                dst[rank] = x;
-            }
-            __syncthreads();
-            for (int i = threadIdx.x; i < c; i += blockDim.x) seg[i] = dst[i];
-            __syncthreads();
-        }
     }
 }
 
@@ -513,8 +462,8 @@ int imba_hashgrid_create(ImbaHashGrid** out) {
 
 int imba_hashgrid_release(ImbaHashGrid* g) {
     if (!g) return IMBA_OK;
-    void* ptrs[] = {g->cell_ends, g->counts, g->order, g->bucket, g->ticket, g->pos4, g->d_params, g->partial,
-                    g->scan_tmp, g->large_list};
+    void* ptrs[] = {g->cell_ends, g->order, g->keys_a, g->vals_a, g->keys_b, g->vals_b, g->pos4, g->d_params,
+                    g->partial, g->scan_tmp, g->digits};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete g;
@@ -533,21 +482,19 @@ int imba_hashgrid_build(ImbaHashGrid* g, const void* photons, int stride, int po
     hipStream_t s = resolve_stream(stream, &sync);
     // Bucket table: max(previous size, 2 * count) (rangesearch.h:33-34).
     const int table = std::max(g->table_size, 2 * count);
-    int tcap = g->table_size;
     if (table > g->table_size) {
-        int c1 = 0, c2 = 0;
-        int r;
-        if ((r = ensure(g->cell_ends, c1, table))) return r;
-        if ((r = ensure(g->counts, c2, table))) return r;
+        int c1 = 0;
+        int r = ensure(g->cell_ends, c1, table);
+        if (r) return r;
         g->table_size = table;
-        tcap = table;
     }
-    (void)tcap;
     if (count > g->cap) {
         int c = 0, r;
         if ((r = ensure(g->order, c, count))) return r;
-        c = 0; if ((r = ensure(g->bucket, c, count))) return r;
-        c = 0; if ((r = ensure(g->ticket, c, count))) return r;
+        c = 0; if ((r = ensure(g->keys_a, c, count))) return r;
+        c = 0; if ((r = ensure(g->vals_a, c, count))) return r;
+        c = 0; if ((r = ensure(g->keys_b, c, count))) return r;
+        c = 0; if ((r = ensure(g->vals_b, c, count))) return r;
         c = 0; if ((r = ensure(g->pos4, c, count))) return r;
         g->cap = count;
     }
@@ -556,40 +503,38 @@ int imba_hashgrid_build(ImbaHashGrid* g, const void* photons, int stride, int po
         int r = ensure(g->scan_tmp, g->scan_cap, ntiles);
         if (r) return r;
     }
-    if (table + 1 > g->large_cap) {
-        int r = ensure(g->large_list, g->large_cap, table + 1);
+    const int ndigits = (int)radix_counts_size<kSortItems>(std::max(count, 1));
+    if (ndigits > g->digits_cap) {
+        int r = ensure(g->digits, g->digits_cap, ndigits);
         if (r) return r;
     }
     g->count = count;
     const unsigned char* ph = (const unsigned char*)photons;
     const int nparts = count > 0 ? std::min(kBoundsBlocks, ceil_div(count, kBlock)) : 0;
     if (count > 0) hipLaunchKernelGGL(k_bounds, dim3(nparts), dim3(kBlock), 0, s, ph, stride, pos_offset, count, g->partial);
-    hipLaunchKernelGGL(k_params, dim3(1), dim3(1), 0, s, g->partial, nparts, g->d_params, radius, table, count);
-    if (table > 0) {
-        hipLaunchKernelGGL(k_zero, dim3(std::min(ceil_div(table, kBlock), 4096)), dim3(kBlock), 0, s, g->counts, table);
-    }
+    hipLaunchKernelGGL(k_params, dim3(1), dim3(kBlock), 0, s, g->partial, nparts, g->d_params, radius, table, count);
+    if (table > 0)
+        hipLaunchKernelGGL(k_zero, dim3(std::min(ceil_div(table, kBlock), 4096)), dim3(kBlock), 0, s, g->cell_ends,
+                           table);
     if (count > 0) {
-        hipLaunchKernelGGL(k_count_buckets, dim3(ceil_div(count, kBlock)), dim3(kBlock), 0, s, ph, stride, pos_offset,
-                           count, g->d_params, g->bucket, g->ticket, g->counts);
+        hipLaunchKernelGGL(k_grid_keys, dim3(ceil_div(count, kBlock)), dim3(kBlock), 0, s, ph, stride, pos_offset,
+                           count, g->d_params, g->keys_a, g->vals_a);
+        int bits = 1;
+        while (bits < 31 && (1 << bits) < table) bits++;  // buckets are < table
+        int* keys_sorted = nullptr;
+        radix_sort_pairs<kSortItems>(g->keys_a, g->vals_a, g->keys_b, g->vals_b, count, bits, g->digits, g->order,
+                                     &keys_sorted, s);
+        hipLaunchKernelGGL(k_mark_ends, dim3(ceil_div(count, kBlock)), dim3(kBlock), 0, s, keys_sorted, count,
+                           g->cell_ends);
     }
     if (table > 0) {
-        hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, s, g->counts, table, g->scan_tmp);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, g->scan_tmp, ntiles);
-        hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(kBlock), 0, s, g->counts, table, g->scan_tmp, g->cell_ends);
+        hipLaunchKernelGGL(k_max_tiles, dim3(ntiles), dim3(kBlock), 0, s, g->cell_ends, table, g->scan_tmp);
+        hipLaunchKernelGGL(k_max_top, dim3(1), dim3(1024), 0, s, g->scan_tmp, ntiles);
+        hipLaunchKernelGGL(k_max_apply, dim3(ntiles), dim3(kBlock), 0, s, g->cell_ends, table, g->scan_tmp);
     }
-    if (count > 0) {
-        hipLaunchKernelGGL(k_scatter, dim3(ceil_div(count, kBlock)), dim3(kBlock), 0, s, count, g->bucket, g->ticket,
-                           g->counts, g->cell_ends, g->order);
-        int* large_count = g->large_list + table;
-        hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlock), 0, s, large_count, 1);
-        hipLaunchKernelGGL(k_order_small, dim3(ceil_div(table, kBlock)), dim3(kBlock), 0, s, table, g->counts,
-                           g->cell_ends, g->order, g->large_list, large_count);
-        // the rank path writes through `bucket` as a slot-ordered temporary (no longer needed)
-        hipLaunchKernelGGL(k_order_large, dim3(256), dim3(1024), 0, s, g->large_list, large_count, g->counts,
-                           g->cell_ends, g->order, g->bucket);
+    if (count > 0)
         hipLaunchKernelGGL(k_gather_pos, dim3(ceil_div(count, kBlock)), dim3(kBlock), 0, s, ph, stride, pos_offset,
                            count, g->order, g->pos4);
-    }
     IMBA_HIP_TRY(hipGetLastError());
     if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
     return IMBA_OK;
