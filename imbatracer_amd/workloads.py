@@ -328,6 +328,7 @@ class PathTracer:
         return out
 
 
+PT_STATE_DTYPE = np.dtype([("pixel", "<i4"), ("sample", "<i4"), ("rng", "<u8"), ("tp", "<f4", 3), ("bounces", "<i4")])
 PHOTON_RAY_DTYPE = np.dtype([("pos", "<f4", 3), ("length", "<i4"), ("out_dir", "<f4", 3), ("pad", "<f4")])
 PHOTON_STATE_DTYPE = np.dtype([("path", "<i4"), ("tp", "<f4", 3)])
 QUERY_DTYPE = np.dtype([("pos", "<f4", 3), ("pixel", "<i4"), ("normal", "<f4", 3), ("pad0", "<f4"),
@@ -406,6 +407,21 @@ class PhotonMapper:
         self.stream = N.Stream()
         self.photon_count = 0
         self.events = {}
+        # with keep_queues, every bounce's queue (rays, states) is read back before it is
+        # traversed, into kept["light"] / kept["camera"] (tests)
+        self.keep_queues = False
+        self.kept = {"light": [], "camera": []}
+
+    def _keep(self, which: str, cur: int, count_index: int):
+        if not self.keep_queues:
+            return
+        c = self.counts.download(np.int32, 1, self.stream, offset=4 * count_index)
+        self.stream.sync()
+        n = int(c[0])
+        rays = self.rays[cur].download(N.RAY_DTYPE, n, self.stream)
+        states = self.states[cur].download(PT_STATE_DTYPE, n, self.stream)
+        self.stream.sync()
+        self.kept[which].append((rays, states))
 
     def _cnt(self, j: int):
         return self.counts.at(4 * j)
@@ -427,8 +443,10 @@ class PhotonMapper:
         N.check(N.lib.imba_ppm_light(N.ptr(self.light_tri), N.ptr(self.light_int), self.light_seed, 0, P,
                                      self.rays[0].ptr, self.states[0].ptr, st), "ppm_light")
         cur = 0
+        self.kept["light"] = []
         for b in range(L):
             cnt = self._cnt(b)
+            self._keep("light", cur, b)
             N.check(N.lib.imba_intersect_dev(self.gpu_scene.h, self.rays[cur].ptr, self.hits.ptr, P, cnt, self._cfg(),
                                              st), "intersect_dev")
             N.check(N.lib.imba_ppm_light_bounce(self.rays[cur].ptr, self.hits.ptr, self.states[cur].ptr, P, cnt,
@@ -462,8 +480,10 @@ class PhotonMapper:
         N.check(N.lib.imba_ppm_camera(N.ptr(self.r2w), N.ptr(self.cam_pos), self.w, self.h, self.spp, self.seed_base,
                                       0, n, self.rays[0].ptr, self.states[0].ptr, self.queries.ptr, st), "ppm_camera")
         cur = 0
+        self.kept["camera"] = []
         for b in range(L):
             cnt = self._cnt(L + 1 + b)
+            self._keep("camera", cur, L + 1 + b)
             N.check(N.lib.imba_intersect_dev(self.gpu_scene.h, self.rays[cur].ptr, self.hits.ptr, n, cnt, self._cfg(),
                                              st), "intersect_dev")
             N.check(N.lib.imba_ppm_camera_bounce(self.rays[cur].ptr, self.hits.ptr, self.states[cur].ptr, n, cnt,
