@@ -596,15 +596,17 @@ def run_c3(args):
     cpu_build_s = time.perf_counter() - tb
     valid = np.nonzero(q["pixel"] >= 0)[0]
     rng = np.random.default_rng(3)
-    sample = rng.choice(valid, size=min(len(valid), args.c3_check), replace=False)
+    sample = np.sort(rng.choice(valid, size=min(len(valid), args.c3_check), replace=False))
+    qs = np.ascontiguousarray(qpos[sample])
     tq = time.perf_counter()
-    bad = 0
-    for j in sample:
-        oi, od = og.query(qpos[j], pm.k)
-        if cnt[j] != len(oi) or not np.array_equal(idx[j, :cnt[j]], oi) or \
-                not np.array_equal(dist[j, :cnt[j]].view(np.int32), od.view(np.int32)):
-            bad += 1
+    o_idx, o_dist, o_cnt = og.query_many(qs, pm.k)
     cpu_query_s = time.perf_counter() - tq
+    bad = 0
+    for r, j in enumerate(sample):
+        c = cnt[j]
+        if c != o_cnt[r] or not np.array_equal(idx[j, :c], o_idx[r, :c]) or \
+                not np.array_equal(dist[j, :c].view(np.int32), o_dist[r, :c].view(np.int32)):
+            bad += 1
     if bad:
         raise RuntimeError("%d of %d sampled photon queries differ from the oracle grid" % (bad, len(sample)))
     info = pm.grid.info()
@@ -649,7 +651,7 @@ def run_c3(args):
         "cpu_grid": {"build_ms": cpu_build_s * 1e3, "mqueries_s": len(sample) / cpu_query_s / 1e6, "cores": 1,
                      "kind": "port", "sample": "oracle/hashgrid.c (serial restatement of HashGrid) building over "
                                                "the frame's %d photons and answering %d sampled queries "
-                                               "(one Python call per query)" % (n_ph, len(sample))},
+                                               "(k = 10, one C loop)" % (n_ph, len(sample))},
     }
     res["roofline"] = res["grid"]["query_roofline"]
     if not args.no_cpu:

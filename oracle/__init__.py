@@ -78,6 +78,8 @@ def lib():
         L.oracle_grid_build.restype = None
         L.oracle_grid_query.argtypes = [C.POINTER(OracleGrid), vp, i32, vp, vp]
         L.oracle_grid_query.restype = i32
+        L.oracle_grid_query_many.argtypes = [C.POINTER(OracleGrid), vp, i32, i32, i32, vp, vp, vp]
+        L.oracle_grid_query_many.restype = None
         L.oracle_grid_free.argtypes = [C.POINTER(OracleGrid)]
         L.oracle_grid_free.restype = None
         _lib = L
@@ -210,6 +212,17 @@ class Grid:
         dist = np.empty(max(k, 1), np.float32)
         n = lib().oracle_grid_query(C.byref(self.g), _p(q), k, _p(idx), _p(dist))
         return idx[:n].copy(), dist[:n].copy()
+
+    def query_many(self, q, k: int):
+        """Queries for every row of q (n x stride floats, position first): (idx, dist, count)."""
+        q = np.ascontiguousarray(q, np.float32)
+        n = len(q)
+        idx = np.empty((n, max(k, 1)), np.int32)
+        dist = np.empty((n, max(k, 1)), np.float32)
+        cnt = np.empty(n, np.int32)
+        lib().oracle_grid_query_many(C.byref(self.g), _p(q), q.shape[1] if q.ndim == 2 else 3, n, k, _p(idx), _p(dist),
+                                     _p(cnt))
+        return idx, dist, cnt
 
     def cell_ends(self):
         return np.ctypeslib.as_array(self.g.cell_ends, (self.g.table_size,)).copy()

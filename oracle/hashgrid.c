@@ -139,3 +139,12 @@ void oracle_grid_free(OracleGrid* g) {
     free(g->positions);
     memset(g, 0, sizeof(*g));
 }
+
+/* The query above for n positions (query q at q_base + q * stride_floats), one thread; results
+ * at out_idx[q * k], out_dist[q * k] and out_count[q].  Used as the serial CPU baseline. */
+void oracle_grid_query_many(const OracleGrid* g, const float* q_base, int stride_floats, int n, int k, int* out_idx,
+                            float* out_dist, int* out_count) {
+    for (int q = 0; q < n; q++)
+        out_count[q] = oracle_grid_query(g, q_base + (size_t)q * stride_floats, k, out_idx + (size_t)q * k,
+                                         out_dist + (size_t)q * k);
+}

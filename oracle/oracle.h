@@ -69,6 +69,8 @@ typedef struct OracleGrid {
 void oracle_grid_build(OracleGrid* g, const float* positions, int stride_floats, int count, float radius);
 /* HashGrid::query; writes up to k photon indices (original numbering) in result order. */
 int oracle_grid_query(const OracleGrid* g, const float* pos, int k, int* out_idx, float* out_dist);
+void oracle_grid_query_many(const OracleGrid* g, const float* q_base, int stride_floats, int n, int k,
+                            int* out_idx, float* out_dist, int* out_count);
 void oracle_grid_free(OracleGrid* g);
 
 #ifdef __cplusplus

@@ -144,3 +144,17 @@ def test_duplicates_from_coinciding_buckets():
         z = other[2] if j & 1 else fl[2]
         probes += bucket(int(x) & M, int(y) & M, int(z) & M, 2) == b_photon
     assert len(idx) == probes >= 2 and set(idx.tolist()) == {0}
+
+
+def test_query_many_is_the_query_loop():
+    rng = np.random.default_rng(12)
+    p = rng.random((4000, 4), dtype=np.float32)
+    g = oracle.Grid()
+    g.build(p, 0.06)
+    q = rng.random((500, 4), dtype=np.float32) * 1.1 - 0.05  # some outside the bounds
+    idx, dist, cnt = g.query_many(q, 7)
+    for j in range(len(q)):
+        a, b = g.query(q[j, :3], 7)
+        assert cnt[j] == len(a)
+        assert np.array_equal(idx[j, :cnt[j]], a) and np.array_equal(dist[j, :cnt[j]], b)
+    assert (cnt == 0).any() and (cnt == 7).any()
