@@ -11,8 +11,9 @@
 //            (k_max_tiles / k_max_top / k_max_apply) carries it over the empty buckets
 //   gather   positions copied to slot order as float4 (x, y, z, index bits)
 // The query visits the same 8 buckets in the same order as the reference (duplicates
-// included) and keeps a k-capped, distance-sorted list in registers with the reference's
-// insertion rule.  Squared distances are computed without fused multiply-add, as on the host.
+// included), one query per wave, and keeps the k-capped, distance-sorted list in lanes 0..k-1
+// with the reference's insertion rule.  Squared distances are computed without fused
+// multiply-add, as on the host.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -240,58 +241,28 @@ __global__ __launch_bounds__(kBlock) void k_gather_pos(const unsigned char* ph, 
     pos4[s] = make_float4(p[0], p[1], p[2], __int_as_float(i));
 }
 
-// Insertion of a candidate that passed the radius test into the k-capped list, in the
-// reference's order (rangesearch.h:110-124): when the list is full the candidate is dropped
-// only if the current k-th distance is strictly smaller; otherwise it replaces the k-th entry
-// and moves before every strictly larger distance.  kth holds dist[k - 1] once the list is full.
-template <int KMAX>
-__device__ __forceinline__ void grid_insert(float (&dist)[KMAX], int (&idx)[KMAX], int& count, float& kth, int k,
-                                            float d2, int pid) {
-    if (count == k) {
-        if (kth < d2) return;
-    } else {
-        count++;
-    }
-    // keep the first count-1 entries, insert d2 after every entry <= d2
-    const int keep = count - 1;
-    int pos = 0;
-#pragma unroll
-    for (int l = 0; l < KMAX; l++) pos += (l < keep && dist[l] <= d2) ? 1 : 0;
-#pragma unroll
-    for (int l = KMAX - 1; l >= 1; l--) {
-        if (l <= keep && l > pos) { dist[l] = dist[l - 1]; idx[l] = idx[l - 1]; }
-    }
-#pragma unroll
-    for (int l = 0; l < KMAX; l++)
-        if (l == pos) { dist[l] = d2; idx[l] = pid; }
-    if (count == k) {
-#pragma unroll
-        for (int l = 0; l < KMAX; l++)
-            if (l == k - 1) kth = dist[l];
-    }
-}
+constexpr int kDppWaveShr1 = 0x138;  // DPP control: lane i reads lane i - 1 of the wave
 
-// One query per thread.  Bucket ranges are scanned in groups of kQueryBatch photons whose
-// positions are loaded together; a group member is inserted (in visiting order) only if its
-// squared distance is within the limit at the start of the group (the radius, or the k-th
-// distance once the list is full).  That limit only shrinks, so a member it rejects would be
-// rejected by the exact test as well: the results equal the one-at-a-time scan.
-
-template <int KMAX, int kQueryBatch>
-__global__ __launch_bounds__(kBlock) void k_query(const GridParams* gp, const int* ends, const float4* pos4,
-                                                  const unsigned char* qs, int qstride, int qoff, int nq, int k,
-                                                  int* out_idx, float* out_dist, int* out_count) {
-    const int q = blockIdx.x * kBlock + threadIdx.x;
-    if (q >= nq) return;
+// One query per wave.  The eight probed bucket ranges are concatenated in visiting order and
+// scanned 64 candidates at a time, lane l taking candidate base + l: loads within a bucket are
+// contiguous across the wave.  Candidates within the limit at the start of the chunk (the
+// radius, or the k-th distance once the list is full) are then inserted one at a time in
+// visiting order with the reference's rule; the list itself lives in lanes 0..k-1 (distance
+// and photon index per lane), so any k up to 64 fits.  The limit only shrinks, so the
+// candidates it rejects would be rejected by the exact test as well: the results are those of
+// the one-at-a-time scan.
+__global__ __launch_bounds__(kBlock) void k_query_wave(const GridParams* gp, const int* ends, const float4* pos4,
+                                                       const unsigned char* qs, int qstride, int qoff, int nq, int k,
+                                                       int* out_idx, float* out_dist, int* out_count) {
+    const int q = blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6);
+    if (q >= nq) return;  // whole waves leave together
+    const int lane = (int)(threadIdx.x & 63);
     const float* qp = photon_pos(qs, qstride, qoff, q);
     const float qx = qp[0], qy = qp[1], qz = qp[2];
-    float dist[KMAX];
-    int idx[KMAX];
-#pragma unroll
-    for (int l = 0; l < KMAX; l++) { dist[l] = 0.0f; idx[l] = -1; }
-    int count = 0;
-    float kth = 0.0f;
     const GridParams g = *gp;
+    float my_dist = 0.0f;
+    int my_idx = -1;
+    int count = 0;
     const bool inside = qx >= g.bmin[0] && qy >= g.bmin[1] && qz >= g.bmin[2] &&
                         qx <= g.bmax[0] && qy <= g.bmax[1] && qz <= g.bmax[2];
     if (inside && g.count > 0) {
@@ -303,126 +274,76 @@ __global__ __launch_bounds__(kBlock) void k_query(const GridParams* gp, const in
         const int pxo = px + ((cx - fx) < 0.5f ? -1 : 1);
         const int pyo = py + ((cy - fy) < 0.5f ? -1 : 1);
         const int pzo = pz + ((cz - fz) < 0.5f ? -1 : 1);
-        for (int j = 0; j < 8; j++) {
-            const int x = (j & 4) ? pxo : px;
-            const int y = (j & 2) ? pyo : py;
-            const int z = (j & 1) ? pzo : pz;
+        // lane j < 8: probe j's bucket range
+        int begin = 0, size = 0;
+        if (lane < 8) {
+            const int x = (lane & 4) ? pxo : px;
+            const int y = (lane & 2) ? pyo : py;
+            const int z = (lane & 1) ? pzo : pz;
             const int b = bucket_of((uint32_t)x, (uint32_t)y, (uint32_t)z, g.table_size);
-            const int begin = b == 0 ? 0 : ends[b - 1];
-            const int end = ends[b];
-            for (int s0 = begin; s0 < end; s0 += kQueryBatch) {
-                float d2[kQueryBatch];
-#pragma unroll
-                for (int u = 0; u < kQueryBatch; u++) {
-                    float px = 0.0f, py = 0.0f, pz = 0.0f;
-                    if (s0 + u < end) {
-                        const float* pp = reinterpret_cast<const float*>(pos4 + s0 + u);
-                        px = pp[0];
-                        py = pp[1];
-                        pz = pp[2];
-                    }
-                    const float dx = qx - px, dy = qy - py, dz = qz - pz;
-                    d2[u] = dx * dx + dy * dy + dz * dz;
-                }
-                const float lim = count == k ? kth : g.radius_sqr;
-                uint32_t m = 0;
-#pragma unroll
-                for (int u = 0; u < kQueryBatch; u++)
-                    if (s0 + u < end && d2[u] <= lim && d2[u] <= g.radius_sqr) m |= 1u << u;
-                while (m) {
-                    const int u = __builtin_ctz(m);
-                    m &= m - 1;
-                    float dd = d2[0];
-#pragma unroll
-                    for (int v = 1; v < kQueryBatch; v++)
-                        if (v == u) dd = d2[v];
-                    const int pid = __float_as_int(pos4[s0 + u].w);
-                    grid_insert<KMAX>(dist, idx, count, kth, k, dd, pid);
-                }
-            }
+            begin = b == 0 ? 0 : ends[b - 1];
+            size = ends[b] - begin;
         }
-    }
-#pragma unroll
-    for (int l = 0; l < KMAX; l++) {
-        if (l < count) {
-            out_idx[(size_t)q * k + l] = idx[l];
-            if (out_dist) out_dist[(size_t)q * k + l] = dist[l];
+        // first candidate number of each probe (exclusive prefix over lanes 0..7)
+        int pre = size;
+        for (int o = 1; o < 8; o <<= 1) {
+            const int y = __shfl_up(pre, o, 64);
+            if (lane >= o) pre += y;
         }
-    }
-    out_count[q] = count;
-}
-
-// Lists longer than 16: one photon at a time (the batched scan's registers do not fit).
-template <int KMAX>
-__global__ __launch_bounds__(kBlock) void k_query_wide(const GridParams* gp, const int* ends, const float4* pos4,
-                                                  const unsigned char* qs, int qstride, int qoff, int nq, int k,
-                                                  int* out_idx, float* out_dist, int* out_count) {
-    const int q = blockIdx.x * kBlock + threadIdx.x;
-    if (q >= nq) return;
-    const float* qp = photon_pos(qs, qstride, qoff, q);
-    const float qx = qp[0], qy = qp[1], qz = qp[2];
-    float dist[KMAX];
-    int idx[KMAX];
+        pre -= size;
+        int beg_j[8], pre_j[8];
 #pragma unroll
-    for (int l = 0; l < KMAX; l++) { dist[l] = 0.0f; idx[l] = -1; }
-    int count = 0;
-    const GridParams g = *gp;
-    const bool inside = qx >= g.bmin[0] && qy >= g.bmin[1] && qz >= g.bmin[2] &&
-                        qx <= g.bmax[0] && qy <= g.bmax[1] && qz <= g.bmax[2];
-    if (inside && g.count > 0) {
-        const float cx = g.inv_cell_size * (qx - g.bmin[0]);
-        const float cy = g.inv_cell_size * (qy - g.bmin[1]);
-        const float cz = g.inv_cell_size * (qz - g.bmin[2]);
-        const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
-        const int px = (int)fx, py = (int)fy, pz = (int)fz;
-        const int pxo = px + ((cx - fx) < 0.5f ? -1 : 1);
-        const int pyo = py + ((cy - fy) < 0.5f ? -1 : 1);
-        const int pzo = pz + ((cz - fz) < 0.5f ? -1 : 1);
         for (int j = 0; j < 8; j++) {
-            const int x = (j & 4) ? pxo : px;
-            const int y = (j & 2) ? pyo : py;
-            const int z = (j & 1) ? pzo : pz;
-            const int b = bucket_of((uint32_t)x, (uint32_t)y, (uint32_t)z, g.table_size);
-            const int begin = b == 0 ? 0 : ends[b - 1];
-            const int end = ends[b];
-            for (int s = begin; s < end; s++) {
-                const float4 p = pos4[s];
+            beg_j[j] = __builtin_amdgcn_readlane(begin, j);
+            pre_j[j] = __builtin_amdgcn_readlane(pre, j);
+        }
+        const int total = __builtin_amdgcn_readlane(pre + size, 7);
+        float kth = 0.0f;  // dist[k - 1] once the list is full (wave-uniform)
+        for (int base = 0; base < total; base += 64) {
+            const int c = base + lane;
+            const bool valid = c < total;
+            int slot = 0;
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                if (c >= pre_j[j]) slot = beg_j[j] + (c - pre_j[j]);
+            float d2 = 0.0f;
+            int pid = -1;
+            if (valid) {
+                const float4 p = pos4[slot];
                 const float dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
-                const float d2 = dx * dx + dy * dy + dz * dz;
-                if (!(d2 <= g.radius_sqr)) continue;
-                float last = 0.0f;
-#pragma unroll
-                for (int l = 0; l < KMAX; l++)
-                    if (l == k - 1) last = dist[l];
+                d2 = dx * dx + dy * dy + dz * dz;
+                pid = __float_as_int(p.w);
+            }
+            const float lim = count == k ? kth : g.radius_sqr;
+            unsigned long long m = __ballot(valid && d2 <= lim && d2 <= g.radius_sqr);
+            while (m) {
+                const int src = __builtin_ctzll(m);
+                m &= m - 1;
+                const float dd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d2), src));
+                const int id = __builtin_amdgcn_readlane(pid, src);
                 if (count == k) {
-                    if (last < d2) continue;
+                    if (kth < dd) continue;
                 } else {
                     count++;
                 }
-                // keep the first count-1 entries, insert d2 after every entry <= d2
+                // keep the first count-1 entries, insert dd after every entry <= dd
                 const int keep = count - 1;
-                int pos = 0;
-#pragma unroll
-                for (int l = 0; l < KMAX; l++) pos += (l < keep && dist[l] <= d2) ? 1 : 0;
-#pragma unroll
-                for (int l = KMAX - 1; l >= 1; l--) {
-                    if (l <= keep && l > pos) { dist[l] = dist[l - 1]; idx[l] = idx[l - 1]; }
-                }
-                const int pid = __float_as_int(p.w);
-#pragma unroll
-                for (int l = 0; l < KMAX; l++)
-                    if (l == pos) { dist[l] = d2; idx[l] = pid; }
+                const int pos = __popcll(__ballot(lane < keep && my_dist <= dd));
+                // the entries one lane down (wave shift right by one lane, DPP)
+                const float up_d = __int_as_float(__builtin_amdgcn_update_dpp(
+                    __float_as_int(my_dist), __float_as_int(my_dist), kDppWaveShr1, 0xF, 0xF, false));
+                const int up_i = __builtin_amdgcn_update_dpp(my_idx, my_idx, kDppWaveShr1, 0xF, 0xF, false);
+                if (lane > pos && lane <= keep) { my_dist = up_d; my_idx = up_i; }
+                if (lane == pos) { my_dist = dd; my_idx = id; }
+                if (count == k) kth = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_dist), k - 1));
             }
         }
     }
-#pragma unroll
-    for (int l = 0; l < KMAX; l++) {
-        if (l < count) {
-            out_idx[(size_t)q * k + l] = idx[l];
-            if (out_dist) out_dist[(size_t)q * k + l] = dist[l];
-        }
+    if (lane < count) {
+        out_idx[(size_t)q * k + lane] = my_idx;
+        if (out_dist) out_dist[(size_t)q * k + lane] = my_dist;
     }
-    out_count[q] = count;
+    if (lane == 0) out_count[q] = count;
 }
 
 template <typename T>
@@ -554,16 +475,8 @@ int imba_hashgrid_query(const ImbaHashGrid* g, const void* queries, int query_st
     bool sync;
     hipStream_t s = resolve_stream(stream, &sync);
     const unsigned char* qs = (const unsigned char*)queries;
-    const dim3 grid(ceil_div(nq, kBlock));
-    if (k <= 16)
-        hipLaunchKernelGGL((k_query<16, 4>), grid, dim3(kBlock), 0, s, g->d_params, g->cell_ends, g->pos4, qs, query_stride,
-                           query_offset, nq, k, out_idx, out_dist, out_count);
-    else if (k <= 32)
-        hipLaunchKernelGGL(k_query_wide<32>, grid, dim3(kBlock), 0, s, g->d_params, g->cell_ends, g->pos4, qs, query_stride,
-                           query_offset, nq, k, out_idx, out_dist, out_count);
-    else
-        hipLaunchKernelGGL(k_query_wide<64>, grid, dim3(kBlock), 0, s, g->d_params, g->cell_ends, g->pos4, qs, query_stride,
-                           query_offset, nq, k, out_idx, out_dist, out_count);
+    hipLaunchKernelGGL(k_query_wave, dim3(ceil_div(nq, kBlock / 64)), dim3(kBlock), 0, s, g->d_params, g->cell_ends,
+                       g->pos4, qs, query_stride, query_offset, nq, k, out_idx, out_dist, out_count);
     IMBA_HIP_TRY(hipGetLastError());
     if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
     return IMBA_OK;
