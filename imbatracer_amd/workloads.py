@@ -21,6 +21,8 @@ from . import native as N
 from . import scene as S
 
 STATE_BYTES = 32
+# the 32-byte path state of the ray producers (pixel / path, sample, rng, throughput, bounces)
+PT_STATE_DTYPE = np.dtype([("pixel", "<i4"), ("sample", "<i4"), ("rng", "<u8"), ("tp", "<f4", 3), ("bounces", "<i4")])
 
 C1_CAMERA = ((0.5, 0.5, -1.5), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0), 60.0)
 C1_LIGHT = np.array([0.5, 2.0, 0.5], np.float32)
@@ -217,13 +219,14 @@ class PathTracer:
     def frame(self, stats: bool = False, keep_queues: int = 0, timed: bool = False):
         """Runs one frame. Returns per-bounce ray counts, and with stats=True the per-bounce
         visit-count sums (nodes, triangles, instances); keep_queues > 0 also returns host
-        copies of the first keep_queues ray queues."""
+        copies of the first keep_queues ray queues (their states go to self.kept_states)."""
         st = self.stream.handle
         n = self.n
         N.check(N.lib.imba_pt_camera(N.ptr(self.r2w), N.ptr(self.cam_pos), self.w, self.h, self.spp, self.seed_base, 0,
                                      n, self.rays[0].ptr, self.states[0].ptr, st), "camera")
         cur = 0
         counts, visit, queues, ev = [], [], [], []
+        self.kept_states = []  # with keep_queues: the states of the kept queues
         if stats and self.counts is None:
             self.counts = N.DeviceBuffer(self.n * 12)
         for bounce in range(self.max_path_len):
@@ -232,6 +235,7 @@ class PathTracer:
             counts.append(n)
             if len(queues) < keep_queues:
                 queues.append(self.rays[cur].download(N.RAY_DTYPE, n, st))
+                self.kept_states.append(self.states[cur].download(PT_STATE_DTYPE, n, st))
             if stats:
                 N.check(N.lib.imba_trace_stats(self.gpu_scene.h, self.rays[cur].ptr, self.hits.ptr, self.counts.ptr,
                                                n, 0, C.byref(self.cfg) if self.cfg else None, st), "trace_stats")
@@ -328,7 +332,6 @@ class PathTracer:
         return out
 
 
-PT_STATE_DTYPE = np.dtype([("pixel", "<i4"), ("sample", "<i4"), ("rng", "<u8"), ("tp", "<f4", 3), ("bounces", "<i4")])
 PHOTON_RAY_DTYPE = np.dtype([("pos", "<f4", 3), ("length", "<i4"), ("out_dir", "<f4", 3), ("pad", "<f4")])
 PHOTON_STATE_DTYPE = np.dtype([("path", "<i4"), ("tp", "<f4", 3)])
 QUERY_DTYPE = np.dtype([("pos", "<f4", 3), ("pixel", "<i4"), ("normal", "<f4", 3), ("pad0", "<f4"),
