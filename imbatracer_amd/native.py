@@ -92,6 +92,15 @@ class TraceItem(C.Structure):
                 ("count_dev", C.c_void_p), ("any_hit", C.c_int)]
 
 
+class ShadeItem(C.Structure):
+    """ImbaShadeItem: one queue of a batched bounce + compaction."""
+    _fields_ = [("rays", C.c_void_p), ("hits", C.c_void_p), ("states", C.c_void_p), ("capacity", C.c_int),
+                ("count_dev", C.c_void_p), ("rays_out", C.c_void_p), ("states_out", C.c_void_p),
+                ("next_count_dev", C.c_void_p), ("tri_normals", C.c_void_p), ("tri_materials", C.c_void_p),
+                ("tri_count", C.c_int), ("albedo", C.c_void_p), ("emission", C.c_void_p),
+                ("material_count", C.c_int), ("offset", C.c_float), ("image", C.c_void_p)]
+
+
 class HashGridInfo(C.Structure):
     _fields_ = [("photon_count", C.c_int), ("table_size", C.c_int),
                 ("bbox_min", C.c_float * 3), ("bbox_max", C.c_float * 3),
@@ -196,6 +205,10 @@ _sig("imba_gen_camera_matrix", i32, vp, vp, vp, f32, i32, i32, vp)
 _sig("imba_pt_camera", i32, vp, vp, i32, i32, i32, C.c_uint32, i32, i32, vp, vp, vp)
 _sig("imba_pt_bounce", i32, vp, vp, vp, i32, vp, vp, i32, vp, vp, i32, i32, i32, f32, vp, vp, vp)
 _sig("imba_pt_bounce_dev", i32, vp, vp, vp, i32, vp, vp, vp, i32, vp, vp, i32, i32, i32, f32, vp, vp, vp)
+_sig("imba_shade_batch_create", i32, vp, i32, C.POINTER(vp))
+_sig("imba_shade_batch_update", i32, vp, vp, i32, vp)
+_sig("imba_shade_batch_launch", i32, vp, i32, i32, vp)
+_sig("imba_shade_batch_release", i32, vp)
 _sig("imba_ppm_light", i32, vp, vp, C.c_uint32, i32, i32, vp, vp, vp)
 _sig("imba_ppm_light_bounce", i32, vp, vp, vp, i32, vp, vp, vp, i32, vp, vp, i32, i32, i32, vp, vp, vp, vp)
 _sig("imba_ppm_camera", i32, vp, vp, i32, i32, i32, C.c_uint32, i32, i32, vp, vp, vp, vp)

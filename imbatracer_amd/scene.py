@@ -352,6 +352,58 @@ class TraceBatch:
             pass
 
 
+def _p(x):
+    """Device pointer of a DeviceBuffer, a c_void_p or an int (None stays None)."""
+    if x is None:
+        return None
+    if isinstance(x, DeviceBuffer):
+        return x.ptr.value
+    if isinstance(x, C.c_void_p):
+        return x.value
+    return x
+
+
+class ShadeBatch:
+    """Many queues bounced (Lambertian) and compacted together: items are N.ShadeItem or dicts
+    with its field names (DeviceBuffer / pointer values)."""
+
+    def __init__(self, items):
+        arr = self._items(items)
+        self.h = C.c_void_p()
+        check(N.lib.imba_shade_batch_create(arr, len(items), C.byref(self.h)), "imba_shade_batch_create")
+
+    def _items(self, items):
+        arr = (N.ShadeItem * max(1, len(items)))()
+        for k, it in enumerate(items):
+            if isinstance(it, N.ShadeItem):
+                arr[k] = it
+                continue
+            e = N.ShadeItem()
+            for name, _ in N.ShadeItem._fields_:
+                v = it.get(name)
+                if name in ("capacity", "tri_count", "material_count"):
+                    setattr(e, name, int(v or 0))
+                elif name == "offset":
+                    setattr(e, name, float(v or 0.0))
+                else:
+                    setattr(e, name, _p(v))
+            arr[k] = e
+        return arr
+
+    def update(self, items, stream=None):
+        arr = self._items(items)
+        check(N.lib.imba_shade_batch_update(self.h, arr, len(items), N._s(stream)), "imba_shade_batch_update")
+
+    def launch(self, max_path_len: int, rr_start: int, stream=None):
+        check(N.lib.imba_shade_batch_launch(self.h, max_path_len, rr_start, N._s(stream)), "imba_shade_batch_launch")
+
+    def __del__(self):
+        try:
+            N.lib.imba_shade_batch_release(self.h)
+        except Exception:
+            pass
+
+
 class Graph:
     """A captured sequence of library calls on one stream: `with Graph.capture(stream) as g:`."""
 

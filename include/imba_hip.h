@@ -446,6 +446,42 @@ int imba_pt_bounce_dev(const ImbaRay* rays, const ImbaHit* hits, void* states, i
                        const float* emission, int material_count, int max_path_len, int rr_start, float offset,
                        float* image, ImbaRay* rays_out, void* stream);
 
+/* ------------------------------ batched bounce + compaction -------------------------------
+ * One Lambertian bounce (imba_pt_bounce) and the stable compaction of the survivors
+ * (imba_compact_rays, state stride 32) for many queues at once, in five launches whatever the
+ * number of queues: the queues' device counts are read when the launch runs, so a bounce loop
+ * over many queues and scenes can be captured in one graph (nothing in it allocates or sets
+ * memory).  For every item the result equals imba_pt_bounce_dev + imba_compact_rays_dev on that
+ * queue alone: rays/states are bounced in place, survivors go to rays_out/states_out in queue
+ * order and their number to *next_count_dev (when non-null).  The reference's schedulers shade
+ * and compact each queue on the host (queue_scheduler.h:177-275, ray_queue.h:188-207). */
+typedef struct ImbaShadeItem {
+    ImbaRay* rays;             /* the queue (bounced in place)                                */
+    const ImbaHit* hits;
+    void* states;              /* 32-byte path-tracing states                                */
+    int capacity;              /* entries the buffers hold                                   */
+    const int* count_dev;      /* device count of the queue (NULL: capacity)                 */
+    ImbaRay* rays_out;         /* compacted survivors                                        */
+    void* states_out;
+    int* next_count_dev;       /* device count of the survivors (nullable)                   */
+    const float* tri_normals;  /* shading data of the queue's scene, as imba_pt_bounce       */
+    const int* tri_materials;
+    int tri_count;
+    const float* albedo;
+    const float* emission;
+    int material_count;
+    float offset;
+    float* image;              /* nullable                                                   */
+} ImbaShadeItem;
+
+typedef struct ImbaShadeBatch ImbaShadeBatch;
+int imba_shade_batch_create(const ImbaShadeItem* items, int count, ImbaShadeBatch** out);
+/* Replaces the items (table copied through pinned memory on `stream`; the host array may be
+ * reused when the call returns). */
+int imba_shade_batch_update(ImbaShadeBatch* b, const ImbaShadeItem* items, int count, void* stream);
+int imba_shade_batch_launch(ImbaShadeBatch* b, int max_path_len, int rr_start, void* stream);
+int imba_shade_batch_release(ImbaShadeBatch* b);
+
 /* ------------------------------ photon-mapping ray producers ------------------------------
  * The PPM mode of the VCM integrator (render/integrators/vcm.cpp, algo == ALGO_PPM) on the
  * device: light paths deposit photons that HashGrid indexes, camera paths end in merge queries
