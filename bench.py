@@ -426,10 +426,94 @@ def run_c4(args):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_trace_batch (closest hit)",
                      "algorithmic_bytes_per_launch": bytes_step, "bytes_per_ray": bytes_step / rays_step},
     }
+    if args.c4_bounces > 0:
+        res.update(run_c4_loop(args, builds, scenes, queues, cfg, rays_step))
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(builds[0], N.BUILDER_BINNED_SAH, 1, [queues[0][6]], [False],
                                            budget_s=args.cpu_seconds)
     return res
+
+
+def run_c4_loop(args, builds, scenes, queues, cfg, rays_first):
+    """The same 256 queues through a Lambertian bounce loop of args.c4_bounces bounces: per bounce
+    one batched traversal and one batched bounce + compaction of every queue, the whole loop one
+    graph launch per step; against the same loop launched eagerly per queue."""
+    from imbatracer_amd import native as N
+    from imbatracer_amd import workloads as W
+
+    shading = []
+    for b in builds:
+        normals, mats = b.tri_attrs()
+        n_mat = int(mats.max()) + 1 if len(mats) else 1
+        shading.append((normals, mats, np.full((n_mat, 3), 0.5, np.float32), np.zeros((n_mat, 3), np.float32), 1e-4))
+    loop = W.QueueLoop(scenes, shading, [(k % 8, q[1], q[3]) for k, q in enumerate(queues)],
+                       max_path_len=args.c4_bounces, rr_start=3, cfg=cfg)
+    loop.build_graph()
+    st = loop.stream
+    loop.step(graph=False)
+    eager_counts = loop.rays_per_bounce()
+    loop.step(graph=True)
+    graph_counts = loop.rays_per_bounce()
+    if not np.array_equal(eager_counts, graph_counts):
+        raise RuntimeError("graph step and eager step traverse different queues: %s vs %s" % (graph_counts,
+                                                                                               eager_counts))
+    if int(graph_counts[0]) != rays_first:
+        raise RuntimeError("the loop's first bounce does not hold the queues' rays")
+    rays_step = int(graph_counts.sum())
+    log("c4 loop: rays per bounce %s" % graph_counts.tolist())
+
+    def timed(graph):
+        for _ in range(args.warmup):
+            loop.step(graph)
+        st.sync()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            loop.step(graph)
+        st.sync()
+        return time.perf_counter() - t
+
+    wall_g = timed(True)
+    wall_e = timed(False)
+
+    # the plain path: every queue on its own, launch by launch (eager, one step)
+    L = args.c4_bounces
+    ws = N.DeviceBuffer(int(N.lib.imba_compact_workspace_size(max(q["n"] for q in loop.q))))
+    N.check(N.lib.imba_memcpy_d2d(loop.counts.ptr, loop.counts0.ptr, 4 * loop.n_counts, st.handle), "counts")
+    st.sync()
+    t = time.perf_counter()
+    for k, e in enumerate(loop.q):
+        n = e["n"]
+        if n == 0:
+            continue
+        tn, tm, nt, al, em, nm, off = loop.shade_bufs[e["scene"]]
+        N.check(N.lib.imba_memcpy_d2d(e["rays"][0].ptr, e["rays0"].ptr, 32 * n, st.handle), "rays")
+        N.check(N.lib.imba_memcpy_d2d(e["states"][0].ptr, e["states0"].ptr, 32 * n, st.handle), "states")
+        for b in range(L):
+            cur, nxt = b % 2, (b + 1) % 2
+            cnt = loop._cnt(k, b)
+            N.check(N.lib.imba_intersect_dev(scenes[e["scene"]].h, e["rays"][cur].ptr, e["hits"].ptr, n, cnt,
+                                             C.byref(cfg) if cfg else None, st.handle), "intersect_dev")
+            N.check(N.lib.imba_pt_bounce_dev(e["rays"][cur].ptr, e["hits"].ptr, e["states"][cur].ptr, n, cnt, tn.ptr,
+                                             tm.ptr, nt, al.ptr, em.ptr, nm, L, 3, off, None, e["rays"][cur].ptr,
+                                             st.handle), "bounce_dev")
+            N.check(N.lib.imba_compact_rays_dev(e["rays"][cur].ptr, e["states"][cur].ptr, 32, n, cnt,
+                                                e["rays"][nxt].ptr, e["states"][nxt].ptr, loop._cnt(k, b + 1),
+                                                ws.ptr, st.handle), "compact_dev")
+    st.sync()
+    wall_p = time.perf_counter() - t
+    if not np.array_equal(loop.rays_per_bounce(), graph_counts):
+        raise RuntimeError("per-queue launches traverse different queues than the batched loop")
+    return {
+        "bounce_loop": {"bounces": L, "rays_per_step": rays_step, "rays_per_bounce": graph_counts.tolist(),
+                        "graph": {"value": rays_step * args.steps / wall_g / 1e6,
+                                  "ms_per_step": wall_g * 1e3 / args.steps},
+                        "eager_batched": {"value": rays_step * args.steps / wall_e / 1e6,
+                                          "ms_per_step": wall_e * 1e3 / args.steps},
+                        "per_queue_launches": {"value": rays_step / wall_p / 1e6, "ms_per_step": wall_p * 1e3,
+                                               "steps": 1},
+                        "step": "one graph launch: per bounce imba_trace_batch_launch + imba_shade_batch_launch "
+                                "over all 256 queues (Lambertian, albedo 0.5, Russian roulette from bounce 3)"},
+    }
 
 
 def run_c5(args):
@@ -677,6 +761,7 @@ def main():
     ap.add_argument("--c5-max-active", type=int, default=64, help="c5: requests in flight at once (batched loop)")
     ap.add_argument("--c3-paths", type=int, default=921_600, help="c3: light paths per frame")
     ap.add_argument("--c3-check", type=int, default=20_000, help="c3: queries checked against the oracle grid")
+    ap.add_argument("--c4-bounces", type=int, default=8, help="c4: bounces of the graph-captured loop (0: off)")
     ap.add_argument("--c4-rays", type=int, default=0,
                     help="c4: rescale the 256 log-uniform queue sizes to this many rays (0: as drawn, about 37M)")
     ap.add_argument("--width", type=int, default=1280)

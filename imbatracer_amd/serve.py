@@ -9,8 +9,9 @@ of the loop:
     the short ones together in one batched launch (imba_trace_batch_update +
     imba_trace_batch_launch), so requests that arrived at different times share launches, and
     each queue long enough to fill the GPU alone in a launch of its own;
-  * each request then runs its Lambertian bounce and compaction with its queue size kept on
-    the device, and one read-back of all the requests' queue sizes ends the step.
+  * the requests' Lambertian bounces and compactions follow in one batched launch
+    (imba_shade_batch_*) with the queue sizes kept on the device, and one read-back of all the
+    requests' queue sizes ends the step.
 A request completes when its queue is empty or its paths reach the maximum length; its latency
 is completion minus arrival.  The plain per-request path (`batched=False`) runs the same
 kernels for one request at a time, in arrival order.
@@ -162,8 +163,17 @@ class Server:
         self.active: list[_Job] = []
         self.done: list[Request] = []
         self.batch = S.TraceBatch([])
+        self.shade = S.ShadeBatch([])
         self.steps = 0
         self.t0 = None
+
+    def _shade_item(self, j) -> N.ShadeItem:
+        sc = j.sc
+        return N.ShadeItem(j.rays[j.cur].value, j.hits.value, j.states[j.cur].value, j.n,
+                           self.counts_ptr(j.slot, j.bounce).value, j.rays[1 - j.cur].value,
+                           j.states[1 - j.cur].value, self.counts_ptr(j.slot, j.bounce + 1).value, sc.tri_n.ptr.value,
+                           sc.tri_m.ptr.value, sc.s.tri_count, sc.albedo.ptr.value, sc.emission.ptr.value,
+                           len(sc.s.albedo), sc.s.offset, j.image.value)
 
     def counts_ptr(self, slot: int, bounce: int):
         return self.counts.at(4 * ((self.max_path_len + 1) * slot + bounce))
@@ -212,16 +222,23 @@ class Server:
             items = [(j.sc.gpu, j.rays[j.cur], j.hits, j.n, self.counts_ptr(j.slot, j.bounce), False) for j in shared]
             self.batch.update(items, self.stream)
             self.batch.launch(self.cfg, self.stream)
+        if self.batched:
+            # every request's bounce and compaction in one batched launch
+            self.shade.update([self._shade_item(j) for j in self.active], self.stream)
+            self.shade.launch(self.max_path_len, self.rr_start, self.stream)
+        else:
+            for j in self.active:
+                sc = j.sc
+                cnt = self.counts_ptr(j.slot, j.bounce)
+                N.check(N.lib.imba_pt_bounce_dev(j.rays[j.cur], j.hits, j.states[j.cur], j.n, cnt, sc.tri_n.ptr,
+                                                 sc.tri_m.ptr, sc.s.tri_count, sc.albedo.ptr, sc.emission.ptr,
+                                                 len(sc.s.albedo), self.max_path_len, self.rr_start, sc.s.offset,
+                                                 j.image, j.rays[j.cur], st), "bounce_dev")
+                N.check(N.lib.imba_compact_rays_dev(j.rays[j.cur], j.states[j.cur], STATE_BYTES, j.n, cnt,
+                                                    j.rays[1 - j.cur], j.states[1 - j.cur],
+                                                    self.counts_ptr(j.slot, j.bounce + 1), j.workspace, st),
+                        "compact_dev")
         for j in self.active:
-            sc = j.sc
-            cnt = self.counts_ptr(j.slot, j.bounce)
-            N.check(N.lib.imba_pt_bounce_dev(j.rays[j.cur], j.hits, j.states[j.cur], j.n, cnt, sc.tri_n.ptr, sc.tri_m.ptr,
-                                             sc.s.tri_count, sc.albedo.ptr, sc.emission.ptr, len(sc.s.albedo),
-                                             self.max_path_len, self.rr_start, sc.s.offset, j.image, j.rays[j.cur], st),
-                    "bounce_dev")
-            N.check(N.lib.imba_compact_rays_dev(j.rays[j.cur], j.states[j.cur], STATE_BYTES, j.n, cnt,
-                                                j.rays[1 - j.cur], j.states[1 - j.cur],
-                                                self.counts_ptr(j.slot, j.bounce + 1), j.workspace, st), "compact_dev")
             j.bounce += 1
             j.cur = 1 - j.cur
         allc = self.counts.download(np.int32, (self.max_path_len + 1) * self.max_active, self.stream)

@@ -564,3 +564,96 @@ class PhotonMapper:
         return (self.out_idx.download(np.int32, n * k, self.stream).reshape(n, k),
                 self.out_dist.download(np.float32, n * k, self.stream).reshape(n, k),
                 self.out_count.download(np.int32, n, self.stream))
+
+
+class QueueLoop:
+    """A bounce loop over many queues at once (the batched persistent traversal of many queues
+    with a graph-captured bounce loop): per bounce one batched traversal of every queue
+    (imba_trace_batch_launch) and one batched Lambertian bounce + compaction of every queue
+    (imba_shade_batch_launch), queue sizes kept on the device.  A step restores every queue's
+    first rays and states and runs max_path_len bounces; it is captured once into a graph.
+
+    shading: per scene (tri_normals, tri_materials, albedo, emission, offset); queues: list of
+    (scene index, rays DeviceBuffer, ray count)."""
+
+    def __init__(self, scenes: list, shading: list, queues: list, max_path_len: int = 8, rr_start: int = 3,
+                 cfg: N.TraceConfig | None = None, seed: int = 0x5EED):
+        from .scene import ShadeBatch, TraceBatch
+
+        self.L, self.rr_start, self.cfg = max_path_len, rr_start, cfg
+        self.stream = N.Stream()
+        self.scenes = scenes
+        self.shade_bufs = [(N.DeviceBuffer.from_array(np.ascontiguousarray(n, np.float32)),
+                            N.DeviceBuffer.from_array(np.ascontiguousarray(m, np.int32)), len(m),
+                            N.DeviceBuffer.from_array(np.ascontiguousarray(a, np.float32)),
+                            N.DeviceBuffer.from_array(np.ascontiguousarray(e, np.float32)), len(a), float(off))
+                           for n, m, a, e, off in shading]
+        self.q = []
+        L1 = max_path_len + 1
+        self.counts = N.DeviceBuffer(4 * L1 * max(len(queues), 1))
+        counts0 = np.zeros(L1 * len(queues), np.int32)
+        rng = np.random.default_rng(seed)
+        for k, (si, rays0, n) in enumerate(queues):
+            st = np.zeros(max(n, 1), PT_STATE_DTYPE)
+            st["pixel"] = np.arange(max(n, 1))
+            st["rng"] = rng.integers(1, 2 ** 63, size=max(n, 1), dtype=np.int64).astype(np.uint64)
+            st["tp"] = 1.0
+            self.q.append({"scene": si, "n": n, "rays0": rays0, "states0": N.DeviceBuffer.from_array(st),
+                           "rays": [N.DeviceBuffer(max(n, 1) * 32), N.DeviceBuffer(max(n, 1) * 32)],
+                           "states": [N.DeviceBuffer(max(n, 1) * STATE_BYTES), N.DeviceBuffer(max(n, 1) * STATE_BYTES)],
+                           "hits": N.DeviceBuffer(max(n, 1) * 16)})
+            counts0[k * L1] = n
+        self.counts0 = N.DeviceBuffer.from_array(counts0)
+        self.n_counts = len(counts0)
+        self.trace = []
+        self.shade = []
+        for b in range(max_path_len):
+            cur, nxt = b % 2, (b + 1) % 2
+            self.trace.append(TraceBatch([(scenes[e["scene"]], e["rays"][cur], e["hits"], e["n"], self._cnt(k, b),
+                                           False) for k, e in enumerate(self.q)]))
+            items = []
+            for k, e in enumerate(self.q):
+                tn, tm, nt, al, em, nm, off = self.shade_bufs[e["scene"]]
+                items.append(N.ShadeItem(e["rays"][cur].ptr.value, e["hits"].ptr.value, e["states"][cur].ptr.value,
+                                         e["n"], self._cnt(k, b).value, e["rays"][nxt].ptr.value,
+                                         e["states"][nxt].ptr.value, self._cnt(k, b + 1).value, tn.ptr.value,
+                                         tm.ptr.value, nt, al.ptr.value, em.ptr.value, nm, off, None))
+            self.shade.append(ShadeBatch(items))
+        self.graph = None
+
+    def _cnt(self, k: int, b: int):
+        return self.counts.at(4 * (k * (self.L + 1) + b))
+
+    def _enqueue(self):
+        st = self.stream.handle
+        N.check(N.lib.imba_memcpy_d2d(self.counts.ptr, self.counts0.ptr, 4 * self.n_counts, st), "counts")
+        for e in self.q:
+            if e["n"]:
+                N.check(N.lib.imba_memcpy_d2d(e["rays"][0].ptr, e["rays0"].ptr, 32 * e["n"], st), "rays")
+                N.check(N.lib.imba_memcpy_d2d(e["states"][0].ptr, e["states0"].ptr, STATE_BYTES * e["n"], st), "states")
+        for b in range(self.L):
+            self.trace[b].launch(self.cfg, self.stream)
+            self.shade[b].launch(self.L, self.rr_start, self.stream)
+
+    def build_graph(self):
+        from .scene import Graph
+
+        self._enqueue()  # sizes spill stacks and counters before capture
+        self.stream.sync()
+        Graph.begin(self.stream)
+        try:
+            self._enqueue()
+        finally:
+            self.graph = Graph.end(self.stream)
+
+    def step(self, graph: bool = True):
+        if graph:
+            self.graph.launch(self.stream)
+        else:
+            self._enqueue()
+
+    def rays_per_bounce(self) -> np.ndarray:
+        """Rays traversed at each bounce by the last step (sum over the queues)."""
+        c = self.counts.download(np.int32, self.n_counts, self.stream)
+        self.stream.sync()
+        return c.reshape(len(self.q), self.L + 1)[:, :self.L].sum(0)
