@@ -411,6 +411,7 @@ def run_c4(args):
     wall_q, gpu_q = timed(per_queue_launches)
     rays_step = int(q_sizes.sum())
     achieved = bytes_step * args.steps / (gpu_b * 1e-3) / 1e9
+    c4_traffic, c4_traffic_src = measured_traffic("c4-256-queues-%d-rays-8-scenes" % rays_step, "k_trace_batch")
     res = {
         "metric": METRIC, "value": rays_step * args.steps / wall_b / 1e6, "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall_b * 1e3 / args.steps,
@@ -423,7 +424,8 @@ def run_c4(args):
         "per_queue_launches": {"value": rays_step * args.steps / wall_q / 1e6, "ms_per_step": wall_q * 1e3 / args.steps,
                                "gpu_ms_per_step": gpu_q / args.steps},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_trace_batch (closest hit)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": c4_traffic, "traffic_source": c4_traffic_src,
+                     "kernel": "k_trace_batch (closest hit)",
                      "algorithmic_bytes_per_launch": bytes_step, "bytes_per_ray": bytes_step / rays_step},
     }
     if args.c4_bounces > 0:
@@ -518,7 +520,8 @@ def run_c4_loop(args, builds, scenes, queues, cfg, rays_first):
 
 def run_c5(args):
     """Serving: Poisson arrivals from 16 sessions over 8 resident sphere-grid scenes (20K to 21M
-    triangles), random cameras, resolutions {256^2, 640x480, 1280x720, 1920x1080}, 1-4 spp, PT;
+    triangles), random cameras, resolutions {256^2, 640x480, 1280x720, 1920x1080}, 1-4 spp, PT or
+    photon mapping (the PPM mode of VCM);
     the request loop with cross-session batching against the plain per-request path on the
     same arrivals."""
     from imbatracer_amd import native as N
@@ -537,7 +540,9 @@ def run_c5(args):
         centres.append((side * 0.5, 0.4, side * 0.5, max(3.0, 1.6 * side)))
         tris.append(sg.tri_count)
     log("8 scenes (%s triangles) built in %.1f s" % (tris, time.time() - t0))
-    reqs = V.poisson_requests(args.c5_requests, args.c5_rate, 16, len(scenes), centres, seed=11)
+    integrators = tuple(args.c5_integrators.split(","))
+    reqs = V.poisson_requests(args.c5_requests, args.c5_rate, 16, len(scenes), centres, seed=11,
+                              integrators=integrators)
     cfg = fixed_config(args)
     arena = args.c5_arena_gb << 30
 
@@ -576,7 +581,9 @@ def run_c5(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded procedural scenes, Poisson arrivals, random cameras)",
         "config": {"workload": "c5-serving-%d-requests-%.0f-per-s-16-sessions-8-scenes" % (len(reqs), args.c5_rate),
-                   "scene_triangles": tris, "integrator": "pt", "arena_gb": args.c5_arena_gb,
+                   "scene_triangles": tris, "integrators": list(integrators),
+                   "requests_per_integrator": {i: sum(r.integrator == i for r in reqs) for i in integrators},
+                   "arena_gb": args.c5_arena_gb,
                    "max_active": args.c5_max_active, "own_launch_rays": V.Server.OWN_LAUNCH_RAYS,
                    "resolutions": ["256x256", "640x480", "1280x720", "1920x1080"], "spp": "1-4"},
         "served": b, "plain_per_request": p,
@@ -759,6 +766,8 @@ def main():
     ap.add_argument("--c5-arena-gb", type=int, default=48, help="c5: device memory for request buffers")
     ap.add_argument("--c5-small", action="store_true", help="c5: eight small scenes (quick runs)")
     ap.add_argument("--c5-max-active", type=int, default=64, help="c5: requests in flight at once (batched loop)")
+    ap.add_argument("--c5-integrators", default="pt,ppm",
+                    help="c5: integrators drawn per request (pt: path tracing, ppm: photon mapping)")
     ap.add_argument("--c3-paths", type=int, default=921_600, help="c3: light paths per frame")
     ap.add_argument("--c3-check", type=int, default=20_000, help="c3: queries checked against the oracle grid")
     ap.add_argument("--c4-bounces", type=int, default=8, help="c4: bounces of the graph-captured loop (0: off)")

@@ -81,6 +81,12 @@ class ServedScene:
         self.albedo = N.DeviceBuffer.from_array(scene.albedo)
         self.emission = N.DeviceBuffer.from_array(scene.emission)
         self.cfg = cfg
+        # photon mapping: material kinds, the emitting triangle, the merge radius
+        self.kinds = N.DeviceBuffer.from_array(np.ascontiguousarray(scene.kinds, np.int32))
+        self.light_tri = np.ascontiguousarray(scene.light_tri, np.float32).reshape(-1)
+        self.light_int = np.ascontiguousarray(scene.emission[len(scene.albedo) - 1], np.float32)
+        lo, hi = np.asarray(scene.bounds[0], np.float64), np.asarray(scene.bounds[1], np.float64)
+        self.radius = float(np.float32(0.005 * np.linalg.norm(hi - lo)))
 
 
 @dataclass
@@ -95,6 +101,7 @@ class Request:
     cam_dir: tuple
     seed: int
     arrival: float  # seconds after the start of the run
+    integrator: str = "pt"  # "pt" or "ppm" (photon mapping: the PPM mode of VCM)
     # filled in by the server
     admitted: float = -1.0
     done: float = -1.0
@@ -106,12 +113,24 @@ class Request:
         return self.width * self.height * self.spp
 
 
+PPM_DEPTH = 6   # light and camera path hits of a photon-mapping request
+PPM_K = 10      # photons per merge query
+
+
 def footprint(req: Request) -> int:
-    """Device bytes a request holds while active: two ray and state buffers, hits, image and
-    the compaction workspace."""
+    """Device bytes a request holds while active.  PT: two ray and state buffers, hits, image and
+    the compaction workspace.  PPM: the same for max(light paths, samples) entries, the photon
+    slots and the compacted photons, the merge queries and their results."""
     n = req.samples
-    ws = int(N.lib.imba_compact_workspace_size(n))
-    return 2 * 32 * n + 2 * STATE_BYTES * n + 16 * n + 12 * req.width * req.height + ws + 8 * ALIGN
+    if req.integrator == "pt":
+        ws = int(N.lib.imba_compact_workspace_size(n))
+        return 2 * 32 * n + 2 * STATE_BYTES * n + 16 * n + 12 * req.width * req.height + ws + 8 * ALIGN
+    p = req.width * req.height
+    cap = max(p, n)
+    slots = PPM_DEPTH * p
+    ws = int(N.lib.imba_compact_workspace_size(max(slots, cap)))
+    return (2 * 32 * cap + 2 * STATE_BYTES * cap + 16 * cap + 2 * 48 * slots + 64 * n + 8 * PPM_K * n + 4 * n +
+            12 * req.width * req.height + ws + 16 * ALIGN)
 
 
 class _Job:
@@ -143,6 +162,148 @@ class _Job:
                                              N.ptr(self.r2w)), "camera matrix")
 
 
+class _PPMJob:
+    """A photon-mapping request (the PPM mode of the VCM integrator, vcm.cpp) as a sequence of
+    phases advanced one per step: `light` (PPM_DEPTH bounces of width x height light paths,
+    photons into slots), `photons` (slots compacted; the count is read back), `grid` (hash grid
+    built, camera rays shot), `camera` (bounces until every path has written its merge query),
+    `merge` (k-nearest queries and merging), then finished.  The light and camera queues join
+    the step's batched traversal like PT queues.  The hash grid's own arrays live outside the
+    arena."""
+
+    def __init__(self, req: Request, slot: int, block, scene: ServedScene, arena: DeviceArena):
+        self.req, self.slot, self.block, self.sc = req, slot, block, scene
+        self.P = req.width * req.height
+        self.n = req.samples
+        self.cap = max(self.P, self.n)
+        self.L = PPM_DEPTH
+        off = block[0]
+
+        def take(nbytes):
+            nonlocal off
+            p = arena.ptr(off)
+            off += (nbytes + ALIGN - 1) // ALIGN * ALIGN
+            return p
+
+        cap, slots = self.cap, self.L * self.P
+        self.rays = [take(32 * cap), take(32 * cap)]
+        self.states = [take(STATE_BYTES * cap), take(STATE_BYTES * cap)]
+        self.hits = take(16 * cap)
+        self.slot_rays, self.slot_states = take(32 * slots), take(16 * slots)
+        self.photon_rays, self.photon_states = take(32 * slots), take(16 * slots)
+        self.queries = take(64 * self.n)
+        self.out_idx, self.out_dist, self.out_count = take(4 * PPM_K * self.n), take(4 * PPM_K * self.n), take(4 * self.n)
+        self.image = take(12 * req.width * req.height)
+        self.workspace = take(int(N.lib.imba_compact_workspace_size(max(slots, cap))))
+        self.grid = S.HashGrid()
+        self.phase = "light"
+        self.bounce = 0
+        self.cur = 0
+        self.queued = self.P
+        self.photons = 0
+        self.r2w = np.zeros(16, np.float32)
+        self.pos = np.asarray(req.cam_pos, np.float32)
+        d = np.asarray(req.cam_dir, np.float32)
+        up = np.asarray((0.0, 1.0, 0.0), np.float32)
+        N.check(N.lib.imba_gen_camera_matrix(N.ptr(self.pos), N.ptr(d), N.ptr(up), 60.0, req.width, req.height,
+                                             N.ptr(self.r2w)), "camera matrix")
+
+    # count slots of the request's row: light bounce b, camera bounce b, photon slots, photons
+    def _light(self, srv, b):
+        return srv.counts_ptr(self.slot, b)
+
+    def _camera(self, srv, b):
+        return srv.counts_ptr(self.slot, srv.L1 + b)
+
+    def start(self, srv, st):
+        c0 = np.zeros(srv.cs, np.int32)
+        c0[0] = self.P
+        c0[srv.L1] = self.n
+        c0[2 * srv.L1] = self.L * self.P
+        self.c0 = c0  # kept until the copy has run (the step ends with a stream synchronisation)
+        N.check(N.lib.imba_memcpy_h2d(srv.counts_ptr(self.slot, 0), N.ptr(c0), c0.nbytes, st), "counts")
+        N.check(N.lib.imba_memset(self.image, 0, 12 * self.req.width * self.req.height, st), "image")
+        N.check(N.lib.imba_ppm_light(N.ptr(self.sc.light_tri), N.ptr(self.sc.light_int),
+                                     (self.req.seed ^ 0x5A5A5A5A) & 0xFFFFFFFF, 0, self.P, self.rays[0],
+                                     self.states[0], st), "ppm_light")
+
+    def tracing(self) -> bool:
+        return self.phase in ("light", "camera")
+
+    def trace_item(self, srv):
+        cnt = self._light(srv, self.bounce) if self.phase == "light" else self._camera(srv, self.bounce)
+        cap = self.P if self.phase == "light" else self.n
+        return (self.sc.gpu, self.rays[self.cur], self.hits, cap, cnt, False)
+
+    def before_trace(self, srv, st):
+        sc, req = self.sc, self.req
+        if self.phase == "grid":
+            if self.photons > 0:
+                N.check(N.lib.imba_hashgrid_build(self.grid.h, self.photon_rays, 32, 0, self.photons, sc.radius, st),
+                        "grid build")
+            N.check(N.lib.imba_ppm_camera(N.ptr(self.r2w), N.ptr(self.pos), req.width, req.height, req.spp, req.seed,
+                                          0, self.n, self.rays[0], self.states[0], self.queries, st), "ppm_camera")
+            self.phase, self.bounce, self.cur = "camera", 0, 0
+        elif self.phase == "merge":
+            if self.photons > 0:
+                N.check(N.lib.imba_hashgrid_query(self.grid.h, self.queries, 64, 0, self.n, PPM_K, self.out_idx,
+                                                  self.out_dist, self.out_count, st), "grid query")
+                N.check(N.lib.imba_ppm_merge(self.queries, self.n, PPM_K, self.out_idx, self.out_dist, self.out_count,
+                                             self.photon_rays, self.photon_states, sc.radius, self.P, self.image, st),
+                        "merge")
+            self.phase = "finished"
+
+    def after_trace(self, srv, st):
+        sc, b, cur = self.sc, self.bounce, self.cur
+        if self.phase == "light":
+            cnt = self._light(srv, b)
+            N.check(N.lib.imba_ppm_light_bounce(self.rays[cur], self.hits, self.states[cur], self.P, cnt, sc.tri_n.ptr,
+                                                sc.tri_m.ptr, sc.s.tri_count, sc.albedo.ptr, sc.kinds.ptr,
+                                                len(sc.s.albedo), self.L, srv.rr_start,
+                                                C.c_void_p(self.slot_rays.value + 32 * b * self.P),
+                                                C.c_void_p(self.slot_states.value + 16 * b * self.P), self.rays[cur],
+                                                st), "ppm_light_bounce")
+            N.check(N.lib.imba_compact_rays_dev(self.rays[cur], self.states[cur], STATE_BYTES, self.P, cnt,
+                                                self.rays[1 - cur], self.states[1 - cur], self._light(srv, b + 1),
+                                                self.workspace, st), "compact")
+            self.bounce, self.cur = b + 1, 1 - cur
+            if self.bounce == self.L:
+                N.check(N.lib.imba_compact_rays_dev(self.slot_rays, self.slot_states, 16, self.L * self.P,
+                                                    srv.counts_ptr(self.slot, 2 * srv.L1), self.photon_rays,
+                                                    self.photon_states, srv.counts_ptr(self.slot, 2 * srv.L1 + 1),
+                                                    self.workspace, st), "compact photons")
+                self.phase = "photons"
+        elif self.phase == "camera":
+            cnt = self._camera(srv, b)
+            N.check(N.lib.imba_ppm_camera_bounce(self.rays[cur], self.hits, self.states[cur], self.n, cnt, sc.tri_n.ptr,
+                                                 sc.tri_m.ptr, sc.s.tri_count, sc.albedo.ptr, sc.emission.ptr,
+                                                 sc.kinds.ptr, len(sc.s.albedo), self.L, srv.rr_start, self.req.spp,
+                                                 self.queries, self.image, self.rays[cur], st), "ppm_camera_bounce")
+            N.check(N.lib.imba_compact_rays_dev(self.rays[cur], self.states[cur], STATE_BYTES, self.n, cnt,
+                                                self.rays[1 - cur], self.states[1 - cur], self._camera(srv, b + 1),
+                                                self.workspace, st), "compact")
+            self.bounce, self.cur = b + 1, 1 - cur
+
+    def update(self, srv, row) -> bool:
+        """Advances the phase from the read-back counts; True when the request is complete."""
+        if self.phase == "light":
+            self.queued = int(row[self.bounce])
+        elif self.phase == "photons":
+            self.photons = int(row[2 * srv.L1 + 1])
+            self.phase = "grid"
+        elif self.phase == "camera":
+            self.queued = int(row[srv.L1 + self.bounce])
+            if self.queued == 0 or self.bounce >= self.L:
+                self.phase = "merge"
+        elif self.phase == "finished":
+            light = [int(x) for x in row[:self.L]]
+            cam = [int(x) for x in row[srv.L1:srv.L1 + self.L]]
+            self.req.counts = light + cam
+            self.req.rays = int(sum(light) + sum(cam))
+            return True
+        return False
+
+
 class Server:
     # A queue at least this long fills the GPU by itself: it gets its own k_trace launch in a
     # batched step, and the shorter queues share one batched launch.
@@ -157,7 +318,11 @@ class Server:
         self.max_path_len, self.rr_start = max_path_len, rr_start
         self.cfg = cfg
         self.stream = N.Stream()
-        self.counts = N.DeviceBuffer(4 * (max_path_len + 1) * self.max_active)
+        # per request slot: PT bounce counts [0, L]; PPM light counts [0, L], camera counts
+        # [L1, L1 + L], photon slots at 2 L1 and photons at 2 L1 + 1
+        self.L1 = max(max_path_len, PPM_DEPTH) + 1
+        self.cs = 2 * self.L1 + 2
+        self.counts = N.DeviceBuffer(4 * self.cs * self.max_active)
         self.free_slots = list(range(self.max_active))
         self.pending: list[Request] = []
         self.active: list[_Job] = []
@@ -175,8 +340,8 @@ class Server:
                            sc.tri_m.ptr.value, sc.s.tri_count, sc.albedo.ptr.value, sc.emission.ptr.value,
                            len(sc.s.albedo), sc.s.offset, j.image.value)
 
-    def counts_ptr(self, slot: int, bounce: int):
-        return self.counts.at(4 * ((self.max_path_len + 1) * slot + bounce))
+    def counts_ptr(self, slot: int, index: int):
+        return self.counts.at(4 * (self.cs * slot + index))
 
     def now(self) -> float:
         return time.perf_counter() - self.t0
@@ -194,10 +359,15 @@ class Server:
             if block is None:
                 break  # first come first served: wait for memory
             self.pending.pop(0)
-            job = _Job(req, self.free_slots.pop(0), block, self.scenes[req.scene], self.arena)
             req.admitted = self.now()
+            if req.integrator == "ppm":
+                job = _PPMJob(req, self.free_slots.pop(0), block, self.scenes[req.scene], self.arena)
+                job.start(self, st)
+                self.active.append(job)
+                continue
+            job = _Job(req, self.free_slots.pop(0), block, self.scenes[req.scene], self.arena)
             n = job.n
-            c0 = np.zeros(self.max_path_len + 1, np.int32)
+            c0 = np.zeros(self.cs, np.int32)
             c0[0] = n
             job.c0 = c0  # kept until the copy has run (the step ends with a stream synchronisation)
             N.check(N.lib.imba_memcpy_h2d(self.counts_ptr(job.slot, 0), N.ptr(c0), c0.nbytes, st), "counts")
@@ -207,27 +377,34 @@ class Server:
             self.active.append(job)
 
     def step(self):
-        """One step: admit, one batched traversal over every active queue, bounces and
-        compactions, one read-back of the queue sizes."""
+        """One step: admit; the photon-mapping requests' non-traversal phases (grid build and
+        camera rays, or queries and merging); one batched traversal over every active queue;
+        the PT bounces and compactions in one batched launch and the photon-mapping ones; one
+        read-back of all queue sizes."""
         self._admit()
         if not self.active:
             return
         st = self.stream.handle
-        own = [j for j in self.active if not self.batched or j.queued >= self.OWN_LAUNCH_RAYS]
-        shared = [j for j in self.active if self.batched and j.queued < self.OWN_LAUNCH_RAYS]
-        for j in own:
-            N.check(N.lib.imba_intersect_dev(j.sc.gpu.h, j.rays[j.cur], j.hits, j.n, self.counts_ptr(j.slot, j.bounce),
-                                             C.byref(self.cfg) if self.cfg else None, st), "intersect_dev")
+        pt_jobs = [j for j in self.active if isinstance(j, _Job)]
+        ppm_jobs = [j for j in self.active if isinstance(j, _PPMJob)]
+        for j in ppm_jobs:
+            j.before_trace(self, st)
+        items = [(j.sc.gpu, j.rays[j.cur], j.hits, j.n, self.counts_ptr(j.slot, j.bounce), False, j.queued)
+                 for j in pt_jobs] + [j.trace_item(self) + (j.queued,) for j in ppm_jobs if j.tracing()]
+        own = [it for it in items if not self.batched or it[6] >= self.OWN_LAUNCH_RAYS]
+        shared = [it[:6] for it in items if self.batched and it[6] < self.OWN_LAUNCH_RAYS]
+        for sc, rays, hits, cap, cnt, _, _ in own:
+            N.check(N.lib.imba_intersect_dev(sc.h, rays, hits, cap, cnt, C.byref(self.cfg) if self.cfg else None, st),
+                    "intersect_dev")
         if shared:
-            items = [(j.sc.gpu, j.rays[j.cur], j.hits, j.n, self.counts_ptr(j.slot, j.bounce), False) for j in shared]
-            self.batch.update(items, self.stream)
+            self.batch.update(shared, self.stream)
             self.batch.launch(self.cfg, self.stream)
-        if self.batched:
-            # every request's bounce and compaction in one batched launch
-            self.shade.update([self._shade_item(j) for j in self.active], self.stream)
+        if self.batched and pt_jobs:
+            # every PT request's bounce and compaction in one batched launch
+            self.shade.update([self._shade_item(j) for j in pt_jobs], self.stream)
             self.shade.launch(self.max_path_len, self.rr_start, self.stream)
-        else:
-            for j in self.active:
+        elif pt_jobs:
+            for j in pt_jobs:
                 sc = j.sc
                 cnt = self.counts_ptr(j.slot, j.bounce)
                 N.check(N.lib.imba_pt_bounce_dev(j.rays[j.cur], j.hits, j.states[j.cur], j.n, cnt, sc.tri_n.ptr,
@@ -238,23 +415,30 @@ class Server:
                                                     j.rays[1 - j.cur], j.states[1 - j.cur],
                                                     self.counts_ptr(j.slot, j.bounce + 1), j.workspace, st),
                         "compact_dev")
-        for j in self.active:
+        for j in ppm_jobs:
+            if j.tracing():
+                j.after_trace(self, st)
+        for j in pt_jobs:
             j.bounce += 1
             j.cur = 1 - j.cur
-        allc = self.counts.download(np.int32, (self.max_path_len + 1) * self.max_active, self.stream)
+        allc = self.counts.download(np.int32, self.cs * self.max_active, self.stream)
         self.stream.sync()
-        allc = allc.reshape(self.max_active, self.max_path_len + 1)
+        allc = allc.reshape(self.max_active, self.cs)
         self.steps += 1
         still = []
         for j in self.active:
             c = allc[j.slot]
-            j.queued = int(c[j.bounce])
-            if c[j.bounce] == 0 or j.bounce >= self.max_path_len:
-                req = j.req
-                req.done = self.now()
-                req.counts = [int(x) for x in c[:j.bounce]]
-                req.rays = int(sum(req.counts))
-                self.done.append(req)
+            if isinstance(j, _PPMJob):
+                finished = j.update(self, c)
+            else:
+                j.queued = int(c[j.bounce])
+                finished = c[j.bounce] == 0 or j.bounce >= self.max_path_len
+                if finished:
+                    j.req.counts = [int(x) for x in c[:j.bounce]]
+                    j.req.rays = int(sum(j.req.counts))
+            if finished:
+                j.req.done = self.now()
+                self.done.append(j.req)
                 self.arena.release(*j.block)
                 self.free_slots.append(j.slot)
             else:
@@ -290,7 +474,7 @@ class Server:
                                                                         len(self.pending)))
         return (self.done, images) if keep_images else self.done
 
-    def _image_now(self, job: _Job) -> np.ndarray:
+    def _image_now(self, job) -> np.ndarray:
         out = np.empty(job.req.width * job.req.height * 3, np.float32)
         N.check(N.lib.imba_memcpy_d2h(N.ptr(out), job.image, out.nbytes, self.stream.handle), "image")
         self.stream.sync()
@@ -298,9 +482,11 @@ class Server:
 
 
 def poisson_requests(n: int, rate: float, n_sessions: int, n_scenes: int, scene_centres: list, seed: int = 7,
-                     resolutions=((256, 256), (640, 480), (1280, 720), (1920, 1080)), spp=(1, 4)) -> list:
+                     resolutions=((256, 256), (640, 480), (1280, 720), (1920, 1080)), spp=(1, 4),
+                     integrators=("pt",)) -> list:
     """n requests with Poisson arrivals at `rate` per second from n_sessions sessions, each for a
-    uniformly drawn scene, resolution and spp and a random camera looking at the scene centre."""
+    uniformly drawn scene, resolution, spp and integrator and a random camera looking at the
+    scene centre."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -312,8 +498,13 @@ def poisson_requests(n: int, rate: float, n_sessions: int, n_scenes: int, scene_
         ang = rng.uniform(0, 2 * np.pi)
         pos = (cx + radius * np.sin(ang), cy + rng.uniform(0.3, 0.9) * radius, cz - radius * np.cos(ang))
         d = (cx - pos[0], cy * 0.3 - pos[1], cz - pos[2])
-        out.append(Request(rid, int(rng.integers(0, n_sessions)), sc, w, h, int(rng.integers(spp[0], spp[1] + 1)),
-                           pos, d, int(rng.integers(1, 2 ** 31 - 1)), t))
+        req = Request(rid, int(rng.integers(0, n_sessions)), sc, w, h, int(rng.integers(spp[0], spp[1] + 1)),
+                      pos, d, int(rng.integers(1, 2 ** 31 - 1)), t)
+        if len(integrators) > 1:
+            req.integrator = integrators[int(rng.integers(0, len(integrators)))]
+        else:
+            req.integrator = integrators[0]
+        out.append(req)
     return out
 
 

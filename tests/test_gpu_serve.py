@@ -59,3 +59,35 @@ def test_admission_waits_for_memory(gpu, scenes):
     spans = sorted((r.admitted, r.done) for r in done)
     for (a0, d0), (a1, d1) in zip(spans, spans[1:]):
         assert a1 >= d0 - 1e-9 or V.footprint(reqs[0]) * 2 <= small.arena.size
+
+
+def _mixed_requests():
+    reqs = _requests()
+    for k, r in enumerate(reqs):
+        r.integrator = "ppm" if k % 2 == 0 else "pt"
+    return reqs
+
+
+def test_photon_mapping_requests_alone_and_in_a_mix(gpu, scenes):
+    """Photon-mapping (PPM) requests served beside PT requests: each request's counts and image
+    are the same alone and in the mix."""
+    mix = V.Server(scenes, 512 << 20, max_active=8, batched=True)
+    done_mix, img_mix = mix.run(_mixed_requests(), keep_images=True)
+    alone = V.Server(scenes, 512 << 20, batched=False)
+    done_alone, img_alone = alone.run(_mixed_requests(), keep_images=True)
+    assert len(done_mix) == len(done_alone) == 6
+    by_mix = {r.rid: r for r in done_mix}
+    n_ppm_lit = 0
+    for r in done_alone:
+        m = by_mix[r.rid]
+        assert m.counts == r.counts and m.rays == r.rays
+        first = r.width * r.height if r.integrator == "ppm" else r.samples
+        assert r.counts[0] == first
+        a, b = img_alone[r.rid], img_mix[r.rid]
+        assert np.allclose(a, b, rtol=1e-5, atol=1e-6)
+        lit = np.abs(a) > 1e-6
+        if lit.any():
+            assert np.mean(np.abs(a[lit] - b[lit]) / np.abs(a[lit])) <= 1e-4
+            n_ppm_lit += r.integrator == "ppm"
+    assert n_ppm_lit >= 2  # the photon-mapping images received photons
+    assert mix.arena.free == [(0, mix.arena.size)]
