@@ -35,7 +35,8 @@ struct GridParams {
 
 struct ImbaHashGrid {
     int device = 0;
-    int table_size = 0;  // grows only
+    int table_size = 0;  // grows only (until imba_hashgrid_reset)
+    int table_cap = 0;   // buckets allocated
     int count = 0;
     int cap = 0;         // photon capacity of the per-photon arrays
     int* cell_ends = nullptr;
@@ -403,12 +404,11 @@ int imba_hashgrid_build(ImbaHashGrid* g, const void* photons, int stride, int po
     hipStream_t s = resolve_stream(stream, &sync);
     // Bucket table: max(previous size, 2 * count) (rangesearch.h:33-34).
     const int table = std::max(g->table_size, 2 * count);
-    if (table > g->table_size) {
-        int c1 = 0;
-        int r = ensure(g->cell_ends, c1, table);
+    if (table > g->table_cap) {
+        int r = ensure(g->cell_ends, g->table_cap, table);
         if (r) return r;
-        g->table_size = table;
     }
+    g->table_size = table;
     if (count > g->cap) {
         int c = 0, r;
         if ((r = ensure(g->order, c, count))) return r;
@@ -479,6 +479,13 @@ int imba_hashgrid_query(const ImbaHashGrid* g, const void* queries, int query_st
                        g->pos4, qs, query_stride, query_offset, nq, k, out_idx, out_dist, out_count);
     IMBA_HIP_TRY(hipGetLastError());
     if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
+    return IMBA_OK;
+}
+
+int imba_hashgrid_reset(ImbaHashGrid* g) {
+    IMBA_CHECK_ARG(g, "grid is NULL");
+    g->table_size = 0;
+    g->count = 0;
     return IMBA_OK;
 }
 

@@ -196,6 +196,7 @@ _sig("imba_hashgrid_release", i32, vp)
 _sig("imba_hashgrid_build", i32, vp, vp, i32, i32, i32, f32, vp)
 _sig("imba_hashgrid_query", i32, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp)
 _sig("imba_hashgrid_info", i32, vp, C.POINTER(HashGridInfo))
+_sig("imba_hashgrid_reset", i32, vp)
 _sig("imba_hashgrid_gather", i32, vp, vp, i32, vp, vp)
 _sig("imba_gen_random_triangles", i32, C.c_uint64, i32, vp, vp)
 _sig("imba_gen_icosphere_counts", i32, i32, ip, ip)

@@ -462,6 +462,10 @@ class HashGrid:
                                         out_dist.ptr if out_dist is not None else None, out_count.ptr, N._s(stream)),
               "imba_hashgrid_query")
 
+    def reset(self):
+        """Forget the table-size history; allocations are kept."""
+        check(N.lib.imba_hashgrid_reset(self.h), "imba_hashgrid_reset")
+
     def info(self) -> N.HashGridInfo:
         i = N.HashGridInfo()
         check(N.lib.imba_hashgrid_info(self.h, C.byref(i)), "imba_hashgrid_info")

@@ -168,7 +168,8 @@ class _PPMJob:
     photons into slots), `photons` (slots compacted; the count is read back), `grid` (hash grid
     built, camera rays shot), `camera` (bounces until every path has written its merge query),
     `merge` (k-nearest queries and merging), then finished.  The light and camera queues join
-    the step's batched traversal like PT queues.  The hash grid's own arrays live outside the
+    the step's batched traversal like PT queues.  Hash grids come from a pool of the server's
+    (reset before use, so a request's table size is its own); their arrays live outside the
     arena."""
 
     def __init__(self, req: Request, slot: int, block, scene: ServedScene, arena: DeviceArena):
@@ -195,7 +196,7 @@ class _PPMJob:
         self.out_idx, self.out_dist, self.out_count = take(4 * PPM_K * self.n), take(4 * PPM_K * self.n), take(4 * self.n)
         self.image = take(12 * req.width * req.height)
         self.workspace = take(int(N.lib.imba_compact_workspace_size(max(slots, cap))))
-        self.grid = S.HashGrid()
+        self.grid = None  # taken from the server's pool when the photon count is known
         self.phase = "light"
         self.bounce = 0
         self.cur = 0
@@ -238,6 +239,7 @@ class _PPMJob:
     def before_trace(self, srv, st):
         sc, req = self.sc, self.req
         if self.phase == "grid":
+            self.grid = srv.take_grid()
             if self.photons > 0:
                 N.check(N.lib.imba_hashgrid_build(self.grid.h, self.photon_rays, 32, 0, self.photons, sc.radius, st),
                         "grid build")
@@ -329,6 +331,7 @@ class Server:
         self.done: list[Request] = []
         self.batch = S.TraceBatch([])
         self.shade = S.ShadeBatch([])
+        self.grids: list = []  # hash grids of finished photon-mapping requests, for reuse
         self.steps = 0
         self.t0 = None
 
@@ -339,6 +342,12 @@ class Server:
                            j.states[1 - j.cur].value, self.counts_ptr(j.slot, j.bounce + 1).value, sc.tri_n.ptr.value,
                            sc.tri_m.ptr.value, sc.s.tri_count, sc.albedo.ptr.value, sc.emission.ptr.value,
                            len(sc.s.albedo), sc.s.offset, j.image.value)
+
+    def take_grid(self):
+        """A hash grid with no table-size history (reused: no allocation once warm)."""
+        g = self.grids.pop() if self.grids else S.HashGrid()
+        g.reset()
+        return g
 
     def counts_ptr(self, slot: int, index: int):
         return self.counts.at(4 * (self.cs * slot + index))
@@ -441,6 +450,9 @@ class Server:
                 self.done.append(j.req)
                 self.arena.release(*j.block)
                 self.free_slots.append(j.slot)
+                if isinstance(j, _PPMJob) and j.grid is not None:
+                    self.grids.append(j.grid)
+                    j.grid = None
             else:
                 still.append(j)
         self.active = still

@@ -419,6 +419,11 @@ int imba_hashgrid_build(ImbaHashGrid* g, const void* photons, int stride, int po
 int imba_hashgrid_query(const ImbaHashGrid* g, const void* queries, int query_stride, int query_offset,
                         int nq, int k, int* out_idx, float* out_dist, int* out_count, void* stream);
 int imba_hashgrid_info(const ImbaHashGrid* g, ImbaHashGridInfo* info);
+/* Forgets the table-size history (the next build sizes its table as a first build would) and
+ * keeps the allocations: one grid object serves independent photon sets, e.g. one request
+ * after another, without reallocating.  No counterpart in the reference, whose grid lives as
+ * long as its integrator. */
+int imba_hashgrid_reset(ImbaHashGrid* g);
 /* Copies photon records into grid order (the reference's photons_ array): dst[slot] = src[order[slot]]. */
 int imba_hashgrid_gather(const ImbaHashGrid* g, const void* src, int stride, void* dst, void* stream);
 

@@ -124,3 +124,21 @@ def test_gather_records(gpu):
     og = oracle.Grid()
     og.build(photons[:, :3], 0.03)
     assert np.array_equal(got, photons[og.order()])
+
+
+def test_reset_forgets_the_table_history(gpu):
+    """After imba_hashgrid_reset a grid builds and answers as a fresh one (table 2N), while
+    keeping its allocations."""
+    big, small = _photons(20000, 31), _photons(3000, 32, clustered=True)
+    g, _ = _grid(big, 0.05)
+    assert g.info().table_size == 40000
+    g.reset()
+    g, pb = _grid(small, 0.04, g=g)
+    fresh, _ = _grid(small, 0.04)
+    assert g.info().table_size == fresh.info().table_size == 6000
+    q = _photons(2000, 33)
+    (ia, da, ca), (ib, db, cb) = _query(g, q, 10), _query(fresh, q, 10)
+    assert np.array_equal(ca, cb) and ca.max() > 0
+    for j in range(len(q)):
+        assert np.array_equal(ia[j, :ca[j]], ib[j, :cb[j]])
+        assert np.array_equal(da[j, :ca[j]].view(np.int32), db[j, :cb[j]].view(np.int32))
