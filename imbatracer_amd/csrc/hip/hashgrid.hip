@@ -300,17 +300,24 @@ __global__ __launch_bounds__(kBlock) void k_query_wave(const GridParams* gp, con
         }
         const int total = __builtin_amdgcn_readlane(pre + size, 7);
         float kth = 0.0f;  // dist[k - 1] once the list is full (wave-uniform)
-        for (int base = 0; base < total; base += 64) {
-            const int c = base + lane;
-            const bool valid = c < total;
+        // the photons of candidate c (x = +inf past the end, which no test accepts); the next
+        // chunk's load is issued before the current chunk's insertions
+        auto fetch = [&](int c) {
+            if (c >= total) return make_float4(__int_as_float(0x7f800000), 0.0f, 0.0f, 0.0f);
             int slot = 0;
 #pragma unroll
             for (int j = 0; j < 8; j++)
                 if (c >= pre_j[j]) slot = beg_j[j] + (c - pre_j[j]);
+            return pos4[slot];
+        };
+        float4 p_next = fetch(lane);
+        for (int base = 0; base < total; base += 64) {
+            const bool valid = base + lane < total;
+            const float4 p = p_next;
+            p_next = fetch(base + 64 + lane);
             float d2 = 0.0f;
             int pid = -1;
             if (valid) {
-                const float4 p = pos4[slot];
                 const float dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
                 d2 = dx * dx + dy * dy + dz * dz;
                 pid = __float_as_int(p.w);
