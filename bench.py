@@ -75,9 +75,10 @@ def fixed_config(args):
     """The launch configuration given on the command line, or None."""
     from imbatracer_amd import native as N
 
-    if args.block or args.persistent or args.blocks_per_cu or args.bvh_width or args.quantized:
+    if args.block or args.persistent or args.blocks_per_cu or args.bvh_width or args.quantized or args.treelet:
         return N.TraceConfig(args.block, args.stack, args.persistent, args.blocks_per_cu, args.bvh_width or
-                             (4 if args.quantized else 0), int(args.quantized))
+                             (4 if args.quantized or args.treelet else 0), int(args.quantized or args.treelet > 0),
+                             args.treelet)
     return None
 
 
@@ -85,7 +86,7 @@ def cfg_dict(cfg):
     if cfg is None:
         return "default"
     return {"block": cfg.block_size or 256, "stack_lds": cfg.stack_lds or 16, "persistent": cfg.persistent,
-            "bvh_width": cfg.bvh_width or 2, "quantized": cfg.quantized}
+            "bvh_width": cfg.bvh_width or 2, "quantized": cfg.quantized, "treelet": cfg.treelet}
 
 
 def node_bytes_of(cfg):
@@ -787,6 +788,8 @@ def main():
     ap.add_argument("--bvh-width", type=int, default=0, choices=[0, 2, 4],
                     help="tree the traversal walks: 2 = the scene's BVH2, 4 = the derived 4-wide tree (0 = default)")
     ap.add_argument("--quantized", action="store_true", help="walk the quantized 4-wide tree")
+    ap.add_argument("--treelet", type=int, default=0,
+                    help="quantized tree: nodes of its top levels each block stages in LDS (0..256)")
     ap.add_argument("--no-autotune", action="store_true", help="use the library's default launch configuration")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)

@@ -112,6 +112,8 @@ int upload_wide(const ImbaSceneDesc& host, const ImbaSceneDesc& dev, const Scene
     dev4q->nodes = (const ImbaNode*)pq;  // NULL when the boxes could not be quantized
     *info4 = info;
     info4->max_stack = w.max_stack;
+    info4->treelet_base = w.treelet_base;
+    info4->treelet_nodes = w.treelet_nodes;
     *bytes = nb + ib + qb;
     return IMBA_OK;
 }

@@ -10,6 +10,8 @@ namespace imba {
 struct SceneInfo {
     int max_stack = 0;      // deepest traversal stack any ray can need
     bool all_opaque = true; // every mask texel is opaque: the texel test can be skipped
+    int treelet_base = 0;   // 4-wide trees: first node of the range that may be staged in LDS
+    int treelet_nodes = 0;  // and its length (top levels breadth first; 0 for the BVH2)
 };
 
 int validate_scene(const ImbaSceneDesc& d, SceneInfo& info);

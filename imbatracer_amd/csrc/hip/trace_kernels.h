@@ -45,6 +45,8 @@ struct TraceArgs {
     int spill_depth;      // spill entries per thread
     int* work_counter;    // persistent variant: next ray to fetch
     const int* count_dev; // device ray count read when the launch runs (<= count), or null
+    int tl_base, tl_nodes; // quantized tree: nodes [tl_base, tl_base + tl_nodes) are read from
+                           // the block's copy in LDS (dynamic shared memory, 64 B per node)
 };
 
 // Rays of this launch: count, or the device count clamped to [0, count].
@@ -149,6 +151,7 @@ struct LaneStack {
     int spill_slot;  // global thread index
     int spill_stride;
     int sp;
+    const uint4* tl;  // LDS copy of the treelet nodes (64 B each, as in memory), or null
 
     __device__ __forceinline__ bool push(int v) {
         if (sp < STACK_LDS) {
@@ -360,8 +363,15 @@ __device__ __forceinline__ void node_step4q(const TraceArgs& a, TraceState& s, L
         s.node = kNoNode;
         return;
     }
-    const uint4* p = reinterpret_cast<const uint4*>(a.nodes) + 4 * (size_t)n;
-    const uint4 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3];
+    uint4 w0, w1, w2, w3;
+    const unsigned in_tl = (unsigned)(n - a.tl_base);
+    if (in_tl < (unsigned)a.tl_nodes) {
+        const uint4* q = st.tl + 4 * in_tl;
+        w0 = q[0]; w1 = q[1]; w2 = q[2]; w3 = q[3];
+    } else {
+        const uint4* p = reinterpret_cast<const uint4*>(a.nodes) + 4 * (size_t)n;
+        w0 = p[0]; w1 = p[1]; w2 = p[2]; w3 = p[3];
+    }
     if (STATS) s.cn++;
     const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
     const float sx = __uint_as_float((uint32_t)((int)(int8_t)(w0.w & 255u) + 127) << 23);
@@ -483,6 +493,22 @@ __device__ __forceinline__ void stack_init(LaneStack<BLOCK, STACK_LDS>& st, int*
     st.spill_slot = blockIdx.x * BLOCK + threadIdx.x;
     st.spill_stride = gridDim.x * BLOCK;
     st.sp = 0;
+    st.tl = nullptr;
+}
+
+// Copies the treelet nodes of a quantized-tree launch into the block's dynamic LDS (sized by
+// the host for a.tl_nodes nodes).  a.tl_nodes is the same for every thread of the grid.  The
+// copy keeps the node layout: a word-major copy (fewer LDS bank conflicts between lanes
+// reading different nodes) needs a runtime stride that costs the kernel 7 VGPRs and one wave
+// per SIMD, more than the conflicts cost.
+template <int BLOCK, int STACK_LDS, int TREE>
+__device__ __forceinline__ void treelet_init(const TraceArgs& a, LaneStack<BLOCK, STACK_LDS>& st) {
+    extern __shared__ uint4 treelet_lds[];
+    if (TREE != 2 || a.tl_nodes <= 0) return;
+    const uint4* src = reinterpret_cast<const uint4*>(a.nodes) + 4 * (size_t)a.tl_base;
+    for (int i = threadIdx.x; i < 4 * a.tl_nodes; i += BLOCK) treelet_lds[i] = src[i];
+    __syncthreads();
+    st.tl = treelet_lds;
 }
 
 template <bool STATS>
@@ -502,6 +528,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     __shared__ int lds_stack[STACK_LDS * BLOCK];
     LaneStack<BLOCK, STACK_LDS> st;
     stack_init(st, lds_stack, a);
+    treelet_init<BLOCK, STACK_LDS, TREE>(a, st);
     const int stride = gridDim.x * BLOCK;
     const int count = launch_count(a);
     for (int id = blockIdx.x * BLOCK + threadIdx.x; id < count; id += stride) {
@@ -524,6 +551,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dyn(TraceArgs a, int refill) {
     __shared__ int lds_stack[STACK_LDS * BLOCK];
     LaneStack<BLOCK, STACK_LDS> st;
     stack_init(st, lds_stack, a);
+    treelet_init<BLOCK, STACK_LDS, TREE>(a, st);
     const int lane = __lane_id();
     int* counter = a.work_counter;
     int id = -1;             // ray of this lane, -1 = none
@@ -616,6 +644,7 @@ __global__ __launch_bounds__(BLOCK, (ANY || MASKED || TREE != 0) ? 1 : 8) void k
     st.spill_slot = blockIdx.x * BLOCK + threadIdx.x;
     st.spill_stride = gridDim.x * BLOCK;
     st.sp = 0;
+    st.tl = nullptr;  // queues of several scenes: no treelet (their TraceArgs have tl_nodes 0)
     const int lane = __lane_id();
     const int total = __builtin_amdgcn_readfirstlane(*total_chunks);
     for (;;) {

@@ -62,11 +62,13 @@ int launch_one(const TraceArgs& args_in, int max_stack, int blocks_per_cu, int r
     const bool dyn = refill > 0;
     void (*kern)(TraceArgs) = k_trace<BLOCK, STACK_LDS, ANY, MASKED, STATS, TREE>;
     void (*kern_dyn)(TraceArgs, int) = k_trace_dyn<BLOCK, STACK_LDS, ANY, MASKED, STATS, TREE>;
+    const size_t tl_bytes = TREE == 2 ? (size_t)a.tl_nodes * 64 : 0;  // dynamic LDS: the treelet copy
+    if (TREE != 2) a.tl_nodes = 0;
     int resident = 0;
     if (dyn)
-        IMBA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern_dyn, BLOCK, 0));
+        IMBA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern_dyn, BLOCK, tl_bytes));
     else
-        IMBA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern, BLOCK, 0));
+        IMBA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern, BLOCK, tl_bytes));
     if (resident <= 0) resident = 1;
     if (blocks_per_cu > 0) resident = std::min(resident, blocks_per_cu);
     const int needed = ceil_div(a.count, BLOCK);
@@ -94,9 +96,9 @@ int launch_one(const TraceArgs& args_in, int max_stack, int blocks_per_cu, int r
         }
         int r = counter_slot(p->device & 63, &a.work_counter);
         if (r) return r;
-        hipLaunchKernelGGL(kern_dyn, dim3(grid), dim3(BLOCK), 0, s, a, std::min(refill, 64));
+        hipLaunchKernelGGL(kern_dyn, dim3(grid), dim3(BLOCK), tl_bytes, s, a, std::min(refill, 64));
     } else {
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, s, a);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), tl_bytes, s, a);
     }
     IMBA_HIP_TRY(hipGetLastError());
     return IMBA_OK;
@@ -183,6 +185,15 @@ int launch_trace(const TreeView& t, int* d_err, const ImbaRay* rays, ImbaHit* hi
     a.root = d.root;
     a.counts = counts;
     a.err = d_err;
+    if (cfg && cfg->treelet) {
+        if (cfg->treelet < 0 || cfg->treelet > kTreeletMaxNodes)
+            return fail(IMBA_ERR_INVALID_ARG, "treelet must be 0..%d nodes", kTreeletMaxNodes);
+        if (t.tree != 2) return fail(IMBA_ERR_INVALID_ARG, "treelet needs the quantized 4-wide tree");
+        if (!counts) {  // visit counts do not depend on where a node is read from
+            a.tl_base = t.info->treelet_base;
+            a.tl_nodes = std::min(cfg->treelet, t.info->treelet_nodes);
+        }
+    }
     bool sync;
     hipStream_t s = resolve_stream(stream, &sync);
     const int ms = t.info->max_stack;
@@ -630,7 +641,8 @@ int imba_autotune(const ImbaScene* sc, const ImbaRay* rays, ImbaHit* hits, int c
     IMBA_CHECK_ARG(count > 0 && reps > 0, "count and reps must be positive");
     static const int kShapes[][2] = {{256, 16}, {256, 8}, {256, 32}, {128, 16}, {128, 32}, {64, 32}, {64, 64}};
     static const int kRefill[] = {0, 32};
-    static const int kTree[][2] = {{2, 0}, {4, 0}, {4, 1}};  // bvh_width, quantized
+    // bvh_width, quantized, treelet nodes staged in LDS
+    static const int kTree[][3] = {{2, 0, 0}, {4, 0, 0}, {4, 1, 0}, {4, 1, 64}, {4, 1, 256}};
     bool sync;
     hipStream_t s = resolve_stream(stream, &sync);
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -641,8 +653,9 @@ int imba_autotune(const ImbaScene* sc, const ImbaRay* rays, ImbaHit* hits, int c
     for (const auto& tree : kTree)
     for (const auto& shape : kShapes) {
         for (int refill : kRefill) {
-            ImbaTraceConfig cfg{shape[0], shape[1], refill, 0, tree[0], tree[1]};
+            ImbaTraceConfig cfg{shape[0], shape[1], refill, 0, tree[0], tree[1], tree[2]};
             if (tree[1] && !sc->dev4q.nodes) continue;
+            if (tree[2] && sc->info4.treelet_nodes <= 0) continue;
             TreeView t{};
             r = tree_of(sc, &cfg, &t);
             if (!r)
@@ -717,14 +730,14 @@ int imba_occluded_dev(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, in
 int imba_intersect_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts, int count,
                          void* stream) {
     IMBA_CHECK_ARG(counts, "counts is NULL");
-    const ImbaTraceConfig bvh2{0, 0, 0, 0, 2, 0};
+    const ImbaTraceConfig bvh2{0, 0, 0, 0, 2, 0, 0};
     return trace_entry(s, rays, hits, counts, count, false, &bvh2, stream, nullptr);
 }
 
 int imba_occluded_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts, int count,
                         void* stream) {
     IMBA_CHECK_ARG(counts, "counts is NULL");
-    const ImbaTraceConfig bvh2{0, 0, 0, 0, 2, 0};
+    const ImbaTraceConfig bvh2{0, 0, 0, 0, 2, 0, 0};
     return trace_entry(s, rays, hits, counts, count, true, &bvh2, stream, nullptr);
 }
 

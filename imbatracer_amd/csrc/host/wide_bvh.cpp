@@ -7,13 +7,15 @@
 // the BVH2's own, so a ray meets the same triangles with the same arithmetic; only boxes of
 // the absorbed inner nodes are no longer tested, and each step tests four boxes instead of
 // two.  Node layout: ImbaNode4 (box j of child j as min/max per axis, children[j]: > 0 node,
-// < 0 leaf link as in the BVH2, 0 unused slot).  Nodes of one tree are in preorder (a child
+// < 0 leaf link as in the BVH2, 0 unused slot).  The first kTreeletMaxNodes nodes of a tree
+// are its top levels in breadth-first order, the rest follow in preorder (a child always
 // follows its parent); mesh trees come first (in increasing BVH2 root order), then the top
 // level.  Instance records are copied with `next` pointing at the mesh's 4-wide root.
 #include <algorithm>
 #include <cmath>
 #include <cfloat>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <vector>
 
@@ -41,17 +43,21 @@ void set_slot(ImbaNode4& n, int j, const ImbaBBox& b) {
 }
 
 // Collapses the BVH2 tree at root2; returns the 4-wide root index and updates *pushes with the
-// largest number of stack entries a ray can hold inside this tree.
+// largest number of stack entries a ray can hold inside this tree.  The first kTreeletMaxNodes
+// nodes are taken from the front of the work list (breadth first), the others from its back.
 int collapse_tree(const ImbaSceneDesc& d, int root2, std::vector<ImbaNode4>& out, int* pushes) {
     struct Task {
         int node2, parent4, slot, depth_pushes;
     };
-    std::vector<Task> work{{root2, -1, 0, 0}};
+    std::deque<Task> work{{root2, -1, 0, 0}};
     int root4 = -1;
+    const size_t first = out.size();
     auto inner_ok = [&](int link) { return link >= 0 && link < d.node_count; };
     while (!work.empty()) {
-        const Task t = work.back();
-        work.pop_back();
+        const bool bfs = out.size() - first < (size_t)kTreeletMaxNodes;
+        const Task t = bfs ? work.front() : work.back();
+        if (bfs) work.pop_front();
+        else work.pop_back();
         if (!inner_ok(t.node2)) return -1;
         const ImbaNode& n = d.nodes[t.node2];
         Entry list[4];
@@ -94,8 +100,13 @@ int collapse_tree(const ImbaSceneDesc& d, int root2, std::vector<ImbaNode4>& out
         else root4 = i4;
         const int here = t.depth_pushes + (k - 1);
         *pushes = std::max(*pushes, here);
-        for (int j = k - 1; j >= 0; j--)
-            if (list[j].link >= 0) work.push_back({list[j].link, i4, j, here});
+        if (out.size() - first < (size_t)kTreeletMaxNodes) {
+            for (int j = 0; j < k; j++)
+                if (list[j].link >= 0) work.push_back({list[j].link, i4, j, here});
+        } else {
+            for (int j = k - 1; j >= 0; j--)
+                if (list[j].link >= 0) work.push_back({list[j].link, i4, j, here});
+        }
     }
     return root4;
 }
@@ -138,6 +149,17 @@ int collapse_scene4(const ImbaSceneDesc& d, WideScene& out) {
     int top_pushes = 0;
     out.root = collapse_tree(d, d.root, out.nodes, &top_pushes);
     if (out.root < 0) return fail(IMBA_ERR_BAD_BVH, "top level has links out of range");
+    // the treelet: the top level, or the mesh when the top level reaches one instance only
+    int reached = 0, only = -1;
+    for (int i = 0; i < d.instance_count; i++)
+        if (used[i]) { reached++; only = i; }
+    int base = out.root, end = (int)out.nodes.size();
+    if (reached == 1) {  // the only mesh tree spans the nodes before the top level
+        base = root_map[d.instances[only].next];
+        end = out.root;
+    }
+    out.treelet_base = base;
+    out.treelet_nodes = std::min(kTreeletMaxNodes, end - base);
     out.instances.assign(d.instances, d.instances + d.instance_count);
     for (int i = 0; i < d.instance_count; i++) out.instances[i].next = used[i] ? root_map[d.instances[i].next] : -1;
     // stack entries of the top level stay while a ray is inside a mesh tree

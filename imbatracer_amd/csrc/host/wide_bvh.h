@@ -30,7 +30,15 @@ struct WideScene {
     std::vector<ImbaInstanceNode> instances;    // `next` = 4-wide root of the mesh
     int root = 0;
     int max_stack = 0;                          // stack entries a ray can need
+    // Nodes [treelet_base, treelet_base + treelet_nodes) are the first levels, breadth first,
+    // of the tree every ray enters first: the top level, or the only mesh when the top level
+    // reaches a single instance.  Kernels may stage them in LDS.
+    int treelet_base = 0;
+    int treelet_nodes = 0;
 };
+
+// Nodes of each tree laid out breadth first before the rest of the tree goes depth first.
+constexpr int kTreeletMaxNodes = 256;
 
 // `d` must have passed validate_scene (links in range, children after parents).
 int collapse_scene4(const ImbaSceneDesc& d, WideScene& out);

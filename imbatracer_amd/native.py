@@ -84,7 +84,7 @@ class MeshDesc(C.Structure):
 
 class TraceConfig(C.Structure):
     _fields_ = [("block_size", C.c_int), ("stack_lds", C.c_int), ("persistent", C.c_int), ("blocks_per_cu", C.c_int),
-                ("bvh_width", C.c_int), ("quantized", C.c_int)]
+                ("bvh_width", C.c_int), ("quantized", C.c_int), ("treelet", C.c_int)]
 
 
 class TraceItem(C.Structure):

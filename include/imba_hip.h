@@ -282,6 +282,9 @@ typedef struct ImbaTraceConfig {
                               it at upload (same leaves); 0 = default                      */
     int quantized;         /* with bvh_width 4: 64-byte nodes whose child boxes are 8-bit
                               offsets from the node box, rounded outward                  */
+    int treelet;           /* quantized tree only: nodes of its top levels (0..256) each
+                              block copies into LDS and reads from there (0 = none);
+                              single-queue launches only, batches ignore it               */
 } ImbaTraceConfig;
 
 int imba_intersect(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int count,
@@ -338,10 +341,10 @@ int imba_trace_batch_update(ImbaTraceBatch* b, const ImbaTraceItem* items, int n
 int imba_trace_batch_launch(ImbaTraceBatch* b, const ImbaTraceConfig* cfg, void* stream);
 int imba_trace_batch_release(ImbaTraceBatch* b);
 
-/* Launch shape autotuning: times every supported shape (static and persistent) on the given
- * device queue with HIP events, `reps` launches each after one warm-up, and returns the
- * fastest in *best (and its mean time per launch in *best_ms, nullable).  Results do not
- * depend on the shape. */
+/* Launch shape autotuning: times every supported shape (static and persistent) of every tree
+ * (BVH2, 4-wide, quantized with LDS treelets of 0 / 64 / 256 nodes) on the given device queue
+ * with HIP events, `reps` launches each after one warm-up, and returns the fastest in *best
+ * (and its mean time per launch in *best_ms, nullable).  Results do not depend on the shape. */
 int imba_autotune(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int count, int any_hit, int reps,
                   ImbaTraceConfig* best, float* best_ms, void* stream);
 

@@ -5,7 +5,8 @@ The 4-wide tree keeps the BVH2's leaves and leaf boxes and uses the same box and
 arithmetic, so closest hits (smallest t, then smaller triangle id, then smaller instance id)
 must equal the oracle's bit for bit.  An occlusion query may stop at another triangle than the
 BVH2's first hit; its occluded / unoccluded answer must be the same, and a reported hit must be
-a real one (t inside the ray's interval).
+a real one (t inside the ray's interval).  Staging the quantized tree's top levels in LDS
+(treelet) changes only where those nodes are read from, so its results are held to the same bar.
 """
 import numpy as np
 import pytest
@@ -18,8 +19,9 @@ from tests import scenes as T
 pytestmark = pytest.mark.gpu
 
 WIDE = N.TraceConfig(0, 0, 0, 0, 4)
-TREES = [N.TraceConfig(0, 0, 0, 0, 4, 0), N.TraceConfig(0, 0, 0, 0, 4, 1)]   # 4-wide, 4-wide quantized
-TREE_IDS = ["wide", "quantized"]
+# 4-wide, 4-wide quantized, quantized with its top 256 nodes staged in LDS
+TREES = [N.TraceConfig(0, 0, 0, 0, 4, 0), N.TraceConfig(0, 0, 0, 0, 4, 1), N.TraceConfig(0, 0, 0, 0, 4, 1, 256)]
+TREE_IDS = ["wide", "quantized", "treelet"]
 
 
 def _same(a, b):
@@ -52,14 +54,14 @@ def test_closest_and_occluded(gpu, builder, tree):
     assert sc.error_count() == 0
 
 
-@pytest.mark.parametrize("quantized", [0, 1])
+@pytest.mark.parametrize("quantized,treelet", [(0, 0), (1, 0), (1, 64), (1, 256)])
 @pytest.mark.parametrize("shape", [(256, 16, 0), (256, 8, 0), (128, 32, 0), (64, 64, 0), (256, 16, 32), (128, 32, 8)])
-def test_launch_shapes(gpu, shape, quantized):
+def test_launch_shapes(gpu, shape, quantized, treelet):
     b, _, _ = T.random_tri_scene(20000, N.BUILDER_SBVH)
     sc = S.Scene(b.desc())
     rays = T.random_rays(30000, seed=8)
     want, _ = oracle.trace(b.arrays(), rays, threads=8)
-    cfg = N.TraceConfig(shape[0], shape[1], shape[2], 0, 4, quantized)
+    cfg = N.TraceConfig(shape[0], shape[1], shape[2], 0, 4, quantized, treelet)
     for _ in range(2):
         assert _same(sc.trace_host(rays, cfg=cfg), want)
 
@@ -131,3 +133,20 @@ def test_visit_counts_and_batch(gpu):
     assert _same(hb.download(N.HIT_DTYPE, len(rays)), h2)
     nq = cq.download(np.uint32, 3 * len(rays)).reshape(-1, 3).sum(0)
     assert 0.9 * n4[0] < nq[0] < 1.5 * n4[0]
+
+
+def test_treelet_arguments(gpu):
+    """A treelet needs the quantized tree and at most 256 nodes; a larger request than the tree
+    has is clamped to the tree."""
+    v = np.array([[-1, -1, 1, 1], [1, -1, 1, 1], [0, 1, 1, 1]], np.float32)
+    b = S.SceneBuild()
+    b.add_mesh(v, np.array([[0, 1, 2, 0]], np.uint32))
+    b.run(N.BUILDER_SBVH, 1)
+    sc = S.Scene(b.desc())
+    rays = S.make_rays(np.array([[0, 0, -1]], np.float32), np.array([[0, 0, 1]], np.float32), 0.0, 1e30)
+    rb, hb = N.DeviceBuffer.from_array(rays), N.DeviceBuffer(16)
+    for cfg in (N.TraceConfig(0, 0, 0, 0, 4, 1, 257), N.TraceConfig(0, 0, 0, 0, 4, 1, -1),
+                N.TraceConfig(0, 0, 0, 0, 2, 0, 64), N.TraceConfig(0, 0, 0, 0, 4, 0, 64)):
+        assert N.lib.imba_intersect(sc.h, rb.ptr, hb.ptr, 1, N.C.byref(cfg), None) == 1  # IMBA_ERR_INVALID_ARG
+    got = sc.trace_host(rays, cfg=N.TraceConfig(0, 0, 0, 0, 4, 1, 256))
+    assert list(got["tri_id"]) == [0]

@@ -159,3 +159,20 @@ def test_quantized_boxes_contain_the_exact_boxes(name, build):
                 step = 2.0 ** int(q["exp"][axis])
                 assert n[mn][j] - lo <= step * 1.01 + 1e-6 and hi - n[mx][j] <= step * 1.01 + 1e-6
                 assert step <= max(parent / 255.0 * 2.0, 2.0 ** -100)
+
+
+@pytest.mark.parametrize("name,build", list(_scenes()), ids=lambda x: x if isinstance(x, str) else "")
+def test_top_levels_breadth_first(name, build):
+    """Each tree's first 256 nodes (the range a launch may stage in LDS) are its top levels in
+    breadth-first order, children in slot order: node root + i is the i-th node a FIFO walk
+    from the root meets."""
+    w = S.wide_tree(build.arrays())
+    nodes = w["nodes"]
+    roots = {int(w["root"])} | {int(r) for r in w["instances"]["next"] if r >= 0}
+    for root in roots:
+        order, queue = [], [root]
+        while queue and len(order) < 256:
+            k = queue.pop(0)
+            order.append(k)
+            queue += [int(c) for c in nodes[k]["children"] if c > 0]
+        assert order == list(range(root, root + len(order)))
