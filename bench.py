@@ -113,6 +113,36 @@ def autotune(scene, rays, any_hit=False):
     return cfg
 
 
+def frame_tune(pt, queues, frames=3):
+    """Per-scene launch configuration for whole frames: the library's autotuner on queues of
+    three sizes (a frame's queues shrink from bounce to bounce), then each distinct winner and
+    the plain quantized launch (with and without a 64-node treelet) timed over whole frames (sum of the traversal kernels' times);
+    the fastest wins."""
+    from imbatracer_amd import native as N
+
+    cands = [autotune(pt.gpu_scene, queues[b]) for b in sorted({1, len(queues) // 2, len(queues) - 2})
+             if 0 < b < len(queues) and len(queues[b])]
+    cands += [N.TraceConfig(256, 16, 0, 0, 4, 1, 0), N.TraceConfig(256, 16, 0, 0, 4, 1, 64)]
+    seen, best = set(), None
+    for c in cands:
+        key = tuple(getattr(c, f) for f, _ in c._fields_)
+        if key in seen:
+            continue
+        seen.add(key)
+        pt.cfg = c
+        pt.frame()
+        total = 0.0
+        for _ in range(frames):
+            pt.frame(timed=True)
+            pt.stream.sync()
+            total += sum(ms for ms, _ in pt.trace_times_ms())
+        log("frame tune: %s %.3f ms of traversal per frame" % (cfg_dict(c), total / frames))
+        if best is None or total < best[0]:
+            best = (total, c)
+    pt.cfg = best[1]
+    return best[1]
+
+
 def cpu_baseline(build, builder, threads_build, ray_sets, any_flags, budget_s=12.0):
     """Times the reference's CPU path restated (oracle/cpu_bvh4.c: BVH4 nodes + 4-triangle
     packets of core/cpu_adapter.cpp, 4-wide SSE/FMA box and triangle tests, one ray per host
@@ -167,10 +197,9 @@ def run_pt(args):
     pt = W.PathTracer(scene, args.width, args.height, spp=1, max_path_len=8, rr_start=3, trace_cfg=cfg)
     log("uploaded %.2f GB" % (pt.gpu_scene.footprint() / 1e9))
     # the queues of a frame (closest hits, hence the queues, do not depend on the configuration)
-    _, _, queues = pt.frame(keep_queues=2)
+    _, _, queues = pt.frame(keep_queues=8)
     if cfg is None and not args.no_autotune:
-        cfg = autotune(pt.gpu_scene, queues[1])
-        pt.cfg = cfg
+        cfg = frame_tune(pt, queues)
     node_bytes = node_bytes_of(cfg)
 
     # untimed: visit counts of every bounce's queue (same seeds -> same rays as the timed frames)
