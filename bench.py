@@ -98,6 +98,8 @@ def kernel_label(cfg, any_hit=False):
     suffix = ""
     if cfg is not None and cfg.bvh_width == 4:
         suffix = "_w4q" if cfg.quantized else "_w4"
+        if cfg.quantized and cfg.treelet:
+            suffix += "_tl%d" % cfg.treelet
     return ("k_trace_occluded" if any_hit else "k_trace_closest") + suffix
 
 
