@@ -386,11 +386,12 @@ def run_c4(args):
     batch = S.TraceBatch([q[:6] for q in queues])
     st = N.Stream()
     if cfg is None and not args.no_autotune:
-        # the batched launch over every queue, timed for each tree width and a few shapes
-        best = None
-        for width, quant in ((2, 0), (4, 0), (4, 1)):
+        # the batched launch over every queue, timed for each tree (quantized also with a 64-node
+        # LDS treelet per wave) and a few shapes
+        best, tried = None, []
+        for width, quant, tl in ((2, 0, 0), (4, 0, 0), (4, 1, 0), (4, 1, 64)):
             for block, stack in ((256, 16), (256, 8), (128, 32)):
-                c = N.TraceConfig(block, stack, 0, 0, width, quant)
+                c = N.TraceConfig(block, stack, 0, 0, width, quant, tl)
                 batch.launch(c, st)
                 e0, e1 = N.Event(), N.Event()
                 e0.record(st)
@@ -398,10 +399,12 @@ def run_c4(args):
                 e1.record(st)
                 st.sync()
                 ms = e0.elapsed_ms(e1)
+                tried.append("%d/%d/w%d%s%s %.2f" % (block, stack, width, "q" if quant else "",
+                                                     "/tl%d" % tl if tl else "", ms))
                 if best is None or ms < best[0]:
                     best = (ms, c)
         cfg = best[1]
-        log("batch autotune: %s, %.2f ms" % (cfg_dict(cfg), best[0]))
+        log("batch autotune: %s, %.2f ms (tried, ms: %s)" % (cfg_dict(cfg), best[0], ", ".join(tried)))
     # visit counts per queue (untimed)
     cb = N.DeviceBuffer(per_queue * 12)
     bytes_step = 0.0

@@ -631,10 +631,15 @@ __global__ void k_batch_prefix(BatchItem* items, BatchGroups groups, int* group_
     }
 }
 
-template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, int TREE>
-__global__ __launch_bounds__(BLOCK, (ANY || MASKED || TREE != 0) ? 1 : 8) void k_trace_batch(const BatchItem* __restrict__ items, int first, int n_items,
+// TL (quantized tree, tl_max > 0): every wave keeps its own LDS copy of the first
+// min(tl_max, a.tl_nodes) treelet nodes of the scene of its current chunk (tl_max x 64 B per
+// wave of dynamic LDS), reloaded when a chunk of another scene comes.  The closest-hit TL
+// variant is held to 6 waves per SIMD (80 VGPRs), which its plain counterpart reaches unbidden.
+template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, int TREE, bool TL>
+__global__ __launch_bounds__(BLOCK, (ANY || MASKED) ? 1 : (TREE == 0 ? 8 : (TL ? 6 : 1))) void k_trace_batch(const BatchItem* __restrict__ items, int first, int n_items,
                                                        const int* __restrict__ total_chunks, int* counter, int* spill,
-                                                       int spill_depth) {
+                                                       int spill_depth, int tl_max) {
+    extern __shared__ uint4 treelet_lds[];
     __shared__ int lds_stack[STACK_LDS * BLOCK];
     LaneStack<BLOCK, STACK_LDS> st;
     st.lds = lds_stack;
@@ -644,7 +649,9 @@ __global__ __launch_bounds__(BLOCK, (ANY || MASKED || TREE != 0) ? 1 : 8) void k
     st.spill_slot = blockIdx.x * BLOCK + threadIdx.x;
     st.spill_stride = gridDim.x * BLOCK;
     st.sp = 0;
-    st.tl = nullptr;  // queues of several scenes: no treelet (their TraceArgs have tl_nodes 0)
+    st.tl = TL ? treelet_lds + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6) * 4 * tl_max) : nullptr;
+    const float4* tl_nodes_of = nullptr;  // scene (node array) whose treelet this wave holds
+    int tl_base_of = -1;
     const int lane = __lane_id();
     const int total = __builtin_amdgcn_readfirstlane(*total_chunks);
     for (;;) {
@@ -661,7 +668,24 @@ __global__ __launch_bounds__(BLOCK, (ANY || MASKED || TREE != 0) ? 1 : 8) void k
             else hi = mid - 1;
         }
         lo = __builtin_amdgcn_readfirstlane(lo);
-        const TraceArgs a = items[lo].a;
+        TraceArgs a = items[lo].a;
+        if (TL) {
+            a.tl_nodes = min(a.tl_nodes, tl_max);
+            if (a.nodes != tl_nodes_of || a.tl_base != tl_base_of) {  // wave-uniform
+                // the wave's earlier LDS reads are complete before the copy is overwritten
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                uint4* w = const_cast<uint4*>(st.tl);
+                const uint4* src = reinterpret_cast<const uint4*>(a.nodes) + 4 * (size_t)a.tl_base;
+                for (int i = lane; i < 4 * a.tl_nodes; i += 64) w[i] = src[i];
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                tl_nodes_of = a.nodes;
+                tl_base_of = a.tl_base;
+            }
+        } else {
+            a.tl_nodes = 0;
+        }
         const int id = (c - items[lo].chunk_begin) * 64 + lane;
         if (id < a.count) {
             TraceState s;

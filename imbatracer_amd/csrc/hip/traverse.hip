@@ -107,6 +107,7 @@ int launch_one(const TraceArgs& args_in, int max_stack, int blocks_per_cu, int r
 // Default launch: 256 threads, 16 LDS stack entries per lane, one ray per thread (measured
 // fastest on both the 65K-triangle and the 21M-triangle workloads).
 constexpr int kDefaultBlock = 256;
+constexpr int kBatchTreeletMax = 64;  // batches: nodes per wave's LDS treelet copy (larger requests clamp)
 constexpr int kDefaultStack = 16;
 constexpr int kDefaultWidth = 2;
 
@@ -394,13 +395,19 @@ namespace imba {
 namespace {
 
 template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, int TREE>
-int launch_batch_group(ImbaTraceBatch* b, int g, int blocks_per_cu, hipStream_t s) {
+int launch_batch_group(ImbaTraceBatch* b, int g, int blocks_per_cu, int tl_max, hipStream_t s) {
     const DeviceProps* p = device_props();
     if (!p) return fail(IMBA_ERR_NO_DEVICE, "no HIP device");
-    auto kern = k_trace_batch<BLOCK, STACK_LDS, ANY, MASKED, TREE>;
+    auto kern = k_trace_batch<BLOCK, STACK_LDS, ANY, MASKED, TREE, false>;
+    if constexpr (TREE == 2) {
+        if (tl_max > 0) kern = k_trace_batch<BLOCK, STACK_LDS, ANY, MASKED, TREE, true>;
+    } else {
+        tl_max = 0;
+    }
     const int max_stack = b->max_stack[TREE ? 1 : 0][g];
+    const size_t tl_bytes = (size_t)(BLOCK / 64) * tl_max * 64;  // one treelet copy per wave
     int resident = 0;
-    IMBA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern, BLOCK, 0));
+    IMBA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern, BLOCK, tl_bytes));
     if (resident <= 0) resident = 1;
     if (blocks_per_cu > 0) resident = std::min(resident, blocks_per_cu);
     const long long needed = (b->capacity_chunks[g] * 64 + BLOCK - 1) / BLOCK;
@@ -418,9 +425,9 @@ int launch_batch_group(ImbaTraceBatch* b, int g, int blocks_per_cu, hipStream_t 
         }
         spill = (int*)b->spill[g].ptr;
     }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, s, b->d_items[TREE], b->groups.first[g],
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), tl_bytes, s, b->d_items[TREE], b->groups.first[g],
                        b->groups.first[g + 1] - b->groups.first[g], b->d_group_chunks + g, b->d_counters + 2 * g,
-                       spill, spill_depth);
+                       spill, spill_depth, tl_max);
     IMBA_HIP_TRY(hipGetLastError());
     return IMBA_OK;
 }
@@ -430,8 +437,9 @@ int dispatch_batch(ImbaTraceBatch* b, int g, const ImbaTraceConfig* cfg, hipStre
     const int block = cfg && cfg->block_size ? cfg->block_size : kDefaultBlock;
     const int stack = cfg && cfg->stack_lds ? cfg->stack_lds : kDefaultStack;
     const int bpc = cfg ? cfg->blocks_per_cu : 0;
+    const int tl = cfg ? std::min(cfg->treelet, kBatchTreeletMax) : 0;
 #define IMBA_SHAPE(B, S) \
-    if (block == B && stack == S) return launch_batch_group<B, S, ANY, MASKED, TREE>(b, g, bpc, s);
+    if (block == B && stack == S) return launch_batch_group<B, S, ANY, MASKED, TREE>(b, g, bpc, tl, s);
     IMBA_SHAPE(256, 16)
     IMBA_SHAPE(256, 8)
     IMBA_SHAPE(256, 32)
@@ -501,6 +509,8 @@ int set_tables(ImbaTraceBatch* b, const ImbaTraceItem* items, int n_items, hipSt
             table4.push_back(e);
             if (it.scene->dev4q.nodes) e.a.nodes = (const float4*)it.scene->dev4q.nodes;
             else quantized_ok = false;
+            e.a.tl_base = it.scene->info4.treelet_base;
+            e.a.tl_nodes = it.scene->info4.treelet_nodes;
             table4q.push_back(e);
         }
     }
@@ -608,6 +618,11 @@ int imba_trace_batch_launch(ImbaTraceBatch* b, const ImbaTraceConfig* cfg, void*
     const int tree = tree_code(cfg);
     if (tree < 0) return fail(IMBA_ERR_INVALID_ARG, "bvh_width must be 0, 2 or 4 (quantized only with 4)");
     if (tree == 2 && !b->quantized_ok) return fail(IMBA_ERR_INVALID_ARG, "a queue's scene has no quantized tree");
+    if (cfg && cfg->treelet) {
+        if (cfg->treelet < 0 || cfg->treelet > kTreeletMaxNodes)
+            return fail(IMBA_ERR_INVALID_ARG, "treelet must be 0..%d nodes", kTreeletMaxNodes);
+        if (tree != 2) return fail(IMBA_ERR_INVALID_ARG, "treelet needs the quantized 4-wide tree");
+    }
     bool sync;
     hipStream_t s = resolve_stream(stream, &sync);
     hipLaunchKernelGGL(k_batch_prefix, dim3(1), dim3(64), 0, s, b->d_items[tree], b->groups, b->d_group_chunks);

@@ -284,7 +284,7 @@ typedef struct ImbaTraceConfig {
                               offsets from the node box, rounded outward                  */
     int treelet;           /* quantized tree only: nodes of its top levels (0..256) each
                               block copies into LDS and reads from there (0 = none);
-                              single-queue launches only, batches ignore it               */
+                              batches keep one copy per wave, of at most 64 nodes        */
 } ImbaTraceConfig;
 
 int imba_intersect(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int count,

@@ -39,7 +39,16 @@ def _filled_hits(n):
     return hb
 
 
-@pytest.mark.parametrize("cfg", [None, (128, 32), (64, 64)])
+# (block, stack) or (block, stack, bvh_width, quantized, treelet); batches clamp treelets to 64
+# nodes per wave, and a treelet never changes a result
+CFGS = [None, (128, 32), (64, 64), (256, 16, 4, 1, 0), (256, 16, 4, 1, 64), (128, 32, 4, 1, 256)]
+
+
+def _cfg(cfg):
+    return N.TraceConfig(cfg[0], cfg[1], 0, 0, *cfg[2:]) if cfg else None
+
+
+@pytest.mark.parametrize("cfg", CFGS)
 def test_batch_matches_single_launches(gpu, cfg):
     scenes = _scenes()
     sizes = [0, 1, 63, 64, 65, 1000, 4097, 20000, 333, 7000]
@@ -51,7 +60,8 @@ def test_batch_matches_single_launches(gpu, cfg):
             (T.primary_rays(128, 128)[:n] if k % 3 == 2 else T.random_rays(max(n, 1), seed=100 + k)[:n])
         rb = N.DeviceBuffer.from_array(rays) if n else N.DeviceBuffer(32)
         queues.append((sc, rb, _filled_hits(n), n, None, any_hit, rays))
-    tc = N.TraceConfig(cfg[0], cfg[1], 0, 0) if cfg else None
+    tc = _cfg(cfg)
+    single = _cfg(cfg[:4]) if cfg and len(cfg) > 4 else tc  # the same tree without a treelet
     batch = S.TraceBatch([q[:6] for q in queues])
     for rep in range(2):  # the work counters must return to zero after a launch
         for q in queues:
@@ -62,15 +72,17 @@ def test_batch_matches_single_launches(gpu, cfg):
             if n == 0:
                 continue
             got = hb.download(N.HIT_DTYPE, n)
-            want = sc.trace_host(rays, any_hit=any_hit)
+            want = sc.trace_host(rays, any_hit=any_hit, cfg=single)
             assert _same(got, want), (rep, n, any_hit)
     for sc in scenes:
         assert sc.error_count() == 0
 
 
-def test_batch_device_counts_and_many_queues(gpu):
+@pytest.mark.parametrize("cfg", [None, (256, 16, 4, 1, 64)])
+def test_batch_device_counts_and_many_queues(gpu, cfg):
     """256 queues over 8 scenes, sizes read from device memory at launch time; entries past a
-    queue's device count are left untouched."""
+    queue's device count are left untouched.  With a treelet every wave reloads its LDS copy
+    whenever its next chunk belongs to another scene."""
     bs = [T.random_tri_scene(2000 + 500 * k, N.BUILDER_SBVH if k % 2 else N.BUILDER_BINNED_SAH, seed=40 + k)[0]
           for k in range(8)]
     scenes = [S.Scene(b.desc()) for b in bs]
@@ -87,12 +99,13 @@ def test_batch_device_counts_and_many_queues(gpu):
         items.append((sc, rb, hb, int(caps[q]), cdev.at(4 * q), q % 5 == 0))
         expect.append(rays)
     batch = S.TraceBatch(items)
-    batch.launch()
+    batch.launch(_cfg(cfg))
+    single = _cfg(cfg[:4]) if cfg else None
     for q, (sc, rb, hb, cap, _, any_hit) in enumerate(items):
         got = hb.download(N.HIT_DTYPE, cap)
         n = int(counts[q])
         if n:
-            assert _same(got[:n], sc.trace_host(expect[q][:n], any_hit=any_hit)), q
+            assert _same(got[:n], sc.trace_host(expect[q][:n], any_hit=any_hit, cfg=single)), q
         assert _same(got[n:], np.repeat(SENTINEL, cap - n)), q
 
 
