@@ -115,6 +115,29 @@ def autotune(scene, rays, any_hit=False):
     return cfg
 
 
+def read_bandwidths(stream):
+    """Measured read bandwidth (GB/s) of the level that holds a buffer of each size: 2 MB (L2),
+    128 MB (Infinity Cache, above the 32 MB of all L2s), 4 GiB (HBM); best of three runs."""
+    from imbatracer_amd import native as N
+
+    out = {}
+    sink = N.DeviceBuffer(32 * 1024 * 256)
+    for name, size, passes in (("l2_2MB", 2 << 20, 2000), ("mall_128MB", 128 << 20, 32), ("hbm_4GiB", 4 << 30, 2)):
+        buf = N.DeviceBuffer(size)
+        N.check(N.lib.imba_memset(buf.ptr, 0, size, stream.handle), "memset")
+        ms = C.c_float()
+        best = None
+        for _ in range(3):
+            N.check(N.lib.imba_read_bandwidth(buf.ptr, size, passes, sink.ptr, sink.nbytes, stream.handle,
+                                              C.byref(ms)), "read_bandwidth")
+            gbs = size * passes / (ms.value * 1e-3) / 1e9
+            best = gbs if best is None else max(best, gbs)
+        out[name] = best
+        del buf
+    log("read bandwidth (GB/s): %s" % {k: round(v) for k, v in out.items()})
+    return out
+
+
 def frame_tune(pt, queues, frames=3):
     """Per-scene launch configuration for whole frames: the library's autotuner on queues of
     three sizes (a frame's queues shrink from bounce to bounce), then each distinct winner and
@@ -344,6 +367,10 @@ def run_c1(args):
                      "kernel": "k_trace (closest hit)", "algorithmic_bytes_per_launch": ray_bytes(vis_p, n, node_bytes),
                      "bytes_per_ray": ray_bytes(vis_p, n, node_bytes) / n},
     }
+    # the scene (a few MB) is served from L2 / Infinity Cache: the same figure against their
+    # measured read bandwidth
+    bw = read_bandwidths(st)
+    res["cache_rooflines"] = {k: {"read_gbs": v, "frac": achieved / v} for k, v in bw.items()}
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(b, N.BUILDER_BINNED_SAH, 1, [prim, shadow], [False, True],
                                            budget_s=args.cpu_seconds)

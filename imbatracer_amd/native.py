@@ -182,6 +182,7 @@ _sig("imba_trace_batch_launch", i32, vp, C.POINTER(TraceConfig), vp)
 _sig("imba_trace_batch_update", i32, vp, C.POINTER(TraceItem), i32, vp)
 _sig("imba_trace_batch_release", i32, vp)
 _sig("imba_autotune", i32, vp, vp, vp, i32, i32, i32, C.POINTER(TraceConfig), C.POINTER(f32), vp)
+_sig("imba_read_bandwidth", i32, vp, C.c_size_t, i32, vp, C.c_size_t, vp, C.POINTER(f32))
 _sig("imba_graph_begin", i32, vp)
 _sig("imba_graph_end", i32, vp, C.POINTER(vp))
 _sig("imba_graph_launch", i32, vp, vp)

@@ -532,6 +532,16 @@ int imba_ppm_merge(const void* queries, int nq, int k, const int* out_idx, const
                    const void* photon_rays, const void* photon_states, float radius, int light_paths, float* image,
                    void* stream);
 
+/* ------------------------------ measurement ----------------------------------------------
+ * (No counterpart in the reference: a probe for roofline reporting.)  Reads the device buffer
+ * of `bytes` bytes `passes` times with 16-byte loads from every CU, each pass in a rotated
+ * order so that a CU's vector L1 never serves a repeat, and returns the kernel's time in *ms
+ * (so bytes * passes / ms is the read bandwidth of the cache level or HBM that holds a buffer
+ * of that size).  sink: device scratch of at least 32 KiB * CUs bytes (less for small
+ * buffers) that the kernel may write.  Synchronizes with the stream. */
+int imba_read_bandwidth(const void* buffer, size_t bytes, int passes, void* sink, size_t sink_bytes, void* stream,
+                        float* ms);
+
 /* ------------------------------ synthetic workloads ---------------------------------------
  * Seeded generators for the measured configurations (host memory outputs). */
 /* Random triangles in the unit cube: per triangle a centre ~ U[0,1)^3, then three vertices
