@@ -715,8 +715,29 @@ def run_c3(args):
         hb = N.DeviceBuffer(pm.P * 16)
         cfg, ms = pm.gpu_scene.autotune(lr, hb, pm.P, reps=3)
         log("autotune (light rays, %d): %s, %.3f ms" % (pm.P, cfg_dict(cfg), ms))
-        pm.cfg = cfg
         del lr, ls, hb
+        # the winner on the first light queue and the plain quantized launches (with and
+        # without a 64-node treelet), each timed over whole frames (light and camera passes,
+        # where the traversal runs); the fastest wins
+        best, seen = None, set()
+        for c in (cfg, N.TraceConfig(256, 16, 0, 0, 4, 1, 0), N.TraceConfig(256, 16, 0, 0, 4, 1, 64),
+                  N.TraceConfig(256, 8, 0, 0, 4, 1, 0)):
+            key = tuple(getattr(c, f) for f, _ in c._fields_)
+            if key in seen:
+                continue
+            seen.add(key)
+            pm.cfg = c
+            pm.frame()
+            t = 0.0
+            for _ in range(2):
+                pm.frame(timed=True)
+                ph = pm.phase_ms()
+                t += ph["light"] + ph["camera"]
+            log("frame tune: %s %.3f ms of light + camera passes per frame" % (cfg_dict(c), t / 2))
+            if best is None or t < best[0]:
+                best = (t, c)
+        cfg = best[1]
+        pm.cfg = cfg
     light, cam, n_ph = pm.frame()
     log("light queues %s, camera queues %s, photons %d" % (light, cam, n_ph))
     for _ in range(args.warmup):
