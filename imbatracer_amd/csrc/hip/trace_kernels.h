@@ -176,6 +176,7 @@ enum : int { kTop = 1, kInstLast = 2, kBad = 4, kDone = 8 };
 // Traversal state of one ray (kept in registers across the steps of the persistent kernel).
 struct WorldRay {
     float ox, oy, oz, dx, dy, dz;
+    float idx, idy, idz;  // 1 / safe_dir(d), as setup_space computes them
 };
 
 struct TraceState {
@@ -191,17 +192,24 @@ struct TraceState {
     uint32_t cn, ct, ci;  // visit counts (stats variant)
 };
 
-// The current space becomes world space (reciprocals recomputed, the same values as at the start).
+// The current space becomes world space (the reciprocals kept from the start: the values
+// setup_space gives).
 __device__ __forceinline__ void to_world(TraceState& s) {
     s.cs.ox = s.world.ox; s.cs.oy = s.world.oy; s.cs.oz = s.world.oz;
     s.cs.dx = s.world.dx; s.cs.dy = s.world.dy; s.cs.dz = s.world.dz;
-    setup_space(s.cs);
+    s.cs.idx = s.world.idx; s.cs.idy = s.world.idy; s.cs.idz = s.world.idz;
+    s.cs.oidx = s.cs.ox * s.cs.idx;
+    s.cs.oidy = s.cs.oy * s.cs.idy;
+    s.cs.oidz = s.cs.oz * s.cs.idz;
 }
 
 __device__ __forceinline__ void state_init(TraceState& s, const float4& ro, const float4& rd, int root) {
     s.world.ox = ro.x; s.world.oy = ro.y; s.world.oz = ro.z;
     s.world.dx = rd.x; s.world.dy = rd.y; s.world.dz = rd.z;
-    to_world(s);
+    s.cs.ox = ro.x; s.cs.oy = ro.y; s.cs.oz = ro.z;
+    s.cs.dx = rd.x; s.cs.dy = rd.y; s.cs.dz = rd.z;
+    setup_space(s.cs);
+    s.world.idx = s.cs.idx; s.world.idy = s.cs.idy; s.world.idz = s.cs.idz;
     s.tmin = ro.w;
     s.best_t = rd.w;
     s.best_u = 0.0f;
@@ -234,7 +242,16 @@ __device__ __forceinline__ int enter_instance(const TraceArgs& a, TraceState& s,
     s.cs.dx = fmaf(m0.x, w.dx, fmaf(m0.y, w.dy, m0.z * w.dz));
     s.cs.dy = fmaf(m1.x, w.dx, fmaf(m1.y, w.dy, m1.z * w.dz));
     s.cs.dz = fmaf(m2.x, w.dx, fmaf(m2.y, w.dy, m2.z * w.dz));
-    setup_space(s.cs);
+    if (__float_as_uint(s.cs.dx) == __float_as_uint(w.dx) && __float_as_uint(s.cs.dy) == __float_as_uint(w.dy) &&
+        __float_as_uint(s.cs.dz) == __float_as_uint(w.dz)) {
+        // the direction is unchanged (a translation or the identity): the world reciprocals
+        s.cs.idx = w.idx; s.cs.idy = w.idy; s.cs.idz = w.idz;
+        s.cs.oidx = s.cs.ox * s.cs.idx;
+        s.cs.oidy = s.cs.oy * s.cs.idy;
+        s.cs.oidz = s.cs.oz * s.cs.idz;
+    } else {
+        setup_space(s.cs);
+    }
     s.inst = i;
     s.inst_id = meta.x;
     s.inst_sp = sp;
