@@ -140,32 +140,43 @@ def read_bandwidths(stream):
 
 def frame_tune(pt, queues, frames=3):
     """Per-scene launch configuration for whole frames: the library's autotuner on queues of
-    three sizes (a frame's queues shrink from bounce to bounce), then each distinct winner and
-    the plain quantized launch (with and without a 64-node treelet) timed over whole frames (sum of the traversal kernels' times);
-    the fastest wins."""
+    three sizes (a frame's queues shrink from bounce to bounce), each distinct winner and a few
+    plain quantized launches (with and without a 64-node treelet, persistent or not) timed over
+    whole frames (the traversal kernels' times per bounce).  The frame's tree is the one of the
+    fastest candidate; each bounce then takes the fastest candidate on that tree for its own
+    queue, so every bounce walks the same tree (same visits, same algorithmic bytes)."""
     from imbatracer_amd import native as N
 
     cands = [autotune(pt.gpu_scene, queues[b]) for b in sorted({1, len(queues) // 2, len(queues) - 2})
              if 0 < b < len(queues) and len(queues[b])]
-    cands += [N.TraceConfig(256, 16, 0, 0, 4, 1, 0), N.TraceConfig(256, 16, 0, 0, 4, 1, 64)]
-    seen, best = set(), None
+    cands += [N.TraceConfig(256, 16, 0, 0, 4, 1, 0), N.TraceConfig(256, 16, 0, 0, 4, 1, 64),
+              N.TraceConfig(128, 16, 0, 0, 4, 1, 0), N.TraceConfig(64, 32, 0, 0, 4, 1, 0),
+              N.TraceConfig(256, 16, 32, 0, 4, 1, 0)]
+    seen, timed = set(), []
     for c in cands:
         key = tuple(getattr(c, f) for f, _ in c._fields_)
         if key in seen:
             continue
         seen.add(key)
-        pt.cfg = c
+        pt.cfg, pt.bounce_cfg = c, None
         pt.frame()
-        total = 0.0
+        per_bounce = None
         for _ in range(frames):
             pt.frame(timed=True)
             pt.stream.sync()
-            total += sum(ms for ms, _ in pt.trace_times_ms())
-        log("frame tune: %s %.3f ms of traversal per frame" % (cfg_dict(c), total / frames))
-        if best is None or total < best[0]:
-            best = (total, c)
-    pt.cfg = best[1]
-    return best[1]
+            t = [ms for ms, _ in pt.trace_times_ms()]
+            per_bounce = t if per_bounce is None else [x + y for x, y in zip(per_bounce, t)]
+        log("frame tune: %s %.3f ms of traversal per frame" % (cfg_dict(c), sum(per_bounce) / frames))
+        timed.append((sum(per_bounce), c, per_bounce))
+    best = min(timed, key=lambda x: x[0])
+    tree = (best[1].bvh_width, best[1].quantized)
+    same = [x for x in timed if (x[1].bvh_width, x[1].quantized) == tree]
+    bounce_cfg = [min(same, key=lambda x: x[2][b])[1] for b in range(len(best[2]))]
+    total = sum(min(x[2][b] for x in same) for b in range(len(best[2])))
+    log("frame tune: per bounce %s, %.3f ms of traversal per frame" % (
+        [cfg_dict(c) for c in bounce_cfg], total / frames))
+    pt.cfg, pt.bounce_cfg = best[1], bounce_cfg
+    return best[1], bounce_cfg
 
 
 def cpu_baseline(build, builder, threads_build, ray_sets, any_flags, budget_s=12.0):
@@ -223,8 +234,9 @@ def run_pt(args):
     log("uploaded %.2f GB" % (pt.gpu_scene.footprint() / 1e9))
     # the queues of a frame (closest hits, hence the queues, do not depend on the configuration)
     _, _, queues = pt.frame(keep_queues=8)
+    bounce_cfg = None
     if cfg is None and not args.no_autotune:
-        cfg = frame_tune(pt, queues)
+        cfg, bounce_cfg = frame_tune(pt, queues)
     node_bytes = node_bytes_of(cfg)
 
     # untimed: visit counts of every bounce's queue (same seeds -> same rays as the timed frames)
@@ -289,7 +301,8 @@ def run_pt(args):
                    "frame": "one graph launch per frame (queue sizes on the device)" if args.mode == "graph"
                             else "eager launches, host reads each bounce's queue size",
                    "trace_kernel": args.trace_kernel,
-                   "trace_config": cfg_dict(cfg), "autotuned": cfg is not None and not args.block},
+                   "trace_config": cfg_dict(cfg), "autotuned": cfg is not None and not args.block,
+                   "trace_config_per_bounce": [cfg_dict(c) for c in bounce_cfg] if bounce_cfg else None},
         "eager_frames": {"value": rays_eager / wall_eager / 1e6, "ms_per_step": wall_eager * 1e3 / args.steps,
                          "frame_latency_ms": {"p50": float(np.percentile(eager_ms, 50)),
                                               "p99": float(np.percentile(eager_ms, 99))}},

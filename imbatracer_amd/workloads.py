@@ -196,6 +196,7 @@ class PathTracer:
         self.n = width * height * spp
         self.max_path_len, self.rr_start, self.seed_base = max_path_len, rr_start, seed_base
         self.cfg = trace_cfg
+        self.bounce_cfg = None  # optional per-bounce launch configurations (see cfg_of)
         pos, d, up, fov = scene.camera
         self.cam_pos = np.asarray(pos, np.float32)
         self.r2w = np.zeros(16, np.float32)
@@ -238,7 +239,7 @@ class PathTracer:
                 self.kept_states.append(self.states[cur].download(PT_STATE_DTYPE, n, st))
             if stats:
                 N.check(N.lib.imba_trace_stats(self.gpu_scene.h, self.rays[cur].ptr, self.hits.ptr, self.counts.ptr,
-                                               n, 0, C.byref(self.cfg) if self.cfg else None, st), "trace_stats")
+                                               n, 0, self._cfg_ref(bounce), st), "trace_stats")
                 c = self.counts.download(np.uint32, 3 * n, st).reshape(-1, 3)
                 visit.append(c.astype(np.int64).sum(0))
             else:
@@ -246,7 +247,7 @@ class PathTracer:
                     e0, e1 = N.Event(), N.Event()
                     e0.record(self.stream)
                 N.check(N.lib.imba_intersect(self.gpu_scene.h, self.rays[cur].ptr, self.hits.ptr, n,
-                                             C.byref(self.cfg) if self.cfg else None, st), "intersect")
+                                             self._cfg_ref(bounce), st), "intersect")
                 if timed:
                     e1.record(self.stream)
                     ev.append((e0, e1, n))
@@ -266,6 +267,17 @@ class PathTracer:
     def trace_times_ms(self):
         return [(e0.elapsed_ms(e1), n) for e0, e1, n in self.events]
 
+    def cfg_of(self, bounce: int):
+        """Launch configuration of a bounce's traversal: bounce_cfg[bounce] when set (the queues
+        of a frame shrink from bounce to bounce), else cfg."""
+        if self.bounce_cfg is not None and bounce < len(self.bounce_cfg) and self.bounce_cfg[bounce] is not None:
+            return self.bounce_cfg[bounce]
+        return self.cfg
+
+    def _cfg_ref(self, bounce: int):
+        c = self.cfg_of(bounce)
+        return C.byref(c) if c else None
+
     # -- device-resident frame: counts stay on the GPU, the frame is one graph launch ----------
     def _frame_dev(self, batches):
         """Enqueues one frame whose queue sizes live in self.dev_counts (entry b = rays of bounce
@@ -278,10 +290,10 @@ class PathTracer:
         for bounce in range(self.max_path_len):
             cnt = self.dev_counts.at(4 * bounce)
             if batches:
-                batches[bounce].launch(self.cfg, self.stream)
+                batches[bounce].launch(self.cfg_of(bounce), self.stream)
             else:
                 N.check(N.lib.imba_intersect_dev(self.gpu_scene.h, self.rays[cur].ptr, self.hits.ptr, n, cnt,
-                                                 C.byref(self.cfg) if self.cfg else None, st), "intersect_dev")
+                                                 self._cfg_ref(bounce), st), "intersect_dev")
             N.check(N.lib.imba_pt_bounce_dev(self.rays[cur].ptr, self.hits.ptr, self.states[cur].ptr, n, cnt,
                                              self.tri_n.ptr, self.tri_m.ptr, self.s.tri_count, self.albedo.ptr,
                                              self.emission.ptr, len(self.s.albedo), self.max_path_len, self.rr_start,
