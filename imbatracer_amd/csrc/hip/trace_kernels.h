@@ -632,19 +632,49 @@ struct BatchGroups {
     int first[5];  // items of group g: [first[g], first[g + 1])
 };
 
-// One thread: ray counts of this launch and each queue's first chunk; chunks per group.
-__global__ void k_batch_prefix(BatchItem* items, BatchGroups groups, int* group_chunks) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    for (int g = 0; g < 4; g++) {
-        int c = 0;
-        for (int i = groups.first[g]; i < groups.first[g + 1]; i++) {
-            int n = items[i].capacity;
-            if (items[i].count_dev) n = min(max(*items[i].count_dev, 0), n);
-            items[i].a.count = n;
-            items[i].chunk_begin = c;
-            c += (n + 63) >> 6;
-        }
-        group_chunks[g] = c;
+// One block of 1024 threads: ray counts of this launch, each queue's first chunk, chunks per group.
+__global__ __launch_bounds__(1024) void k_batch_prefix(BatchItem* items, BatchGroups groups, int* group_chunks) {
+    // thread t owns the queues [b0, b1); items are ordered by group, so a queue's chunk_begin is
+    // its exclusive chunk prefix over all queues minus that of its group's first queue
+    __shared__ int part[1024];
+    __shared__ int base[5];
+    const int n_items = groups.first[4];
+    const int t = threadIdx.x;
+    const int per = (n_items + 1023) / 1024;
+    const int b0 = min(n_items, t * per), b1 = min(n_items, b0 + per);
+    int sum = 0;
+    for (int i = b0; i < b1; i++) {
+        int n = items[i].capacity;
+        if (items[i].count_dev) n = min(max(*items[i].count_dev, 0), n);
+        items[i].a.count = n;
+        sum += (n + 63) >> 6;
+    }
+    part[t] = sum;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {  // inclusive scan of the per-thread sums
+        const int v = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    const int total = part[1023];
+    int run = part[t] - sum;  // exclusive prefix at b0
+    for (int i = b0; i < b1; i++) {
+        for (int g = 0; g < 5; g++)
+            if (groups.first[g] == i) base[g] = run;
+        run += (items[i].a.count + 63) >> 6;
+    }
+    __syncthreads();
+    run = part[t] - sum;
+    int g = 0;
+    for (int i = b0; i < b1; i++) {
+        while (g < 3 && groups.first[g + 1] <= i) g++;
+        items[i].chunk_begin = run - base[g];
+        run += (items[i].a.count + 63) >> 6;
+    }
+    if (t < 4) {
+        const int f0 = groups.first[t], f1 = groups.first[t + 1];
+        group_chunks[t] = f0 == f1 ? 0 : (f1 < n_items ? base[t + 1] : total) - base[t];
     }
 }
 

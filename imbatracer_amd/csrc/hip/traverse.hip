@@ -625,7 +625,7 @@ int imba_trace_batch_launch(ImbaTraceBatch* b, const ImbaTraceConfig* cfg, void*
     }
     bool sync;
     hipStream_t s = resolve_stream(stream, &sync);
-    hipLaunchKernelGGL(k_batch_prefix, dim3(1), dim3(64), 0, s, b->d_items[tree], b->groups, b->d_group_chunks);
+    hipLaunchKernelGGL(k_batch_prefix, dim3(1), dim3(1024), 0, s, b->d_items[tree], b->groups, b->d_group_chunks);
     IMBA_HIP_TRY(hipGetLastError());
     for (int g = 0; g < 4; g++) {
         if (b->groups.first[g + 1] == b->groups.first[g] || b->capacity_chunks[g] == 0) continue;
