@@ -142,6 +142,33 @@ __device__ __forceinline__ int texel_opaque(const TraceArgs& a, int tri, float u
     return a.mask_buf[at] != 0 ? 1 : 0;
 }
 
+// Address-space qualifiers for the device pass (the host pass sees plain pointers).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define IMBA_AS(n) __attribute__((address_space(n)))
+#else
+#define IMBA_AS(n)
+#endif
+
+// The LDS (address space 3) view of a pointer into a kernel's __shared__ memory: accesses through
+// it are ds_read / ds_write, not flat accesses (which the compiler otherwise emits when an LDS
+// and a global access are merged into one load through a selected pointer).
+template <typename T>
+__device__ __forceinline__ IMBA_AS(3) T* lds_view(T* p) {
+    return (IMBA_AS(3) T*)p;
+}
+
+// The global (address space 1) view of a scene or queue array: loads through it are
+// global_load, not flat_load, also where the pointer itself was read from memory (the batched
+// kernel's queue table), which the compiler cannot otherwise place.
+template <typename T>
+__device__ __forceinline__ const IMBA_AS(1) T* gview(const T* p) {
+    return (const IMBA_AS(1) T*)p;
+}
+template <typename T>
+__device__ __forceinline__ IMBA_AS(1) T* gview_w(T* p) {
+    return (IMBA_AS(1) T*)p;
+}
+
 template <int BLOCK, int STACK_LDS>
 struct LaneStack {
     int* lds;     // [STACK_LDS][BLOCK]
@@ -155,7 +182,7 @@ struct LaneStack {
 
     __device__ __forceinline__ bool push(int v) {
         if (sp < STACK_LDS) {
-            lds[sp * BLOCK + lane_slot] = v;
+            lds_view(lds)[sp * BLOCK + lane_slot] = v;
         } else {
             const int k = sp - STACK_LDS;
             if (k >= spill_depth) return false;
@@ -166,7 +193,7 @@ struct LaneStack {
     }
     __device__ __forceinline__ int pop() {
         sp--;
-        if (sp < STACK_LDS) return lds[sp * BLOCK + lane_slot];
+        if (sp < STACK_LDS) return lds_view(lds)[sp * BLOCK + lane_slot];
         return spill[(size_t)(sp - STACK_LDS) * spill_stride + spill_slot];
     }
 };
@@ -232,8 +259,9 @@ __device__ __forceinline__ int enter_instance(const TraceArgs& a, TraceState& s,
         s.flags |= kBad | kInstLast;
         return kNoNode;
     }
-    const float4 m0 = a.inst[4 * i + 0], m1 = a.inst[4 * i + 1], m2 = a.inst[4 * i + 2];
-    const int4 meta = reinterpret_cast<const int4*>(a.inst)[4 * i + 3];
+    const auto* inst = gview(a.inst);
+    const float4 m0 = inst[4 * i + 0], m1 = inst[4 * i + 1], m2 = inst[4 * i + 2];
+    const int4 meta = gview(reinterpret_cast<const int4*>(a.inst))[4 * i + 3];
     if (STATS) s.ci++;
     const WorldRay& w = s.world;
     s.cs.ox = fmaf(m0.x, w.ox, fmaf(m0.y, w.oy, fmaf(m0.z, w.oz, m0.w)));
@@ -287,10 +315,11 @@ __device__ __forceinline__ void node_step2(const TraceArgs& a, TraceState& s, La
         s.node = kNoNode;
         return;
     }
-    const float4 n0 = a.nodes[4 * n + 0];
-    const float4 n1 = a.nodes[4 * n + 1];
-    const float4 n2 = a.nodes[4 * n + 2];
-    const int4 n3 = reinterpret_cast<const int4*>(a.nodes)[4 * n + 3];
+    const auto* nodes = gview(a.nodes);
+    const float4 n0 = nodes[4 * n + 0];
+    const float4 n1 = nodes[4 * n + 1];
+    const float4 n2 = nodes[4 * n + 2];
+    const int4 n3 = gview(reinterpret_cast<const int4*>(a.nodes))[4 * n + 3];
     if (STATS) s.cn++;
     float tl = 0.0f, tr = 0.0f;
     bool hl = box_hit(s.cs, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, s.tmin, s.best_t, tl);
@@ -329,9 +358,9 @@ __device__ __forceinline__ void node_step4(const TraceArgs& a, TraceState& s, La
         s.node = kNoNode;
         return;
     }
-    const float4* p = a.nodes + 8 * (size_t)n;
+    const auto* p = gview(a.nodes + 8 * (size_t)n);
     const float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
-    const int4 ch = reinterpret_cast<const int4*>(p)[6];
+    const int4 ch = gview(reinterpret_cast<const int4*>(a.nodes + 8 * (size_t)n))[6];
     if (STATS) s.cn++;
     float d0, d1, d2, d3;
     bool h0 = ch.x != 0 && box_hit(s.cs, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, s.tmin, s.best_t, d0);
@@ -372,7 +401,7 @@ __device__ __forceinline__ float dequant(uint32_t word, int j, float scale, floa
 
 // One node of the quantized 4-wide tree (64 B: origin, exponents, children, 8-bit child
 // bounds per axis); otherwise as node_step4.  The dequantized boxes contain the exact ones.
-template <int BLOCK, int STACK_LDS, bool STATS>
+template <int BLOCK, int STACK_LDS, bool STATS, bool TLK>
 __device__ __forceinline__ void node_step4q(const TraceArgs& a, TraceState& s, LaneStack<BLOCK, STACK_LDS>& st) {
     const int n = s.node;
     if (n >= a.n_nodes) {
@@ -382,11 +411,15 @@ __device__ __forceinline__ void node_step4q(const TraceArgs& a, TraceState& s, L
     }
     uint4 w0, w1, w2, w3;
     const unsigned in_tl = (unsigned)(n - a.tl_base);
-    if (in_tl < (unsigned)a.tl_nodes) {
+    if (TLK && in_tl < (unsigned)a.tl_nodes) {
         const uint4* q = st.tl + 4 * in_tl;
         w0 = q[0]; w1 = q[1]; w2 = q[2]; w3 = q[3];
-    } else {
+    } else if (TLK) {
+        // merged with the branch above into one flat load by the compiler
         const uint4* p = reinterpret_cast<const uint4*>(a.nodes) + 4 * (size_t)n;
+        w0 = p[0]; w1 = p[1]; w2 = p[2]; w3 = p[3];
+    } else {
+        const auto* p = gview(reinterpret_cast<const uint4*>(a.nodes) + 4 * (size_t)n);
         w0 = p[0]; w1 = p[1]; w2 = p[2]; w3 = p[3];
     }
     if (STATS) s.cn++;
@@ -430,7 +463,7 @@ __device__ __forceinline__ void node_step4q(const TraceArgs& a, TraceState& s, L
 // Advances one ray by one step: bookkeeping (next instance, back to world space, pop) if
 // nothing is pending, then one triangle test or one inner node.  Returns true when the ray
 // is finished.
-template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS, int TREE>
+template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS, int TREE, bool TLK = true>
 __device__ __forceinline__ bool advance(const TraceArgs& a, TraceState& s, LaneStack<BLOCK, STACK_LDS>& st) {
     if (s.tri < 0 && s.node == kNoNode) {
         // the current instance's tree is exhausted: next instance of the leaf or world space
@@ -457,7 +490,8 @@ __device__ __forceinline__ bool advance(const TraceArgs& a, TraceState& s, LaneS
             s.tri = -1;
             return false;
         }
-        const float4 v0 = a.tris[t], v1 = a.tris[t + 1], v2 = a.tris[t + 2];
+        const auto* tris = gview(a.tris);
+        const float4 v0 = tris[t], v1 = tris[t + 1], v2 = tris[t + 2];
         if (STATS) s.ct++;
         s.tri = (__float_as_uint(v2.w) == kLeafEnd) ? -1 : t + 3;
         float th, uh, vh;
@@ -494,7 +528,7 @@ __device__ __forceinline__ bool advance(const TraceArgs& a, TraceState& s, LaneS
         return false;
     }
     if (s.node >= 0) {
-        if (TREE == 2) node_step4q<BLOCK, STACK_LDS, STATS>(a, s, st);
+        if (TREE == 2) node_step4q<BLOCK, STACK_LDS, STATS, TLK>(a, s, st);
         else if (TREE == 1) node_step4<BLOCK, STACK_LDS, STATS>(a, s, st);
         else node_step2<BLOCK, STACK_LDS, STATS>(a, s, st);
     }
@@ -530,7 +564,7 @@ __device__ __forceinline__ void treelet_init(const TraceArgs& a, LaneStack<BLOCK
 
 template <bool STATS>
 __device__ __forceinline__ void write_result(const TraceArgs& a, const TraceState& s, int id) {
-    a.hits[id] = make_int4(s.best_tri, s.best_inst, __float_as_int(s.best_t), __float_as_int(s.best_u));
+    gview_w(a.hits)[id] = make_int4(s.best_tri, s.best_inst, __float_as_int(s.best_t), __float_as_int(s.best_u));
     if (STATS) {
         a.counts[3 * (size_t)id + 0] = s.cn;
         a.counts[3 * (size_t)id + 1] = s.ct;
@@ -736,9 +770,9 @@ __global__ __launch_bounds__(BLOCK, (ANY || MASKED) ? 1 : (TREE == 0 ? 8 : (TL ?
         const int id = (c - items[lo].chunk_begin) * 64 + lane;
         if (id < a.count) {
             TraceState s;
-            state_init(s, a.rays[2 * (size_t)id], a.rays[2 * (size_t)id + 1], a.root);
+            state_init(s, gview(a.rays)[2 * (size_t)id], gview(a.rays)[2 * (size_t)id + 1], a.root);
             st.sp = 0;
-            while (!advance<BLOCK, STACK_LDS, ANY, MASKED, false, TREE>(a, s, st)) {
+            while (!advance<BLOCK, STACK_LDS, ANY, MASKED, false, TREE, TL>(a, s, st)) {
             }
             write_result<false>(a, s, id);
         }
