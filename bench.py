@@ -624,10 +624,12 @@ def run_c5(args):
     def compare(request_list, label):
         out = {}
         for name, batched in (("batched", True), ("plain", False)):
-            server = V.Server(scenes, arena, max_active=args.c5_max_active, batched=batched, cfg=cfg)
+            server = V.Server(scenes, arena, max_active=args.c5_max_active, batched=batched, cfg=cfg,
+                              max_rays_in_flight=args.c5_max_rays)
             # warm-up on a few requests (kernels loaded, batch tables sized)
             server.run(V.poisson_requests(4, 1000.0, 1, len(scenes), centres, seed=99))
-            server = V.Server(scenes, arena, max_active=args.c5_max_active, batched=batched, cfg=cfg)
+            server = V.Server(scenes, arena, max_active=args.c5_max_active, batched=batched, cfg=cfg,
+                              max_rays_in_flight=args.c5_max_rays)
             t = time.perf_counter()
             done = server.run([V.Request(**{**r.__dict__}) for r in request_list], progress=log)
             wall = time.perf_counter() - t
@@ -659,7 +661,7 @@ def run_c5(args):
                    "scene_triangles": tris, "integrators": list(integrators),
                    "requests_per_integrator": {i: sum(r.integrator == i for r in reqs) for i in integrators},
                    "arena_gb": args.c5_arena_gb,
-                   "max_active": args.c5_max_active, "own_launch_rays": V.Server.OWN_LAUNCH_RAYS,
+                   "max_active": args.c5_max_active, "max_rays_in_flight": args.c5_max_rays, "own_launch_rays": V.Server.OWN_LAUNCH_RAYS,
                    "resolutions": ["256x256", "640x480", "1280x720", "1920x1080"], "spp": "1-4"},
         "served": b, "plain_per_request": p,
         "served_vs_plain": {"throughput": b["mrays_s"] / p["mrays_s"], "p99": b["p99_ms"] / p["p99_ms"]},
@@ -862,6 +864,8 @@ def main():
     ap.add_argument("--c5-arena-gb", type=int, default=48, help="c5: device memory for request buffers")
     ap.add_argument("--c5-small", action="store_true", help="c5: eight small scenes (quick runs)")
     ap.add_argument("--c5-max-active", type=int, default=64, help="c5: requests in flight at once (batched loop)")
+    ap.add_argument("--c5-max-rays", type=int, default=32 << 20,
+                    help="c5: first-traversal rays of the requests in flight at once (batched loop; 0 = no limit)")
     ap.add_argument("--c5-integrators", default="pt,ppm",
                     help="c5: integrators drawn per request (pt: path tracing, ppm: photon mapping)")
     ap.add_argument("--c3-paths", type=int, default=921_600, help="c3: light paths per frame")

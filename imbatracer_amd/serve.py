@@ -117,6 +117,14 @@ PPM_DEPTH = 6   # light and camera path hits of a photon-mapping request
 PPM_K = 10      # photons per merge query
 
 
+def first_rays(req: Request) -> int:
+    """Rays of a request's first traversal: its camera samples (PT), its light paths plus its
+    camera samples (PPM: the light pass, then the camera pass)."""
+    if req.integrator == "pt":
+        return req.samples
+    return req.width * req.height + req.samples
+
+
 def footprint(req: Request) -> int:
     """Device bytes a request holds while active.  PT: two ray and state buffers, hits, image and
     the compaction workspace.  PPM: the same for max(light paths, samples) entries, the photon
@@ -312,10 +320,17 @@ class Server:
     OWN_LAUNCH_RAYS = 131072
 
     def __init__(self, scenes: list, arena_bytes: int, max_active: int = 32, batched: bool = True,
-                 max_path_len: int = 8, rr_start: int = 3, cfg: N.TraceConfig | None = None):
+                 max_path_len: int = 8, rr_start: int = 3, cfg: N.TraceConfig | None = None,
+                 max_rays_in_flight: int | None = None):
+        """max_rays_in_flight: admission also waits (first come, first served) while the
+        requests in flight already hold this many first-traversal rays (first_rays), so that
+        large requests are not spread over too many at once (each one's latency grows with the
+        work it shares the GPU with) while small ones still batch up to max_active."""
         self.scenes = scenes
         self.arena = DeviceArena(arena_bytes)
         self.max_active = max_active if batched else 1
+        self.max_rays = max_rays_in_flight if batched else None
+        self.rays_in_flight = 0
         self.batched = batched
         self.max_path_len, self.rr_start = max_path_len, rr_start
         self.cfg = cfg
@@ -364,11 +379,14 @@ class Server:
         st = self.stream.handle
         while self.pending and self.free_slots:
             req = self.pending[0]
+            if self.max_rays and self.active and self.rays_in_flight + first_rays(req) > self.max_rays:
+                break  # first come first served: wait for work in flight to finish
             block = self.arena.alloc(footprint(req))
             if block is None:
                 break  # first come first served: wait for memory
             self.pending.pop(0)
             req.admitted = self.now()
+            self.rays_in_flight += first_rays(req)
             if req.integrator == "ppm":
                 job = _PPMJob(req, self.free_slots.pop(0), block, self.scenes[req.scene], self.arena)
                 job.start(self, st)
@@ -447,6 +465,7 @@ class Server:
                     j.req.rays = int(sum(j.req.counts))
             if finished:
                 j.req.done = self.now()
+                self.rays_in_flight -= first_rays(j.req)
                 self.done.append(j.req)
                 self.arena.release(*j.block)
                 self.free_slots.append(j.slot)
