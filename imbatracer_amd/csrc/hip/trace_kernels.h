@@ -716,10 +716,11 @@ __global__ __launch_bounds__(1024) void k_batch_prefix(BatchItem* items, BatchGr
 
 // TL (quantized tree, tl_max > 0): every wave keeps its own LDS copy of the first
 // min(tl_max, a.tl_nodes) treelet nodes of the scene of its current chunk (tl_max x 64 B per
-// wave of dynamic LDS), reloaded when a chunk of another scene comes.  The closest-hit TL
-// variant is held to 6 waves per SIMD (80 VGPRs), which its plain counterpart reaches unbidden.
+// wave of dynamic LDS), reloaded when a chunk of another scene comes.  The 4-wide closest-hit
+// variants are held to 7 waves per SIMD (72 VGPRs, a few bytes of scratch per lane), faster
+// than their natural 6.
 template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, int TREE, bool TL>
-__global__ __launch_bounds__(BLOCK, (ANY || MASKED) ? 1 : (TREE == 0 ? 8 : (TL ? 6 : 1))) void k_trace_batch(const BatchItem* __restrict__ items, int first, int n_items,
+__global__ __launch_bounds__(BLOCK, (ANY || MASKED) ? 1 : (TREE == 0 ? 8 : 7)) void k_trace_batch(const BatchItem* __restrict__ items, int first, int n_items,
                                                        const int* __restrict__ total_chunks, int* counter, int* spill,
                                                        int spill_depth, int tl_max) {
     extern __shared__ uint4 treelet_lds[];
