@@ -213,6 +213,8 @@ class PathTracer:
         self.emission = N.DeviceBuffer.from_array(scene.emission)
         self.image = N.DeviceBuffer(width * height * 12)
         self.stream = N.Stream()
+        # frames add into the image: it starts at zero (device allocations are not cleared)
+        N.check(N.lib.imba_memset(self.image.ptr, 0, width * height * 12, self.stream.handle), "memset")
         self.events = []
         self.dev_counts = None
         self.graph = None
@@ -420,6 +422,7 @@ class PhotonMapper:
         self.light_int = np.ascontiguousarray(scene.emission[light], np.float32)
         self.grid = HashGrid()
         self.stream = N.Stream()
+        self.clear_image()  # frames add into the image; device allocations are not cleared
         self.photon_count = 0
         self.events = {}
         # with keep_queues, every bounce's queue (rays, states) is read back before it is
