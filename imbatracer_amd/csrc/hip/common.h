@@ -47,7 +47,13 @@ struct Scratch {
     size_t bytes = 0;
 };
 
+// Grows s to at least `bytes` (device-wide synchronisation first when it must be replaced).
+// The replaced buffer is retired, not freed, while graphs exist (see retire_buffer).
 int grow_buffer(Scratch& s, size_t bytes);
+
+// Frees a library-owned device buffer, or keeps it until the last live graph is released when
+// graphs exist (a captured launch may still name it).
+int retire_buffer(void* p);
 
 inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 

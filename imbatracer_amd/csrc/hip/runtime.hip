@@ -58,11 +58,32 @@ hipStream_t thread_stream() {
     return t_streams.s[dev];
 }
 
+// Device buffers replaced while graphs exist are kept until the last graph is released: a
+// captured launch keeps the address it was recorded with, so the buffer it names must outlive
+// it.  Without graphs they are freed at once.
+namespace {
+std::mutex g_retired_mutex;
+std::vector<void*> g_retired;
+int g_live_graphs = 0;
+}  // namespace
+
+int retire_buffer(void* p) {
+    if (!p) return IMBA_OK;
+    std::lock_guard<std::mutex> lock(g_retired_mutex);
+    if (g_live_graphs > 0) {
+        g_retired.push_back(p);
+        return IMBA_OK;
+    }
+    IMBA_HIP_TRY(hipFree(p));
+    return IMBA_OK;
+}
+
 int grow_buffer(Scratch& s, size_t bytes) {
     if (s.bytes >= bytes && s.ptr) return IMBA_OK;
     if (s.ptr) {
         IMBA_HIP_TRY(hipDeviceSynchronize());
-        IMBA_HIP_TRY(hipFree(s.ptr));
+        int r = retire_buffer(s.ptr);
+        if (r) return r;
         s.ptr = nullptr;
         s.bytes = 0;
     }
@@ -252,6 +273,12 @@ int imba_device_count(int* count) {
 
 int imba_set_device(int device) {
     IMBA_HIP_TRY(hipSetDevice(device));
+    return IMBA_OK;
+}
+
+int imba_get_device(int* device) {
+    IMBA_CHECK_ARG(device, "device is NULL");
+    IMBA_HIP_TRY(hipGetDevice(device));
     return IMBA_OK;
 }
 
@@ -474,6 +501,10 @@ int imba_graph_end(void* stream, ImbaGraph** out) {
     auto* r = new ImbaGraph();
     r->graph = g;
     r->exec = exec;
+    {
+        std::lock_guard<std::mutex> lock(g_retired_mutex);
+        g_live_graphs++;
+    }
     *out = r;
     return IMBA_OK;
 }
@@ -493,6 +524,13 @@ int imba_graph_release(ImbaGraph* g) {
     hipGraphExecDestroy(g->exec);
     hipGraphDestroy(g->graph);
     delete g;
+    std::vector<void*> to_free;
+    {
+        std::lock_guard<std::mutex> lock(g_retired_mutex);
+        if (g_live_graphs > 0) g_live_graphs--;
+        if (g_live_graphs == 0) to_free.swap(g_retired);
+    }
+    for (void* p : to_free) IMBA_HIP_TRY(hipFree(p));
     return IMBA_OK;
 }
 

@@ -38,6 +38,9 @@ struct ImbaShadeBatch {
     ImbaShadeItem* staging[2] = {nullptr, nullptr};  // pinned host copies of the table
     hipEvent_t staged[2] = {nullptr, nullptr};
     int next_staging = 0;
+    bool captured = false;  // launched into a graph: the device tables may no longer move
+    long long captured_chunks = 0;  // grid sizes recorded into graphs (the largest)
+    long long captured_tiles = 0;
 };
 
 namespace imba {
@@ -254,10 +257,23 @@ int set_items(ImbaShadeBatch* b, const ImbaShadeItem* items, int n, hipStream_t 
         tiles += ceil_div(it.capacity, kCTile);
     }
     if (chunks > INT32_MAX) return fail(IMBA_ERR_INVALID_ARG, "too many entries for one batch");
+    // A graph replays its launches with the table addresses they were recorded with, so the
+    // tables of a captured batch cannot be reallocated.
+    if (b->captured && (n > b->cap || tiles > b->tiles_cap || chunks > b->captured_chunks ||
+                        tiles > b->captured_tiles))
+        return fail(IMBA_ERR_CAPACITY, "shade batch was launched into a graph with grids for %lld chunks / %lld tiles "
+                    "and tables for %d items; %lld / %lld / %d do not fit", b->captured_chunks, b->captured_tiles,
+                    b->cap, chunks, tiles, n);
     if (n > b->cap) {
         IMBA_HIP_TRY(hipDeviceSynchronize());
-        if (b->d_items) IMBA_HIP_TRY(hipFree(b->d_items));
-        if (b->d_info) IMBA_HIP_TRY(hipFree(b->d_info));
+        if (b->d_items) {
+            int r = retire_buffer(b->d_items);
+            if (r) return r;
+        }
+        if (b->d_info) {
+            int r = retire_buffer(b->d_info);
+            if (r) return r;
+        }
         b->d_items = nullptr;
         b->d_info = nullptr;
         for (int k = 0; k < 2; k++) {
@@ -275,7 +291,10 @@ int set_items(ImbaShadeBatch* b, const ImbaShadeItem* items, int n, hipStream_t 
     }
     if (tiles > b->tiles_cap) {
         IMBA_HIP_TRY(hipDeviceSynchronize());
-        if (b->d_tile_counts) IMBA_HIP_TRY(hipFree(b->d_tile_counts));
+        if (b->d_tile_counts) {
+            int r = retire_buffer(b->d_tile_counts);
+            if (r) return r;
+        }
         b->d_tile_counts = nullptr;
         IMBA_HIP_TRY(hipMalloc((void**)&b->d_tile_counts, sizeof(int) * (size_t)std::max(tiles, 1LL)));
         b->tiles_cap = tiles;
@@ -345,6 +364,13 @@ int imba_shade_batch_launch(ImbaShadeBatch* b, int max_path_len, int rr_start, v
     if (b->n == 0) return IMBA_OK;
     bool sync;
     hipStream_t s = resolve_stream(stream, &sync);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    IMBA_HIP_TRY(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone) {
+        b->captured = true;
+        b->captured_chunks = std::max(b->captured_chunks, b->max_chunks);
+        b->captured_tiles = std::max(b->captured_tiles, b->max_tiles);
+    }
     const int n = b->n;
     hipLaunchKernelGGL(k_shade_prefix, dim3(1), dim3(1024), 0, s, (const ImbaShadeItem*)b->d_items, n, b->d_info);
     if (b->max_chunks > 0)

@@ -23,25 +23,41 @@ namespace {
 std::mutex g_scratch_mutex;
 std::map<std::pair<int, hipStream_t>, Scratch> g_spill;
 
-// Work counters of the persistent kernel: a pool of slots per device, handed out round robin;
-// each launch's last wave sets its slot back to zero (so a slot is reused only after 1024
-// later launches).
+// Work counters of the persistent kernel, per device: kCounterSlots slots handed out round
+// robin to uncaptured launches (each launch's last wave sets its slot back to zero, so a slot
+// is reused only after 1024 later launches), and kCaptureSlots slots of which every launch
+// recorded into a graph gets one of its own for good (a graph replays its launches with the
+// slot they were recorded with, so no later launch may share it).
 constexpr int kCounterSlots = 1024;
+constexpr int kCaptureSlots = 4096;
 std::mutex g_counter_mutex;
 int* g_counters[64] = {nullptr};
 unsigned g_counter_next[64] = {0};
+unsigned g_capture_next[64] = {0};
 
-int counter_slot(int dev, int** slot) {
+int counter_slot(int dev, bool capturing, int** slot) {
     std::lock_guard<std::mutex> lock(g_counter_mutex);
     if (!g_counters[dev]) {
         int* p = nullptr;
-        IMBA_HIP_TRY(hipMalloc((void**)&p, sizeof(int) * 2 * kCounterSlots));
-        IMBA_HIP_TRY(hipMemset(p, 0, sizeof(int) * 2 * kCounterSlots));
+        const size_t bytes = sizeof(int) * 2 * (kCounterSlots + kCaptureSlots);
+        IMBA_HIP_TRY(hipMalloc((void**)&p, bytes));
+        IMBA_HIP_TRY(hipMemset(p, 0, bytes));
         IMBA_HIP_TRY(hipDeviceSynchronize());
         g_counters[dev] = p;
     }
-    *slot = g_counters[dev] + 2 * (g_counter_next[dev]++ % kCounterSlots);
+    if (capturing) {
+        if (g_capture_next[dev] >= (unsigned)kCaptureSlots)
+            return fail(IMBA_ERR_CAPACITY, "more than %d persistent launches recorded into graphs", kCaptureSlots);
+        *slot = g_counters[dev] + 2 * (kCounterSlots + g_capture_next[dev]++);
+    } else {
+        *slot = g_counters[dev] + 2 * (g_counter_next[dev]++ % kCounterSlots);
+    }
     return IMBA_OK;
+}
+
+bool is_capturing(hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
 }
 
 // Allocation is not allowed while `s` is being captured into a graph.
@@ -94,7 +110,7 @@ int launch_one(const TraceArgs& args_in, int max_stack, int blocks_per_cu, int r
             int r = not_capturing(s, "the persistent kernel's work counters");
             if (r) return r;
         }
-        int r = counter_slot(p->device & 63, &a.work_counter);
+        int r = counter_slot(p->device & 63, is_capturing(s), &a.work_counter);
         if (r) return r;
         hipLaunchKernelGGL(kern_dyn, dim3(grid), dim3(BLOCK), tl_bytes, s, a, std::min(refill, 64));
     } else {
@@ -228,34 +244,18 @@ int tree_of(const ImbaScene* s, const ImbaTraceConfig* cfg, TreeView* t) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Reference FFI: the caller passes only device pointers and the ray count.  Array sizes are
-// taken from the allocations holding them, and the structure is validated once per set of
-// arrays (the host copy made for that is dropped afterwards).
-struct FfiKey {
-    int root;
-    const void *nodes, *instances, *tris, *indices, *texcoords, *masks, *mask_buffer;
-    int counts[7];
-    bool operator<(const FfiKey& o) const {
-        return std::tie(root, nodes, instances, tris, indices, texcoords, masks, mask_buffer, counts[0], counts[1],
-                        counts[2], counts[3], counts[4], counts[5], counts[6]) <
-               std::tie(o.root, o.nodes, o.instances, o.tris, o.indices, o.texcoords, o.masks, o.mask_buffer,
-                        o.counts[0], o.counts[1], o.counts[2], o.counts[3], o.counts[4], o.counts[5], o.counts[6]);
-    }
-};
-struct FfiScene {
-    ImbaSceneDesc dev;
-    SceneInfo info;
-    ImbaSceneDesc dev4;  // derived 4-wide tree (nodes and instance records owned)
-    ImbaSceneDesc dev4q;
-    SceneInfo info4;
-    int* d_err = nullptr;
-};
-
-int ffi_tree(FfiScene* fs, const ImbaTraceConfig* cfg, TreeView* t) {
-    return pick_tree(cfg, &fs->dev, &fs->info, &fs->dev4, &fs->dev4q, &fs->info4, t);
-}
+// Reference FFI: the caller passes only device pointers and the ray count, and may rewrite or
+// reallocate the arrays between calls (Scene::upload_top_level_accel copies a new top level
+// into the arrays it has, render/scene.cpp:166-188).  So nothing derived from the arrays'
+// contents is kept from one call to the next: each call takes the array sizes from the
+// allocations that hold the pointers at that moment, traverses the BVH2 as given with the mask
+// test on, and gives every ray a stack of kFfiStack entries (kDefaultStack in LDS, the rest in
+// the per-stream spill buffer).  The kernel bounds-checks every index against those sizes, so a
+// call is memory safe whatever the arrays hold; a ray that needs a deeper stack is counted as
+// an error (imba_ffi_error_count).
+constexpr int kFfiStack = 64;
 std::mutex g_ffi_mutex;
-std::map<FfiKey, FfiScene> g_ffi_scenes;
+int* g_ffi_err[64] = {nullptr};  // per device: rays of FFI calls that met a malformed structure
 
 size_t bytes_after(const void* p) {
     if (!p) return 0;
@@ -265,77 +265,52 @@ size_t bytes_after(const void* p) {
     return (size_t)((const char*)base + size - (const char*)p);
 }
 
-template <typename T>
-std::vector<T> fetch(const void* p, size_t bytes) {
-    std::vector<T> v(bytes / sizeof(T));
-    if (!v.empty() && hipMemcpy(v.data(), p, v.size() * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) v.clear();
-    return v;
+int ffi_err_counter(int** out) {
+    int dev = 0;
+    IMBA_HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_ffi_mutex);
+    int*& c = g_ffi_err[dev & 63];
+    if (!c) {
+        IMBA_HIP_TRY(hipMalloc((void**)&c, sizeof(int)));
+        IMBA_HIP_TRY(hipMemset(c, 0, sizeof(int)));
+        IMBA_HIP_TRY(hipDeviceSynchronize());
+    }
+    *out = c;
+    return IMBA_OK;
 }
 
-// `given` holds device pointers and, where known, element counts (0 = take the count from the
-// allocation that holds the pointer).
-int ffi_scene(const ImbaSceneDesc& given, FfiScene** out) {
-    const int root = given.root;
-    const void *nodes = given.nodes, *instances = given.instances, *tris = given.tris, *indices = given.indices,
-               *texcoords = given.texcoords, *masks = given.masks, *mask_buffer = given.mask_buffer;
-    FfiKey key{root, nodes, instances, tris, indices, texcoords, masks, mask_buffer,
-               {given.node_count, given.instance_count, given.tri_vec4_count, given.index_count,
-                given.texcoord_count, given.mask_count, given.mask_buffer_size}};
-    std::lock_guard<std::mutex> lock(g_ffi_mutex);
-    auto it = g_ffi_scenes.find(key);
-    if (it != g_ffi_scenes.end()) { *out = &it->second; return IMBA_OK; }
-
+// `given` holds device pointers and, where known, element counts (0 = the count the
+// allocation holding the pointer has room for, from the pointer to the allocation's end).
+int ffi_launch(const ImbaSceneDesc& given, const ImbaRay* rays, ImbaHit* hits, int count, bool any, void* stream) {
     auto count_of = [](int given_count, const void* p, size_t rec) {
         if (given_count > 0) return given_count;
         return (int)std::min<size_t>(bytes_after(p) / rec, INT32_MAX);
     };
-    ImbaSceneDesc d{};
-    d.root = root;
-    d.node_count = count_of(given.node_count, nodes, sizeof(ImbaNode));
-    d.instance_count = count_of(given.instance_count, instances, sizeof(ImbaInstanceNode));
-    d.tri_vec4_count = count_of(given.tri_vec4_count, tris, sizeof(ImbaVec4));
-    d.index_count = count_of(given.index_count, indices, sizeof(int)) / 4 * 4;
-    d.texcoord_count = count_of(given.texcoord_count, texcoords, sizeof(ImbaVec2));
-    d.mask_count = count_of(given.mask_count, masks, sizeof(ImbaTransparencyMask));
-    d.mask_buffer_size = count_of(given.mask_buffer_size, mask_buffer, 1);
+    ImbaSceneDesc d = given;
+    d.node_count = count_of(given.node_count, given.nodes, sizeof(ImbaNode));
+    d.instance_count = count_of(given.instance_count, given.instances, sizeof(ImbaInstanceNode));
+    d.tri_vec4_count = count_of(given.tri_vec4_count, given.tris, sizeof(ImbaVec4));
+    d.index_count = count_of(given.index_count, given.indices, sizeof(int)) / 4 * 4;
+    d.texcoord_count = count_of(given.texcoord_count, given.texcoords, sizeof(ImbaVec2));
+    d.mask_count = count_of(given.mask_count, given.masks, sizeof(ImbaTransparencyMask));
+    d.mask_buffer_size = count_of(given.mask_buffer_size, given.mask_buffer, 1);
     if (d.node_count == 0 || d.instance_count == 0 || d.tri_vec4_count == 0)
         return fail(IMBA_ERR_INVALID_ARG, "scene arrays are not device allocations");
-
-    auto h_nodes = fetch<ImbaNode>(nodes, (size_t)d.node_count * sizeof(ImbaNode));
-    auto h_inst = fetch<ImbaInstanceNode>(instances, (size_t)d.instance_count * sizeof(ImbaInstanceNode));
-    auto h_tris = fetch<ImbaVec4>(tris, (size_t)d.tri_vec4_count * sizeof(ImbaVec4));
-    auto h_masks = fetch<ImbaTransparencyMask>(masks, (size_t)d.mask_count * sizeof(ImbaTransparencyMask));
-    auto h_mbuf = fetch<uint8_t>(mask_buffer, (size_t)d.mask_buffer_size);
-    ImbaSceneDesc h = d;
-    h.nodes = h_nodes.data();
-    h.instances = h_inst.data();
-    h.tris = h_tris.data();
-    h.masks = h_masks.data();
-    h.mask_buffer = h_mbuf.data();
-    FfiScene fs;
-    int r = validate_scene(h, fs.info);
+    if (d.root < 0 || d.root >= d.node_count) return fail(IMBA_ERR_INVALID_ARG, "root %d outside the node array", d.root);
+    int* err = nullptr;
+    int r = ffi_err_counter(&err);
     if (r) return r;
-    fs.dev = d;
-    fs.dev.nodes = (const ImbaNode*)nodes;
-    fs.dev.instances = (const ImbaInstanceNode*)instances;
-    fs.dev.tris = (const ImbaVec4*)tris;
-    fs.dev.indices = (const int*)indices;
-    fs.dev.texcoords = (const ImbaVec2*)texcoords;
-    fs.dev.masks = (const ImbaTransparencyMask*)masks;
-    fs.dev.mask_buffer = (const uint8_t*)mask_buffer;
-    size_t wide_bytes = 0;
-    r = upload_wide(h, fs.dev, fs.info, &fs.dev4, &fs.dev4q, &fs.info4, &wide_bytes);
-    if (r) return r;
-    IMBA_HIP_TRY(hipMalloc((void**)&fs.d_err, sizeof(int)));
-    IMBA_HIP_TRY(hipMemset(fs.d_err, 0, sizeof(int)));
-    auto ins = g_ffi_scenes.emplace(key, fs);
-    *out = &ins.first->second;
-    return IMBA_OK;
+    SceneInfo info;
+    info.max_stack = kFfiStack;
+    info.all_opaque = false;
+    const TreeView t{&d, &info, 0};
+    // the mask test runs unless the scene has no mask data at all (then every hit is opaque)
+    const bool masked = d.index_count > 0 && d.mask_count > 0 && d.mask_buffer_size > 0;
+    return launch_trace(t, err, rays, hits, nullptr, count, any, masked, nullptr, stream);
 }
 
 void ffi_trace(bool any, int root, void* nodes, void* instances, void* tris, void* rays, void* hits, int* indices,
                void* texcoords, void* masks, char* mask_buffer, int count) {
-    FfiScene* fs = nullptr;
     ImbaSceneDesc given{};
     given.root = root;
     given.nodes = (const ImbaNode*)nodes;
@@ -345,11 +320,7 @@ void ffi_trace(bool any, int root, void* nodes, void* instances, void* tris, voi
     given.texcoords = (const ImbaVec2*)texcoords;
     given.masks = (const ImbaTransparencyMask*)masks;
     given.mask_buffer = (const uint8_t*)mask_buffer;
-    int r = ffi_scene(given, &fs);
-    TreeView t{};
-    if (!r) r = ffi_tree(fs, nullptr, &t);
-    if (!r) r = launch_trace(t, fs->d_err, (const ImbaRay*)rays, (ImbaHit*)hits, nullptr, count, any,
-                             !fs->info.all_opaque, nullptr, nullptr);
+    const int r = ffi_launch(given, (const ImbaRay*)rays, (ImbaHit*)hits, count, any, nullptr);
     if (r) fprintf(stderr, "imbatracer_amd: %s_gpu_masked_instanced failed: %s\n", any ? "occluded" : "intersect",
                    imba_last_error());
 }
@@ -389,6 +360,9 @@ struct ImbaTraceBatch {
     int* d_group_chunks = nullptr;  // 4
     int* d_counters = nullptr;      // 2 per group
     imba::Scratch spill[4];
+    bool captured = false;  // launched into a graph: the tables may no longer move, and the
+                            // queues per launch group (kernel arguments) may no longer change
+    imba::BatchGroups captured_groups{};
 };
 
 namespace imba {
@@ -516,12 +490,22 @@ int set_tables(ImbaTraceBatch* b, const ImbaTraceItem* items, int n_items, hipSt
     }
     groups.first[4] = (int)table.size();
     const int n = (int)table.size();
+    if (b->captured && std::memcmp(&groups, &b->captured_groups, sizeof(groups)) != 0)
+        return fail(IMBA_ERR_INVALID_ARG, "batch was launched into a graph: the number of queues per kind "
+                    "(closest / any hit, masked / opaque) must stay as captured");
     const size_t bytes = sizeof(BatchItem) * (size_t)n;
     if (n > b->cap) {
-        // grow: device tables and staging buffers (synchronous)
+        // grow: device tables and staging buffers (synchronous).  A graph replays its launches
+        // with the table addresses they were recorded with, so a captured batch cannot grow.
+        if (b->captured)
+            return fail(IMBA_ERR_CAPACITY, "batch was launched into a graph with room for %d queues; %d do not fit "
+                        "(create the batch with its largest queue set before capturing)", b->cap, n);
         IMBA_HIP_TRY(hipDeviceSynchronize());
         for (int k = 0; k < 3; k++) {
-            if (b->d_items[k]) IMBA_HIP_TRY(hipFree(b->d_items[k]));
+            if (b->d_items[k]) {
+                int r = retire_buffer(b->d_items[k]);
+                if (r) return r;
+            }
             b->d_items[k] = nullptr;
         }
         for (int k = 0; k < 2; k++) {
@@ -625,6 +609,10 @@ int imba_trace_batch_launch(ImbaTraceBatch* b, const ImbaTraceConfig* cfg, void*
     }
     bool sync;
     hipStream_t s = resolve_stream(stream, &sync);
+    if (is_capturing(s)) {
+        b->captured = true;
+        b->captured_groups = b->groups;
+    }
     hipLaunchKernelGGL(k_batch_prefix, dim3(1), dim3(1024), 0, s, b->d_items[tree], b->groups, b->d_group_chunks);
     IMBA_HIP_TRY(hipGetLastError());
     for (int g = 0; g < 4; g++) {
@@ -765,13 +753,16 @@ int imba_trace_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uin
 int imba_trace_desc(const ImbaSceneDesc* dev, const ImbaRay* rays, ImbaHit* hits, int count, int any_hit,
                     void* stream) {
     IMBA_CHECK_ARG(dev, "desc is NULL");
-    FfiScene* fs = nullptr;
-    int r = ffi_scene(*dev, &fs);
+    return ffi_launch(*dev, rays, hits, count, any_hit != 0, stream);
+}
+
+int imba_ffi_error_count(int* count) {
+    IMBA_CHECK_ARG(count, "count is NULL");
+    int* err = nullptr;
+    int r = ffi_err_counter(&err);
     if (r) return r;
-    TreeView t{};
-    r = ffi_tree(fs, nullptr, &t);
-    if (r) return r;
-    return launch_trace(t, fs->d_err, rays, hits, nullptr, count, any_hit != 0, !fs->info.all_opaque, nullptr, stream);
+    IMBA_HIP_TRY(hipMemcpy(count, err, sizeof(int), hipMemcpyDeviceToHost));
+    return IMBA_OK;
 }
 
 // Reference FFI (render/scheduling/ray_queue.h:297-308, 347-358): synchronous, device pointers.

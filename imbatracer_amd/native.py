@@ -122,6 +122,7 @@ _sig("imba_last_error", C.c_char_p)
 _sig("imba_version", C.c_char_p)
 _sig("imba_device_count", i32, ip)
 _sig("imba_set_device", i32, i32)
+_sig("imba_get_device", i32, ip)
 _sig("imba_device_info", i32, ip, ip, ip, C.c_char_p, i32)
 _sig("imba_malloc", i32, C.POINTER(vp), sz)
 _sig("imba_free", i32, vp)
@@ -158,6 +159,7 @@ _sig("imba_bvh_emit_boxes", i32, vp, vp, i32, i32, i32, vp, sz, C.POINTER(sz))
 _sig("imba_scene_validate", i32, C.POINTER(SceneDesc), ip, ip)
 _sig("imba_scene_upload", i32, C.POINTER(SceneDesc), C.POINTER(vp))
 _sig("imba_trace_desc", i32, C.POINTER(SceneDesc), vp, vp, i32, i32, vp)
+_sig("imba_ffi_error_count", i32, ip)
 _sig("imba_scene_release", i32, vp)
 _sig("imba_scene_device_desc", i32, vp, C.POINTER(SceneDesc))
 _sig("imba_scene_footprint", i32, vp, C.POINTER(sz))

@@ -12,7 +12,10 @@
 //     traversal; states are uploaded), keep the hits in order and put the misses after them
 //     in order (the reference permutes the misses, which callers drop with shrink());
 //   - HashGrid builds and queries on the GPU; query() is one GPU call per position (kept for
-//     source compatibility), query_batch() is the form to use from a bounce loop.
+//     source compatibility; reentrant, as the reference's shading threads need), query_batch()
+//     is the form to use from a bounce loop;
+//   - traverse_cpu / traverse_occluded_cpu are declared but throw: the library has no CPU
+//     traversal.
 // Errors throw imba::Error (the reference asserts or does nothing).
 #ifndef IMBA_RAY_QUEUE_HPP
 #define IMBA_RAY_QUEUE_HPP
@@ -23,6 +26,7 @@
 #include <cstring>
 #include <functional>
 #include <iterator>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -34,6 +38,15 @@ namespace imba {
 struct Error : std::runtime_error {
     int code;
     Error(int c, const std::string& where) : std::runtime_error(where + ": " + imba_last_error()), code(c) {}
+
+protected:
+    struct Plain {};
+    Error(int c, const std::string& msg, Plain) : std::runtime_error(msg), code(c) {}
+};
+
+// A member of the reference interface this library does not provide.
+struct Unsupported : Error {
+    explicit Unsupported(const std::string& msg) : Error(IMBA_ERR_INVALID_ARG, msg, Plain()) {}
 };
 
 inline void check(int code, const char* where) {
@@ -137,6 +150,33 @@ public:
     }
     RayQueue(const RayQueue&) = delete;
     RayQueue& operator=(const RayQueue&) = delete;
+
+    // ray_queue.h:84-102 (the device buffers move with the queue as well).
+    RayQueue(RayQueue&& rhs)
+        : ray_buffer_(std::move(rhs.ray_buffer_)), hit_buffer_(std::move(rhs.hit_buffer_)),
+          state_buffer_(std::move(rhs.state_buffer_)), sorted_indices_(std::move(rhs.sorted_indices_)),
+          last_(rhs.last_.load()), gpu_buffers_(rhs.gpu_buffers_), dev_rays_(std::move(rhs.dev_rays_)),
+          dev_rays2_(std::move(rhs.dev_rays2_)), dev_hits_(std::move(rhs.dev_hits_)),
+          dev_hits2_(std::move(rhs.dev_hits2_)), dev_states_(std::move(rhs.dev_states_)),
+          dev_states2_(std::move(rhs.dev_states2_)) {
+        rhs.last_ = -1;
+    }
+    RayQueue& operator=(RayQueue&& rhs) {
+        ray_buffer_ = std::move(rhs.ray_buffer_);
+        hit_buffer_ = std::move(rhs.hit_buffer_);
+        state_buffer_ = std::move(rhs.state_buffer_);
+        sorted_indices_ = std::move(rhs.sorted_indices_);
+        last_ = rhs.last_.load();
+        gpu_buffers_ = rhs.gpu_buffers_;
+        dev_rays_ = std::move(rhs.dev_rays_);
+        dev_rays2_ = std::move(rhs.dev_rays2_);
+        dev_hits_ = std::move(rhs.dev_hits_);
+        dev_hits2_ = std::move(rhs.dev_hits2_);
+        dev_states_ = std::move(rhs.dev_states_);
+        dev_states2_ = std::move(rhs.dev_states2_);
+        rhs.last_ = -1;
+        return *this;
+    }
 
     int size() const { return last_ + 1; }
     int capacity() const { return (int)state_buffer_.size(); }
@@ -245,6 +285,19 @@ public:
         order.download(sorted_indices_.data(), count);
     }
 
+    // The reference's CPU traversal (ray_queue.h:247-266, 296-316: the AnyDSL CPU kernels) is
+    // not part of this library, which traverses on the GPU only.  The members are declared so
+    // that a caller written against the reference compiles; calling them throws
+    // imba::Unsupported (an imba::Error, code IMBA_ERR_INVALID_ARG).
+    template <typename Node>
+    void traverse_cpu(const TraversalData<Node>&) {
+        throw Unsupported("RayQueue::traverse_cpu is not supported (GPU traversal only; use traverse_gpu)");
+    }
+    template <typename Node>
+    void traverse_occluded_cpu(const TraversalData<Node>&) {
+        throw Unsupported("RayQueue::traverse_occluded_cpu is not supported (GPU traversal only; use traverse_occluded_gpu)");
+    }
+
     // Traverses all rays of the queue on the GPU (closest hit).
     template <typename Node>
     void traverse_gpu(const TraversalData<Node>& c_data) {
@@ -315,9 +368,75 @@ private:
     DeviceArray<char> dev_states_, dev_states2_;
 };
 
+namespace detail {
+
+// Device and host buffers of one host thread's grid queries.  query() is const and the
+// integrators call it from many shading threads at once (vcm.cpp:370 -> vertex_merging ->
+// get_merge), so every thread queries through buffers of its own, on its own stream (the
+// library's per-thread stream), and nothing of the grid object is written.  The buffers are
+// allocated on a thread's first query on a device (or when a query asks for more results than
+// they hold), not on every call, and freed when the thread ends.
+struct QueryScratch {
+    size_t cap = 0;          // positions the buffers hold
+    int k_cap = 0;           // results per position they hold
+    float* dq = nullptr;     // device: 3 floats per position
+    int* di = nullptr;       // device: k_cap ints per position
+    int* dc = nullptr;       // device: 1 int per position
+    std::vector<int> host_idx;
+    std::vector<int> host_cnt;
+
+    QueryScratch() {}
+    QueryScratch(const QueryScratch&) = delete;
+    QueryScratch& operator=(const QueryScratch&) = delete;
+    ~QueryScratch() { release(); }
+
+    void release() {
+        if (dq) imba_free(dq);
+        if (di) imba_free(di);
+        if (dc) imba_free(dc);
+        dq = nullptr;
+        di = nullptr;
+        dc = nullptr;
+        cap = 0;
+        k_cap = 0;
+    }
+
+    // Buffers for nq positions with k results each (on the device this scratch belongs to,
+    // which is current).
+    void reserve(size_t nq, int k) {
+        if (nq <= cap && k <= k_cap) return;
+        const size_t n = std::max(nq, cap);
+        const int kk = std::max(k, k_cap);
+        release();
+        void* p = nullptr;
+        check(imba_malloc(&p, 3 * sizeof(float) * n), "imba_malloc");
+        dq = static_cast<float*>(p);
+        check(imba_malloc(&p, sizeof(int) * n * (size_t)kk), "imba_malloc");
+        di = static_cast<int*>(p);
+        check(imba_malloc(&p, sizeof(int) * n), "imba_malloc");
+        dc = static_cast<int*>(p);
+        cap = n;
+        k_cap = kk;
+    }
+};
+
+// The calling thread's scratch on the current device.
+inline QueryScratch& query_scratch() {
+    thread_local std::vector<std::unique_ptr<QueryScratch>> per_device;
+    int dev = 0;
+    check(imba_get_device(&dev), "imba_get_device");
+    if ((size_t)dev >= per_device.size()) per_device.resize((size_t)dev + 1);
+    if (!per_device[dev]) per_device[dev].reset(new QueryScratch());
+    return *per_device[dev];
+}
+
+}  // namespace detail
+
 // Photon hash grid (rangesearch.h:21-158).  Photon must be trivially copyable with a public
 // `float3`-like position of three floats at offset 0 as `position` (VCMPhoton) or returned
 // by `position()` (LightPathVertex) — the grid reads 3 floats at the start of each record.
+// build() must not overlap with queries; queries may run concurrently from any number of
+// host threads (each uses buffers of its own, see detail::QueryScratch).
 template <typename Iter, typename Photon>
 class HashGrid {
 public:
@@ -339,38 +458,51 @@ public:
     }
 
     // Up to k photons within the radius of query_pos, nearest first (out[i] = const Photon*).
+    // Reentrant: concurrent calls from several threads are allowed.
     template <typename Container, typename Float3>
     int query(const Float3& query_pos, Container& out, int k) const {
+        if (k <= 0) return 0;
         const float q[3] = {query_pos.x, query_pos.y, query_pos.z};
-        std::vector<int> idx;
-        const int n = query_batch(q, 1, k, idx);
-        for (int i = 0; i < n; i++) out[i] = &photons_[idx[i]];
+        detail::QueryScratch& sc = detail::query_scratch();
+        run_queries(sc, q, 1, k);
+        const int n = sc.host_cnt[0];
+        for (int i = 0; i < n; i++) out[i] = &photons_[sc.host_idx[i]];
         return n;
     }
 
     // nq positions (3 floats each, host) -> per query up to k photon indices (into the build
-    // input) in idx[q * k ...]; returns the count of the last query (all counts in counts()).
-    int query_batch(const float* positions, int nq, int k, std::vector<int>& idx) const {
-        DeviceArray<float> dq(3 * (size_t)std::max(nq, 1));
-        DeviceArray<int> di((size_t)std::max(nq, 1) * k), dc(std::max(nq, 1));
-        dq.upload(positions, 3 * (size_t)nq);
-        check(imba_hashgrid_query(grid_, dq.data(), 12, 0, nq, k, di.data(), nullptr, dc.data(), nullptr),
-              "imba_hashgrid_query");
-        idx.resize((size_t)nq * k);
-        counts_.resize(nq);
-        di.download(idx.data(), (size_t)nq * k);
-        dc.download(counts_.data(), nq);
-        return nq ? counts_.back() : 0;
+    // input) in idx[q * k ...] and the number found in counts[q].  Reentrant.
+    void query_batch(const float* positions, int nq, int k, std::vector<int>& idx, std::vector<int>& counts) const {
+        idx.assign((size_t)std::max(nq, 0) * std::max(k, 0), -1);
+        counts.assign((size_t)std::max(nq, 0), 0);
+        if (nq <= 0 || k <= 0) return;
+        detail::QueryScratch& sc = detail::query_scratch();
+        run_queries(sc, positions, nq, k);
+        for (int q = 0; q < nq; q++) {
+            counts[q] = sc.host_cnt[q];
+            std::copy(sc.host_idx.begin() + (size_t)q * k, sc.host_idx.begin() + (size_t)q * k + counts[q],
+                      idx.begin() + (size_t)q * k);
+        }
     }
-    const std::vector<int>& counts() const { return counts_; }
 
 private:
+    // Runs nq queries on the calling thread's stream through its scratch; results end in
+    // sc.host_idx (k per query) and sc.host_cnt.
+    void run_queries(detail::QueryScratch& sc, const float* positions, int nq, int k) const {
+        sc.reserve((size_t)nq, k);
+        check(imba_memcpy_h2d(sc.dq, positions, 3 * sizeof(float) * (size_t)nq, nullptr), "imba_memcpy_h2d");
+        check(imba_hashgrid_query(grid_, sc.dq, 12, 0, nq, k, sc.di, nullptr, sc.dc, nullptr), "imba_hashgrid_query");
+        sc.host_idx.resize((size_t)nq * k);
+        sc.host_cnt.resize((size_t)nq);
+        check(imba_memcpy_d2h(sc.host_idx.data(), sc.di, sizeof(int) * (size_t)nq * k, nullptr), "imba_memcpy_d2h");
+        check(imba_memcpy_d2h(sc.host_cnt.data(), sc.dc, sizeof(int) * (size_t)nq, nullptr), "imba_memcpy_d2h");
+    }
+
     typedef typename std::iterator_traits<Iter>::value_type SrcT;
     ImbaHashGrid* grid_ = nullptr;
     std::vector<SrcT> src_;
     std::vector<Photon> photons_;
     DeviceArray<char> dev_src_;
-    mutable std::vector<int> counts_;
 };
 
 }  // namespace imba

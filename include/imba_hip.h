@@ -91,6 +91,7 @@ const char* imba_version(void);
  * (ray_queue.h:77-80,277,292; scene.cpp:14-27,121-151). */
 int imba_device_count(int* count);
 int imba_set_device(int device);
+int imba_get_device(int* device);   /* the calling thread's current device */
 int imba_device_info(int* cu_count, int* lds_bytes_per_block, int* l2_bytes, char* name, int name_len);
 int imba_malloc(void** ptr, size_t bytes);
 int imba_free(void* ptr);
@@ -309,11 +310,18 @@ int imba_occluded_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, 
 int imba_trace_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts, int count,
                      int any_hit, const ImbaTraceConfig* cfg, void* stream);
 /* Traversal over caller-owned device arrays described by `dev` (device pointers and
- * element counts), for callers that manage their own TraversalData.  The arrays are copied
- * to the host and validated once per distinct set of pointers and counts.  any_hit: 0 closest
- * hit, 1 occluded. */
+ * element counts; a count of 0 means "as many records as the allocation holding the pointer
+ * has room for"), for callers that manage their own TraversalData.  Nothing derived from the
+ * arrays is kept between calls, so the caller may rewrite or reallocate them at any time:
+ * every call traverses the BVH2 as given, with the mask test on (unless indices, masks or
+ * mask_buffer are empty), a stack of 64 entries per ray and every index bounds-checked.
+ * any_hit: 0 closest hit, 1 occluded.  This is also the path of the reference FFI entry points
+ * (traversal_gpu.h). */
 int imba_trace_desc(const ImbaSceneDesc* dev, const ImbaRay* rays, ImbaHit* hits, int count, int any_hit,
                     void* stream);
+/* Rays of imba_trace_desc / FFI calls on the current device that met a malformed structure or
+ * needed more than 64 stack entries (cumulative). */
+int imba_ffi_error_count(int* count);
 
 /* Batched traversal of many queues (the loop of RayQueue::traverse_gpu / traverse_occluded_gpu
  * calls that QueueScheduler::process_queues and the TileScheduler threads issue,

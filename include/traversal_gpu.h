@@ -14,7 +14,10 @@
  * The two functions are implemented by libimba_hip.so (HIP, gfx950).  As in the reference
  * (render/scheduling/ray_queue.h:297-308, 347-358) every pointer is device memory, `count` rays
  * are traversed, and the call returns when the hits are written.  Errors are reported on
- * stderr (the reference's functions return nothing).
+ * stderr (the reference's functions return nothing).  Array sizes are those of the allocations
+ * holding the pointers at the time of the call, and nothing derived from the arrays is kept
+ * between calls, so the caller may rewrite or reallocate them between calls (imba_trace_desc
+ * in imba_hip.h describes the traversal).
  */
 #ifndef IMBA_TRAVERSAL_GPU_H
 #define IMBA_TRAVERSAL_GPU_H

@@ -3,14 +3,17 @@
 // light_vertices.h:70-100) and checks every result against the CPU oracle.
 // Built by __graft_entry__.build() into tests/cpp/ray_queue_test.bin; run by
 // tests/test_gpu_cpp_interface.py on the GPU.  Exit status 0 = all checks passed.
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <random>
+#include <thread>
 #include <vector>
 
 #include "imba/ray_queue.hpp"
+#include "traversal_gpu.h"
 #include "../../oracle/oracle.h"
 
 namespace {
@@ -45,6 +48,7 @@ int failures = 0;
     } while (0)
 
 bool same_hit(const ImbaHit& a, const ImbaHit& b) { return std::memcmp(&a, &b, sizeof(ImbaHit)) == 0; }
+
 
 }  // namespace
 
@@ -165,15 +169,18 @@ int main() {
     const int k = 10, nq = 500;
     std::vector<float> qpos(3 * nq);
     for (float& c : qpos) c = U(rng);
-    std::vector<int> got;
-    grid.query_batch(qpos.data(), nq, k, got);
+    std::vector<int> got, got_counts;
+    grid.query_batch(qpos.data(), nq, k, got, got_counts);
     int nonempty = 0;
+    std::vector<int> want_idx((size_t)nq * k, -1), want_cnt(nq);
     for (int qi = 0; qi < nq; qi++) {
         int oi[64];
         float od[64];
         const int m = oracle_grid_query(&og, &qpos[3 * qi], k, oi, od);
-        EXPECT(grid.counts()[qi] == m, "query %d count %d, oracle %d", qi, grid.counts()[qi], m);
-        for (int j = 0; j < m && j < grid.counts()[qi]; j++) EXPECT(got[qi * k + j] == oi[j], "query %d slot %d", qi, j);
+        want_cnt[qi] = m;
+        for (int j = 0; j < m; j++) want_idx[(size_t)qi * k + j] = oi[j];
+        EXPECT(got_counts[qi] == m, "query %d count %d, oracle %d", qi, got_counts[qi], m);
+        for (int j = 0; j < m && j < got_counts[qi]; j++) EXPECT(got[qi * k + j] == oi[j], "query %d slot %d", qi, j);
         nonempty += m > 0;
     }
     EXPECT(nonempty > 50, "too few non-empty queries (%d)", nonempty);
@@ -181,10 +188,186 @@ int main() {
     const Photon* out[16];
     const F3 p0{qpos[0], qpos[1], qpos[2]};
     const int m0 = grid.query(p0, out, k);
-    EXPECT(m0 == grid.counts()[0] || nq == 0, "query() count");
+    EXPECT(m0 == got_counts[0] || nq == 0, "query() count");
     for (int j = 0; j < m0; j++)
         EXPECT(std::memcmp(out[j]->position, photons[got[j]].pos, 12) == 0, "query() photon %d", j);
+
+    // query() from 8 threads at once (vertex_merging runs on the TBB shading threads,
+    // vcm.cpp:370-380): every thread's results equal the oracle's.  Results point into the
+    // grid's own photon array, whose start follows from the first query's results.
+    {
+        const Photon* base = m0 > 0 ? out[0] - got[0] : nullptr;
+        std::atomic<int> bad(0);
+        std::vector<std::thread> pool;
+        for (int t = 0; t < 8; t++) {
+            pool.emplace_back([&, t]() {
+                try {
+                    for (int rep = 0; rep < 3; rep++) {
+                        for (int qi = t; qi < nq; qi += 8) {
+                            const Photon* o[16];
+                            const F3 p{qpos[3 * qi], qpos[3 * qi + 1], qpos[3 * qi + 2]};
+                            const int m = grid.query(p, o, k);
+                            if (m != want_cnt[qi]) {
+                                bad++;
+                                continue;
+                            }
+                            for (int j = 0; j < m; j++) {
+                                const int want_j = want_idx[(size_t)qi * k + j];
+                                if (std::memcmp(o[j]->position, photons[want_j].pos, 12) != 0 ||
+                                    (base && o[j] != base + want_j))
+                                    bad++;
+                            }
+                        }
+                    }
+                } catch (const std::exception& e) {
+                    std::printf("thread %d: %s\n", t, e.what());
+                    bad++;
+                }
+            });
+        }
+        for (auto& th : pool) th.join();
+        EXPECT(bad.load() == 0, "concurrent HashGrid::query: %d mismatches", bad.load());
+    }
     oracle_grid_free(&og);
+
+    // four RayQueues traversed from four threads at once, as TileScheduler's render threads do
+    // (tile_scheduler.h:69-78,120-132): each queue's hits equal the oracle's
+    {
+        std::atomic<int> bad(0);
+        std::vector<std::thread> pool;
+        for (int t = 0; t < 4; t++) {
+            pool.emplace_back([&, t]() {
+                try {
+                    imba::RayQueue<State> tq(w * h, true);
+                    for (int i = t; i < w * h; i += 4) tq.push(cam[i], State{i, 0, 0});
+                    for (int rep = 0; rep < 3; rep++) {
+                        tq.traverse_gpu(data);
+                        for (int j = 0; j < tq.size(); j++)
+                            if (!same_hit(tq.hits()[j], want[tq.states()[j].pixel_id])) bad++;
+                        tq.traverse_occluded_gpu(data);
+                    }
+                } catch (const std::exception& e) {
+                    std::printf("thread %d: %s\n", t, e.what());
+                    bad++;
+                }
+            });
+        }
+        for (auto& th : pool) th.join();
+        EXPECT(bad.load() == 0, "concurrent traverse_gpu: %d mismatches", bad.load());
+    }
+
+    // a moved queue keeps its rays, states and device buffers
+    {
+        imba::RayQueue<State> a(256, true);
+        for (int i = 0; i < 256; i++) a.push(cam[i], State{i, 0, 0});
+        imba::RayQueue<State> moved(std::move(a));
+        EXPECT(moved.size() == 256 && a.size() == 0, "move: sizes %d / %d", moved.size(), a.size());
+        moved.traverse_gpu(data);
+        for (int i = 0; i < 256; i++) EXPECT(same_hit(moved.hits()[i], want[i]), "moved queue hit %d", i);
+        imba::RayQueue<State> assigned;
+        assigned = std::move(moved);
+        assigned.traverse_gpu(data);
+        for (int i = 0; i < 256; i++) EXPECT(same_hit(assigned.hits()[i], want[i]), "move-assigned queue hit %d", i);
+        bool threw = false;
+        try {
+            assigned.traverse_cpu(data);
+        } catch (const imba::Unsupported&) {
+            threw = true;
+        }
+        EXPECT(threw, "traverse_cpu must throw imba::Unsupported");
+    }
+
+    // reference FFI over caller-owned arrays that are rewritten in place and reallocated
+    // between calls (Scene::upload_top_level_accel writes into the arrays it has,
+    // scene.cpp:166-188): every call's hits equal the oracle's for what the arrays hold then
+    {
+        std::vector<float> verts2(12 * ntri);
+        std::vector<uint32_t> idx2(4 * ntri);
+        imba::check(imba_gen_random_triangles(0xBEEF, ntri, verts2.data(), idx2.data()), "gen");
+        for (int t = 0; t < ntri; t++) idx2[4 * t + 3] = t % 3;
+        ImbaSceneBuild* b2 = nullptr;
+        imba::check(imba_build_create(&b2), "build_create");
+        ImbaMeshDesc md2{verts2.data(), 3 * ntri, idx2.data(), ntri, nullptr};
+        imba::check(imba_build_add_mesh(b2, &md2, nullptr), "add_mesh");
+        // a second instance (translated), so the top level differs too
+        const float shift[16] = {1, 0, 0, 0.25f, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+        imba::check(imba_build_add_instance(b2, 0, shift, nullptr), "add_instance");
+        imba::check(imba_build_run(b2, IMBA_BUILDER_BINNED_SAH, 2), "build_run");
+        ImbaSceneDesc host2;
+        imba::check(imba_build_get_desc(b2, &host2), "get_desc");
+        const ImbaSceneDesc* scenes[2] = {&host, &host2};
+        std::vector<ImbaHit> ffi_want[2];
+        for (int k2 = 0; k2 < 2; k2++) {
+            ffi_want[k2].resize(w * h);
+            oracle_trace(scenes[k2], cam.data(), w * h, 0, ffi_want[k2].data(), nullptr);
+        }
+        struct Arrays {
+            void *nodes = nullptr, *inst = nullptr, *tris = nullptr, *indices = nullptr, *tex = nullptr,
+                 *masks = nullptr, *mbuf = nullptr;
+        };
+        auto alloc = [](Arrays& a, const ImbaSceneDesc& d0, const ImbaSceneDesc& d1) {
+            auto mx = [](int x, int y) { return (size_t)std::max(std::max(x, y), 1); };
+            imba::check(imba_malloc(&a.nodes, sizeof(ImbaNode) * mx(d0.node_count, d1.node_count)), "malloc");
+            imba::check(imba_malloc(&a.inst, sizeof(ImbaInstanceNode) * mx(d0.instance_count, d1.instance_count)), "malloc");
+            imba::check(imba_malloc(&a.tris, sizeof(ImbaVec4) * mx(d0.tri_vec4_count, d1.tri_vec4_count)), "malloc");
+            imba::check(imba_malloc(&a.indices, sizeof(int) * mx(d0.index_count, d1.index_count)), "malloc");
+            imba::check(imba_malloc(&a.tex, sizeof(ImbaVec2) * mx(d0.texcoord_count, d1.texcoord_count)), "malloc");
+            imba::check(imba_malloc(&a.masks, sizeof(ImbaTransparencyMask) * mx(d0.mask_count, d1.mask_count)), "malloc");
+            imba::check(imba_malloc(&a.mbuf, mx(d0.mask_buffer_size, d1.mask_buffer_size)), "malloc");
+        };
+        auto release = [](Arrays& a) {
+            for (void* p : {a.nodes, a.inst, a.tris, a.indices, a.tex, a.masks, a.mbuf}) imba_free(p);
+            a = Arrays();
+        };
+        auto upload = [](Arrays& a, const ImbaSceneDesc& d) {
+            imba::check(imba_memcpy_h2d(a.nodes, d.nodes, sizeof(ImbaNode) * d.node_count, nullptr), "h2d");
+            imba::check(imba_memcpy_h2d(a.inst, d.instances, sizeof(ImbaInstanceNode) * d.instance_count, nullptr), "h2d");
+            imba::check(imba_memcpy_h2d(a.tris, d.tris, sizeof(ImbaVec4) * d.tri_vec4_count, nullptr), "h2d");
+            if (d.index_count) imba::check(imba_memcpy_h2d(a.indices, d.indices, sizeof(int) * d.index_count, nullptr), "h2d");
+            if (d.texcoord_count)
+                imba::check(imba_memcpy_h2d(a.tex, d.texcoords, sizeof(ImbaVec2) * d.texcoord_count, nullptr), "h2d");
+            if (d.mask_count)
+                imba::check(imba_memcpy_h2d(a.masks, d.masks, sizeof(ImbaTransparencyMask) * d.mask_count, nullptr), "h2d");
+            if (d.mask_buffer_size) imba::check(imba_memcpy_h2d(a.mbuf, d.mask_buffer, d.mask_buffer_size, nullptr), "h2d");
+        };
+        void *drays = nullptr, *dhits = nullptr;
+        imba::check(imba_malloc(&drays, sizeof(ImbaRay) * w * h), "malloc");
+        imba::check(imba_malloc(&dhits, sizeof(ImbaHit) * w * h), "malloc");
+        imba::check(imba_memcpy_h2d(drays, cam.data(), sizeof(ImbaRay) * w * h, nullptr), "h2d");
+        std::vector<ImbaHit> got_h(w * h);
+        auto ffi_run = [&](const Arrays& a, const ImbaSceneDesc& d, int which, const char* what) {
+            intersect_gpu_masked_instanced(d.root, (Node*)a.nodes, (InstanceNode*)a.inst, (Vec4*)a.tris, (Ray*)drays,
+                                           (Hit*)dhits, (int*)a.indices, (Vec2*)a.tex, (TransparencyMask*)a.masks,
+                                           (char*)a.mbuf, w * h);
+            imba::check(imba_memcpy_d2h(got_h.data(), dhits, sizeof(ImbaHit) * w * h, nullptr), "d2h");
+            int diff = 0;
+            for (int i = 0; i < w * h; i++) diff += !same_hit(got_h[i], ffi_want[which][i]);
+            EXPECT(diff == 0, "FFI (%s): %d hits differ from the oracle", what, diff);
+        };
+        int err0 = 0;
+        imba::check(imba_ffi_error_count(&err0), "ffi_error_count");
+        Arrays a;
+        alloc(a, host, host2);
+        upload(a, host);
+        ffi_run(a, host, 0, "scene A");
+        upload(a, host2);  // same allocations, new contents
+        ffi_run(a, host2, 1, "scene B rewritten in place over A");
+        upload(a, host);
+        ffi_run(a, host, 0, "scene A rewritten in place over B");
+        const void* old_nodes = a.nodes;
+        release(a);
+        alloc(a, host2, host2);  // freed and reallocated (often at the same addresses)
+        upload(a, host2);
+        ffi_run(a, host2, 1, "scene B after free and reallocation");
+        std::printf("FFI reallocation reused the node array address: %s\n", a.nodes == old_nodes ? "yes" : "no");
+        release(a);
+        int err1 = 0;
+        imba::check(imba_ffi_error_count(&err1), "ffi_error_count");
+        EXPECT(err1 == err0, "FFI calls reported %d malformed rays", err1 - err0);
+        imba_free(drays);
+        imba_free(dhits);
+        imba_build_release(b2);
+    }
     imba_build_release(b);
 
     if (failures) {
