@@ -138,6 +138,44 @@ def read_bandwidths(stream):
     return out
 
 
+def shadow_tune(pt, frames=3):
+    """Launch configuration of the shadow-ray (any-hit) traversal, per bounce: the library's
+    autotuner on the first bounces' shadow slots plus a few plain launches, each timed over
+    whole frames; the tree is the fastest candidate's, and every bounce takes the fastest
+    candidate on that tree."""
+    from imbatracer_amd import native as N
+
+    pt.frame(keep_queues=8)
+    shadows = [sr for sr, _, _ in pt.kept_shadow]
+    cands = [autotune(pt.gpu_scene, shadows[b], any_hit=True) for b in sorted({0, len(shadows) // 2})
+             if b < len(shadows) and len(shadows[b])]
+    cands += [N.TraceConfig(256, 16, 0, 0, 4, 1, 0), N.TraceConfig(256, 16, 0, 0, 2, 0, 0),
+              N.TraceConfig(128, 16, 0, 0, 4, 1, 0), N.TraceConfig(64, 32, 0, 0, 4, 1, 0)]
+    seen, timed = set(), []
+    for c in cands:
+        key = tuple(getattr(c, f) for f, _ in c._fields_)
+        if key in seen:
+            continue
+        seen.add(key)
+        pt.shadow_cfg, pt.shadow_bounce_cfg = c, None
+        pt.frame()
+        per_bounce = None
+        for _ in range(frames):
+            pt.frame(timed=True)
+            pt.stream.sync()
+            t = [ms for ms, _ in pt.shadow_times_ms()]
+            per_bounce = t if per_bounce is None else [x + y for x, y in zip(per_bounce, t)]
+        log("shadow tune: %s %.3f ms of shadow traversal per frame" % (cfg_dict(c), sum(per_bounce) / frames))
+        timed.append((sum(per_bounce), c, per_bounce))
+    best = min(timed, key=lambda x: x[0])
+    tree = (best[1].bvh_width, best[1].quantized)
+    same = [x for x in timed if (x[1].bvh_width, x[1].quantized) == tree]
+    bounce_cfg = [min(same, key=lambda x: x[2][b])[1] for b in range(len(best[2]))]
+    pt.shadow_cfg, pt.shadow_bounce_cfg = best[1], bounce_cfg
+    log("shadow tune: per bounce %s" % [cfg_dict(c) for c in bounce_cfg])
+    return best[1], bounce_cfg
+
+
 def frame_tune(pt, queues, frames=3):
     """Per-scene launch configuration for whole frames: the library's autotuner on queues of
     three sizes (a frame's queues shrink from bounce to bounce), each distinct winner and a few
@@ -179,18 +217,76 @@ def frame_tune(pt, queues, frames=3):
     return best[1], bounce_cfg
 
 
-def cpu_baseline(build, builder, threads_build, ray_sets, any_flags, budget_s=12.0):
+def cgroup_cpus():
+    """The CPU quota of this process's control group in cores (None if unlimited or unknown)."""
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        if quota != "max":
+            return float(quota) / float(period)
+    except (OSError, ValueError):
+        pass
+    try:
+        with open("/sys/fs/cgroup/cpu/cpu.cfs_quota_us") as f:
+            quota = float(f.read())
+        with open("/sys/fs/cgroup/cpu/cpu.cfs_period_us") as f:
+            period = float(f.read())
+        if quota > 0:
+            return quota / period
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+def affinity_cores() -> int:
+    try:
+        return max(1, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        return max(1, os.cpu_count() or 1)
+
+
+def host_cores() -> int:
+    """Host cores this process may run on: the affinity mask, capped by the control group's CPU
+    quota when there is one.  TileScheduler starts one render thread per hardware thread
+    (render/scheduling/tile_scheduler.h:69-78); more threads than the quota allows would only
+    queue for it."""
+    n = affinity_cores()
+    q = cgroup_cpus()
+    if q is not None:
+        n = min(n, max(1, int(q + 0.5)))
+    return n
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def sample_queues(queues, per_set: int):
+    """An every-k-th-ray sample of each queue with the same k for all, so the sample keeps the
+    queues' proportions (about per_set rays from the largest one)."""
+    k = max(1, max(len(q) for q in queues) // max(per_set, 1))
+    return [np.ascontiguousarray(q[::k]) for q in queues if len(q[::k])]
+
+
+def cpu_baseline(build, builder, threads_build, ray_sets, any_flags, budget_s=12.0, note=""):
     """Times the reference's CPU path restated (oracle/cpu_bvh4.c: BVH4 nodes + 4-triangle
     packets of core/cpu_adapter.cpp, 4-wide SSE/FMA box and triangle tests, one ray per host
-    thread at a time) on the host cores over a bounded sample of the benchmarked rays: one
-    untimed pass, then whole passes until the time budget is used."""
+    thread at a time) on every host core the process may use over a bounded sample of the
+    benchmarked rays: one untimed pass, then whole passes until the time budget is used."""
     import oracle
 
     t0 = time.perf_counter()
     build.cpu_layout(builder, threads_build)
     t_layout = time.perf_counter() - t0
     desc = build.cpu_desc()
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_cores()
     od = oracle.SceneDesc(*[getattr(desc, f) for f, _ in oracle.SceneDesc._fields_])
     L = oracle.lib()
     sets = [(np.ascontiguousarray(r), a) for r, a in zip(ray_sets, any_flags)]
@@ -209,15 +305,17 @@ def cpu_baseline(build, builder, threads_build, ray_sets, any_flags, budget_s=12
     one_pass()
     n_pass = sum(len(r) for r, _ in sets)
     times = []
-    while sum(times) < budget_s and len(times) < 100:
+    while sum(times) < budget_s and len(times) < 1000:
         times.append(one_pass())
     elapsed = sum(times)
     return {"value": n_pass * len(times) / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": "%d timed passes (after 1 warm-up) over %s rays of the benchmarked queues, %s; "
-                      "oracle/cpu_bvh4.c (the reference CPU layout: BVH4 + 4-triangle packets, 4-wide SSE/FMA) "
-                      "on %d host threads, %.1f s; median pass %.1f Mrays/s; CPU layout built in %.1f s" % (
+            "nproc": os.cpu_count(), "affinity_cores": affinity_cores(), "cgroup_cpu_quota": cgroup_cpus(),
+            "cpu_model": cpu_model(), "threads": threads,
+            "sample": "%d timed passes (after 1 warm-up) over %s rays (%s)%s; oracle/cpu_bvh4.c (the reference CPU "
+                      "layout: BVH4 + 4-triangle packets, 4-wide SSE/FMA) on %d host threads (one per core of the "
+                      "affinity mask, capped by the CPU quota), %.1f s; median pass %.1f Mrays/s; CPU layout built in %.1f s" % (
                           len(times), "+".join(str(len(r)) for r, _ in sets),
-                          "/".join("occlusion" if a else "closest" for _, a in sets), threads, elapsed,
+                          "/".join("occlusion" if a else "closest" for _, a in sets), note, threads, elapsed,
                           n_pass / float(np.median(times)) / 1e6, t_layout)}
 
 
@@ -226,7 +324,7 @@ def run_pt(args):
     from imbatracer_amd import workloads as W
 
     t0 = time.time()
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_cores()
     scene = W.SphereGridScene(side=args.side, level=5, builder=N.BUILDER_SBVH, threads=threads)
     log("scene: %d triangles, %d nodes, built in %.1f s" % (scene.tri_count, scene.desc.node_count, time.time() - t0))
     cfg = fixed_config(args)
@@ -234,34 +332,50 @@ def run_pt(args):
     log("uploaded %.2f GB" % (pt.gpu_scene.footprint() / 1e9))
     # the queues of a frame (closest hits, hence the queues, do not depend on the configuration)
     _, _, queues = pt.frame(keep_queues=8)
-    bounce_cfg = None
+    bounce_cfg = shadow_bounce_cfg = None
     if cfg is None and not args.no_autotune:
         cfg, bounce_cfg = frame_tune(pt, queues)
+        _, shadow_bounce_cfg = shadow_tune(pt)
     node_bytes = node_bytes_of(cfg)
 
-    # untimed: visit counts of every bounce's queue (same seeds -> same rays as the timed frames)
+    # untimed: visit counts of every bounce's closest-hit and shadow launches (same seeds -> same
+    # rays as the timed frames)
     counts, visits, _ = pt.frame(stats=True)
+    shadow_counts, shadow_visits = list(pt.shadow_counts), list(pt.shadow_visit)
     stat_bytes = [ray_bytes(v, n, node_bytes) for v, n in zip(visits, counts)]
-    log("rays per bounce %s" % counts)
+    shadow_bytes = [ray_bytes(v, n, node_bytes_of(pt.shadow_cfg_of(b)))
+                    for b, (v, n) in enumerate(zip(shadow_visits, counts))]
+    log("rays per bounce %s, shadow rays %s" % (counts, shadow_counts))
 
     # eager frames (host reads each bounce's queue size): traversal kernel times for the roofline
     for _ in range(args.warmup):
         pt.frame()
     pt.stream.sync()
     eager_ms, trace_ms, rays_eager, bytes_total = [], 0.0, 0, 0.0
+    shadow_ms, shadow_bytes_total, shadow_rays_eager = 0.0, 0.0, 0
+    per_bounce_ms = np.zeros(len(counts))
+    per_bounce_shadow_ms = np.zeros(len(counts))
     t_eager = time.perf_counter()
     for _ in range(args.steps):
         f0 = time.perf_counter()
         c, _, _ = pt.frame(timed=True)
         pt.stream.sync()
         eager_ms.append((time.perf_counter() - f0) * 1e3)
-        if c != counts:
+        if c != counts or pt.shadow_counts != shadow_counts:
             raise RuntimeError("frame ray counts changed between frames: %s vs %s" % (c, counts))
-        for (ms, n), b in zip(pt.trace_times_ms(), stat_bytes):
+        for b, ((ms, n), sb) in enumerate(zip(pt.trace_times_ms(), stat_bytes)):
             trace_ms += ms
-            bytes_total += b
+            bytes_total += sb
+            per_bounce_ms[b] += ms
+        for b, ((ms, n), sb) in enumerate(zip(pt.shadow_times_ms(), shadow_bytes)):
+            shadow_ms += ms
+            shadow_bytes_total += sb
+            per_bounce_shadow_ms[b] += ms
         rays_eager += sum(c)
+        shadow_rays_eager += sum(shadow_counts)
     wall_eager = time.perf_counter() - t_eager
+    per_bounce_ms /= args.steps
+    per_bounce_shadow_ms /= args.steps
 
     # the measured frames
     if args.mode == "graph":
@@ -287,8 +401,16 @@ def run_pt(args):
     if pt.gpu_scene.error_count():
         raise RuntimeError("traversal reported a malformed scene")
     achieved = bytes_total / (trace_ms * 1e-3) / 1e9
-    workload = "pt-%dx%d-1spp-depth8-rr3-sphere-grid-%dtri-sbvh" % (args.width, args.height, scene.tri_count)
+    shadow_achieved = shadow_bytes_total / (shadow_ms * 1e-3) / 1e9
+    workload = "pt-%dx%d-1spp-depth8-rr3-nee-sphere-grid-%dtri-sbvh" % (args.width, args.height, scene.tri_count)
     traffic, traffic_src = measured_traffic(workload, kernel_label(cfg))
+    shadow_frame = sum(shadow_counts) * args.steps
+    table = [{"bounce": b, "rays": int(n), "closest_ms": float(per_bounce_ms[b]),
+              "closest_mrays_s": n / (per_bounce_ms[b] * 1e-3) / 1e6,
+              "closest_cfg": cfg_dict(pt.cfg_of(b)),
+              "shadow_rays": int(shadow_counts[b]), "occluded_ms": float(per_bounce_shadow_ms[b]),
+              "occluded_mrays_s": shadow_counts[b] / (per_bounce_shadow_ms[b] * 1e-3) / 1e6,
+              "occluded_cfg": cfg_dict(pt.shadow_cfg_of(b))} for b, n in enumerate(counts)]
     res = {
         "metric": METRIC, "value": rays_total / wall / 1e6, "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
@@ -296,17 +418,33 @@ def run_pt(args):
         "data": "synthetic (seeded procedural scene and camera)",
         "config": {"workload": workload,
                    "width": args.width, "height": args.height, "spp": 1, "max_path_len": 8,
+                   "integrator": "PathTracer::process_primary_rays: emitter hits with MIS, one next-event "
+                                 "shadow ray per surface hit (traversed any-hit), Lambertian bounce; Russian "
+                                 "roulette drawn every bounce, applied from bounce 3",
                    "rays_per_frame": int(sum(counts)), "rays_per_bounce": counts,
+                   "shadow_rays_per_frame": int(sum(shadow_counts)), "shadow_rays_per_bounce": shadow_counts,
                    "triangles": scene.tri_count, "nodes": scene.desc.node_count,
                    "frame": "one graph launch per frame (queue sizes on the device)" if args.mode == "graph"
                             else "eager launches, host reads each bounce's queue size",
                    "trace_kernel": args.trace_kernel,
                    "trace_config": cfg_dict(cfg), "autotuned": cfg is not None and not args.block,
-                   "trace_config_per_bounce": [cfg_dict(c) for c in bounce_cfg] if bounce_cfg else None},
+                   "trace_config_per_bounce": [cfg_dict(c) for c in bounce_cfg] if bounce_cfg else None,
+                   "shadow_config_per_bounce": [cfg_dict(c) for c in shadow_bounce_cfg] if shadow_bounce_cfg
+                   else None},
+        "value_note": "closest-hit rays per second of frame wall time; a frame also traverses its shadow rays",
+        "rays_incl_shadow_mrays_s": (rays_total + shadow_frame) / wall / 1e6,
         "eager_frames": {"value": rays_eager / wall_eager / 1e6, "ms_per_step": wall_eager * 1e3 / args.steps,
                          "frame_latency_ms": {"p50": float(np.percentile(eager_ms, 50)),
                                               "p99": float(np.percentile(eager_ms, 99))}},
         "trace_mrays_s": rays_eager / (trace_ms * 1e-3) / 1e6,
+        "occluded": {"mrays_s": shadow_rays_eager / (shadow_ms * 1e-3) / 1e6,
+                     "ms_per_frame": shadow_ms / args.steps, "rays_per_frame": int(sum(shadow_counts)),
+                     "roofline": {"bound": "hbm", "achieved": shadow_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": shadow_achieved / HBM_PEAK_GBS,
+                                  "kernel": "k_trace (any hit) over the shadow slots",
+                                  "algorithmic_bytes_per_launch": sum(shadow_bytes) / len(shadow_bytes),
+                                  "visits_per_slot": [float(x) / sum(counts) for x in np.sum(shadow_visits, 0)]}},
+        "per_bounce": table,
         "frame_latency_ms": {"p50": float(np.percentile(frame_ms, 50)), "p99": float(np.percentile(frame_ms, 99))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
@@ -316,8 +454,11 @@ def run_pt(args):
                      "visits_per_ray": [float(x) / sum(counts) for x in np.sum(visits, 0)]},
     }
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(scene.build, N.BUILDER_SBVH, threads, queues[:2], [False, False],
-                                           budget_s=args.cpu_seconds)
+        sample = sample_queues(queues, 200_000)
+        res["cpu_baseline"] = cpu_baseline(scene.build, N.BUILDER_SBVH, threads, sample, [False] * len(sample),
+                                           budget_s=args.cpu_seconds,
+                                           note=": every k-th ray of each of the frame's %d closest-hit queues" %
+                                                len(sample))
     return res
 
 
@@ -522,7 +663,9 @@ def run_c4_loop(args, builds, scenes, queues, cfg, rays_first):
     for b in builds:
         normals, mats = b.tri_attrs()
         n_mat = int(mats.max()) + 1 if len(mats) else 1
-        shading.append((normals, mats, np.full((n_mat, 3), 0.5, np.float32), np.zeros((n_mat, 3), np.float32), 1e-4))
+        shading.append(W.PtShadingData(normals, mats, np.full((n_mat, 3), 0.5, np.float32),
+                                       np.zeros((n_mat, 3), np.float32), np.zeros(n_mat, np.float32),
+                                       np.zeros((0, 12), np.float32)))
     loop = W.QueueLoop(scenes, shading, [(k % 8, q[1], q[3]) for k, q in enumerate(queues)],
                        max_path_len=args.c4_bounces, rr_start=3, cfg=cfg)
     loop.build_graph()
@@ -562,18 +705,18 @@ def run_c4_loop(args, builds, scenes, queues, cfg, rays_first):
         n = e["n"]
         if n == 0:
             continue
-        tn, tm, nt, al, em, nm, off = loop.shade_bufs[e["scene"]]
+        sh = loop.sh[e["scene"]]
         N.check(N.lib.imba_memcpy_d2d(e["rays"][0].ptr, e["rays0"].ptr, 32 * n, st.handle), "rays")
-        N.check(N.lib.imba_memcpy_d2d(e["states"][0].ptr, e["states0"].ptr, 32 * n, st.handle), "states")
+        N.check(N.lib.imba_memcpy_d2d(e["states"][0].ptr, e["states0"].ptr, W.PT_STATE_BYTES * n, st.handle),
+                "states")
         for b in range(L):
             cur, nxt = b % 2, (b + 1) % 2
             cnt = loop._cnt(k, b)
             N.check(N.lib.imba_intersect_dev(scenes[e["scene"]].h, e["rays"][cur].ptr, e["hits"].ptr, n, cnt,
                                              C.byref(cfg) if cfg else None, st.handle), "intersect_dev")
-            N.check(N.lib.imba_pt_bounce_dev(e["rays"][cur].ptr, e["hits"].ptr, e["states"][cur].ptr, n, cnt, tn.ptr,
-                                             tm.ptr, nt, al.ptr, em.ptr, nm, L, 3, off, None, e["rays"][cur].ptr,
-                                             st.handle), "bounce_dev")
-            N.check(N.lib.imba_compact_rays_dev(e["rays"][cur].ptr, e["states"][cur].ptr, 32, n, cnt,
+            N.check(N.lib.imba_pt_shade(C.byref(sh), e["rays"][cur].ptr, e["hits"].ptr, e["states"][cur].ptr, n, cnt,
+                                        None, e["rays"][cur].ptr, None, None, st.handle), "pt_shade")
+            N.check(N.lib.imba_compact_rays_dev(e["rays"][cur].ptr, e["states"][cur].ptr, W.PT_STATE_BYTES, n, cnt,
                                                 e["rays"][nxt].ptr, e["states"][nxt].ptr, loop._cnt(k, b + 1),
                                                 ws.ptr, st.handle), "compact_dev")
     st.sync()
@@ -589,7 +732,8 @@ def run_c4_loop(args, builds, scenes, queues, cfg, rays_first):
                         "per_queue_launches": {"value": rays_step / wall_p / 1e6, "ms_per_step": wall_p * 1e3,
                                                "steps": 1},
                         "step": "one graph launch: per bounce imba_trace_batch_launch + imba_shade_batch_launch "
-                                "over all 256 queues (Lambertian, albedo 0.5, Russian roulette from bounce 3)"},
+                                "over all 256 queues (Lambertian, albedo 0.5, no lights, Russian roulette from "
+                                "bounce 3)"},
     }
 
 
@@ -603,7 +747,7 @@ def run_c5(args):
     from imbatracer_amd import serve as V
     from imbatracer_amd import workloads as W
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_cores()
     grids = [(1, 5), (2, 5), (4, 5), (6, 5), (8, 5), (12, 5), (20, 5), (32, 5)]
     if args.c5_small:
         grids = [(1, 3), (2, 3), (2, 4), (3, 4), (4, 4), (4, 5), (6, 5), (8, 5)]
@@ -716,7 +860,7 @@ def run_c3(args):
     from imbatracer_amd import workloads as W
 
     t0 = time.time()
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_cores()
     scene = W.SphereGridScene(side=args.side, level=5, builder=N.BUILDER_SBVH, threads=threads,
                               specular_fraction=0.3)
     log("scene: %d triangles, built in %.1f s" % (scene.tri_count, time.time() - t0))

@@ -48,7 +48,7 @@ struct LightArgs {
     int first, count;
 };
 
-__global__ __launch_bounds__(256) void k_light(LightArgs a, float4* rays, PtState* states) {
+__global__ __launch_bounds__(256) void k_light(LightArgs a, float4* rays, PathState* states) {
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= a.count) return;
     const int i = a.first + j;
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void k_light(LightArgs a, float4* rays, PtStat
                       l.x * bn.z + l.y * tg.z + l.z * n.z);
     rays[2 * (size_t)j] = make_float4(pos.x, pos.y, pos.z, 1e-3f);
     rays[2 * (size_t)j + 1] = make_float4(d.x, d.y, d.z, kFltMax);
-    PtState s;
+    PathState s;
     s.pixel_id = i;
     s.sample_id = 0;
     s.rng = rng.s;
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(256) void k_light(LightArgs a, float4* rays, PtStat
 struct SurfArgs {
     const float4* rays;
     const int4* hits;
-    PtState* states;
+    PathState* states;
     float4* rays_out;      // may alias rays
     int count;             // entries (or, with count_dev, the capacity)
     const int* count_dev;  // device entry count read when the launch runs, or null
@@ -109,7 +109,7 @@ struct SurfArgs {
 };
 
 // Russian roulette (render/random.h:183-188); false ends the path.
-__device__ __forceinline__ bool roulette(PtState& s, Rng& rng, int rr_start) {
+__device__ __forceinline__ bool roulette(PathState& s, Rng& rng, int rr_start) {
     if (s.bounces < rr_start) return true;
     const float lum = 0.2126f * s.tp[0] + 0.7152f * s.tp[1] + 0.0722f * s.tp[2];
     const float p = fminf(1.0f, lum * 2.0f);
@@ -136,7 +136,7 @@ __device__ __forceinline__ Vec3 specular_dir(int kind, Vec3 d, Vec3 nf, bool ent
     return madd3(scale3(d, eta), nf, eta * cos_i - cos_t);
 }
 
-__device__ __forceinline__ void end_path(const SurfArgs& a, int i, PtState& s) {
+__device__ __forceinline__ void end_path(const SurfArgs& a, int i, PathState& s) {
     s.pixel_id = -1;
     a.states[i] = s;
 }
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void k_light_bounce(SurfArgs a) {
     const int n = a.count_dev ? min(max(*a.count_dev, 0), a.count) : a.count;
     a.photon_states[i] = make_float4(__int_as_float(-1), 0.0f, 0.0f, 0.0f);
     if (i >= n) return;
-    PtState s = a.states[i];
+    PathState s = a.states[i];
     const int4 h = a.hits[i];
     if (h.x < 0 || h.x >= a.n_tris) return end_path(a, i, s);
     const int mat = a.tri_material[h.x];
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(256) void k_light_bounce(SurfArgs a) {
     a.rays_out[2 * (size_t)i + 1] = make_float4(w.x, w.y, w.z, kFltMax);
 }
 
-__device__ __forceinline__ void end_camera_path(const SurfArgs& a, int i, PtState& s) {
+__device__ __forceinline__ void end_camera_path(const SurfArgs& a, int i, PathState& s) {
     const size_t q = (size_t)s.pixel_id * a.spp + s.sample_id;
     a.queries[4 * q] = make_float4(__int_as_float(0x7f800000), __int_as_float(0x7f800000),
                                    __int_as_float(0x7f800000), __int_as_float(-1));
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(256) void k_camera_bounce(SurfArgs a) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     const int n = a.count_dev ? min(max(*a.count_dev, 0), a.count) : a.count;
     if (i >= n) return;
-    PtState s = a.states[i];
+    PathState s = a.states[i];
     const int4 h = a.hits[i];
     if (h.x < 0 || h.x >= a.n_tris) return end_camera_path(a, i, s);
     const int mat = a.tri_material[h.x];
@@ -340,7 +340,7 @@ int imba_ppm_light(const float* light_tri9, const float* intensity3, uint32_t se
     a.count = count;
     bool sync;
     hipStream_t s = resolve_stream(stream, &sync);
-    hipLaunchKernelGGL(k_light, dim3(ceil_div(count, 256)), dim3(256), 0, s, a, (float4*)rays, (PtState*)states);
+    hipLaunchKernelGGL(k_light, dim3(ceil_div(count, 256)), dim3(256), 0, s, a, (float4*)rays, (PathState*)states);
     IMBA_HIP_TRY(hipGetLastError());
     if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
     return IMBA_OK;
@@ -358,7 +358,7 @@ int imba_ppm_light_bounce(const ImbaRay* rays, const ImbaHit* hits, void* states
     SurfArgs a = {};
     a.rays = (const float4*)rays;
     a.hits = (const int4*)hits;
-    a.states = (PtState*)states;
+    a.states = (PathState*)states;
     a.rays_out = (float4*)rays_out;
     a.count = count;
     a.count_dev = count_dev;
@@ -387,7 +387,7 @@ int imba_ppm_camera_bounce(const ImbaRay* rays, const ImbaHit* hits, void* state
     SurfArgs a = {};
     a.rays = (const float4*)rays;
     a.hits = (const int4*)hits;
-    a.states = (PtState*)states;
+    a.states = (PathState*)states;
     a.rays_out = (float4*)rays_out;
     a.count = count;
     a.count_dev = count_dev;

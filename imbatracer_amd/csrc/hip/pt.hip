@@ -1,19 +1,20 @@
-// Path-tracing ray producers for the measured workloads: camera rays and Lambertian bounces
-// written straight into device ray queues, so a frame's bounce loop never leaves the GPU.
+// Path-tracing ray producers for the measured workloads: camera rays, the shading step of every
+// hit (next-event shadow ray, emitter contribution, Lambertian bounce) and the shadow-ray
+// contributions, written straight into device ray queues, so a frame's bounce loop never
+// leaves the GPU.
 //
 // Camera rays follow PathTracer's sample function (render/integrators/pt.cpp:168-177) with the
 // per-sample seeding of PixelRayGen::fill_queue (render/ray_gen/ray_gen.h:51-72) and the
-// pinhole camera of render/ray_gen/camera.h:57-79.  A bounce follows PathTracer::bounce
-// (pt.cpp:46-80) for a Lambertian material: terminate past the maximum path length, Russian
-// roulette (render/random.h:183-188) from bounce `rr_start` on, cosine-weighted direction
-// (random.h:80-83) in the frame of random.h:57-72 around the geometric normal, throughput
-// times the albedo.  Rays that hit an emissive material add throughput * emission to the
-// image and end.  Terminated rays get pixel_id = -1 (integrator.h:125-128) and are removed by
-// imba_compact_rays.
+// pinhole camera of render/ray_gen/camera.h:57-79.  The shading step is pt_shade_entry
+// (pt_shade.h): PathTracer::process_primary_rays for Lambertian and emitting triangles.
+// Terminated paths get pixel_id = -1 (integrator.h:125-128) and are removed by
+// imba_compact_rays; empty shadow slots have pixel_id = -1 in their shadow state.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "common.h"
-#include "pt_bounce.h"
+#include "pt_shade.h"
 #include "sampling.h"
 
 namespace imba {
@@ -29,7 +30,9 @@ struct CamArgs {
     float4* queries;  // photon-mapping merge queries (4 float4 per sample) to mark empty, or null
 };
 
-__global__ __launch_bounds__(256) void k_camera(CamArgs c, float4* rays, PtState* states) {
+// S: PtState (path tracer, 40 bytes) or PathState (photon mapping, 32 bytes).
+template <typename S>
+__global__ __launch_bounds__(256) void k_camera(CamArgs c, float4* rays, S* states) {
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= c.count) return;
     const int i = c.first + j;
@@ -46,26 +49,52 @@ __global__ __launch_bounds__(256) void k_camera(CamArgs c, float4* rays, PtState
     dx *= inv; dy *= inv; dz *= inv;
     rays[2 * (size_t)j] = make_float4(c.pos[0], c.pos[1], c.pos[2], 0.0f);
     rays[2 * (size_t)j + 1] = make_float4(dx, dy, dz, 3.402823466e38f);
-    PtState s;
+    S s;
     s.pixel_id = pixel;
     s.sample_id = sample;
     s.rng = rng.s;
     s.tp[0] = s.tp[1] = s.tp[2] = 1.0f;
     s.bounces = 0;
+    if constexpr (std::is_same<S, PtState>::value) {
+        s.last_pdf = 0.0f;  // read only after a bounce (bounces > 0)
+        s.last_specular = 0;
+    }
     states[j] = s;
     if (c.queries)
         c.queries[4 * (size_t)i] = make_float4(__int_as_float(0x7f800000), __int_as_float(0x7f800000),
                                                __int_as_float(0x7f800000), __int_as_float(-1));
 }
 
-__global__ __launch_bounds__(256) void k_bounce(BounceArgs a) {
+__device__ __forceinline__ int entries(int capacity, const int* count_dev) {
+    return count_dev ? min(max(*count_dev, 0), capacity) : capacity;
+}
+
+__global__ __launch_bounds__(256) void k_pt_shade(PtShadeArgs a, int capacity, const int* count_dev) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    const int n = a.count_dev ? min(max(*a.count_dev, 0), a.count) : a.count;
-    if (i >= n) return;
-    bounce_entry(a, i);
+    if (i >= entries(capacity, count_dev)) return;
+    pt_shade_entry(a, i);
+}
+
+__global__ __launch_bounds__(256) void k_pt_shadow(const int4* hits, const ShadowState* st, float* image, int capacity,
+                                                   const int* count_dev) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= entries(capacity, count_dev)) return;
+    pt_shadow_entry(hits, st, image, i);
 }
 
 }  // namespace
+
+// Checks the shading description (host side of the device pointers it holds).
+int check_shading(const ImbaPtShading* sh) {
+    IMBA_CHECK_ARG(sh, "shading is NULL");
+    IMBA_CHECK_ARG(sh->tri_normals && sh->tri_materials && sh->albedo && sh->emission && sh->emitter_area,
+                   "shading: NULL array");
+    IMBA_CHECK_ARG(sh->tri_count >= 0 && sh->material_count >= 0, "shading: negative count");
+    IMBA_CHECK_ARG(sh->light_count >= 0 && (sh->light_count == 0 || sh->lights), "shading: lights");
+    IMBA_CHECK_ARG(sh->max_path_len >= 1 && sh->rr_start >= 0, "shading: max_path_len >= 1 and rr_start >= 0");
+    return IMBA_OK;
+}
+
 }  // namespace imba
 
 using namespace imba;
@@ -73,7 +102,8 @@ using namespace imba;
 extern "C" {
 
 static int camera(const float* raster_to_world16, const float* cam_pos3, int width, int height, int spp,
-                  uint32_t seed_base, int first, int count, ImbaRay* rays, void* states, void* queries, void* stream) {
+                  uint32_t seed_base, int first, int count, ImbaRay* rays, void* states, void* queries, bool pt,
+                  void* stream) {
     IMBA_CHECK_ARG(raster_to_world16 && cam_pos3 && rays && states, "NULL argument");
     IMBA_CHECK_ARG(width > 0 && height > 0 && spp > 0 && first >= 0 && count >= 0, "bad sizes");
     IMBA_CHECK_ARG((int64_t)first + count <= (int64_t)width * height * spp, "samples out of range");
@@ -90,7 +120,12 @@ static int camera(const float* raster_to_world16, const float* cam_pos3, int wid
     c.queries = (float4*)queries;
     bool sync;
     hipStream_t s = resolve_stream(stream, &sync);
-    hipLaunchKernelGGL(k_camera, dim3(ceil_div(count, 256)), dim3(256), 0, s, c, (float4*)rays, (PtState*)states);
+    if (pt)
+        hipLaunchKernelGGL(k_camera<PtState>, dim3(ceil_div(count, 256)), dim3(256), 0, s, c, (float4*)rays,
+                           (PtState*)states);
+    else
+        hipLaunchKernelGGL(k_camera<PathState>, dim3(ceil_div(count, 256)), dim3(256), 0, s, c, (float4*)rays,
+                           (PathState*)states);
     IMBA_HIP_TRY(hipGetLastError());
     if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
     return IMBA_OK;
@@ -99,7 +134,7 @@ static int camera(const float* raster_to_world16, const float* cam_pos3, int wid
 int imba_pt_camera(const float* raster_to_world16, const float* cam_pos3, int width, int height, int spp,
                    uint32_t seed_base, int first, int count, ImbaRay* rays, void* states, void* stream) {
     return camera(raster_to_world16, cam_pos3, width, height, spp, seed_base, first, count, rays, states, nullptr,
-                  stream);
+                  true, stream);
 }
 
 int imba_ppm_camera(const float* raster_to_world16, const float* cam_pos3, int width, int height, int spp,
@@ -107,57 +142,47 @@ int imba_ppm_camera(const float* raster_to_world16, const float* cam_pos3, int w
                     void* stream) {
     IMBA_CHECK_ARG(queries, "NULL argument");
     return camera(raster_to_world16, cam_pos3, width, height, spp, seed_base, first, count, rays, states, queries,
-                  stream);
+                  false, stream);
 }
 
-static int pt_bounce(const ImbaRay* rays, const ImbaHit* hits, void* states, int count, const int* count_dev,
-                     const float* tri_normals, const int* tri_materials, int tri_count, const float* albedo,
-                     const float* emission, int material_count, int max_path_len, int rr_start, float offset,
-                     float* image, ImbaRay* rays_out, void* stream) {
-    IMBA_CHECK_ARG(count >= 0, "count < 0");
-    if (count == 0) return IMBA_OK;
-    IMBA_CHECK_ARG(rays && hits && states && tri_normals && tri_materials && albedo && emission && rays_out,
-                   "NULL argument");
-    BounceArgs a;
+int imba_pt_shade(const ImbaPtShading* sh, const ImbaRay* rays, const ImbaHit* hits, void* states, int capacity,
+                  const int* count_dev, float* image, ImbaRay* rays_out, ImbaRay* shadow_rays, void* shadow_states,
+                  void* stream) {
+    int r = check_shading(sh);
+    if (r) return r;
+    IMBA_CHECK_ARG(capacity >= 0, "capacity < 0");
+    if (capacity == 0) return IMBA_OK;
+    IMBA_CHECK_ARG(rays && hits && states && rays_out, "NULL argument");
+    IMBA_CHECK_ARG((shadow_rays == nullptr) == (shadow_states == nullptr), "shadow_rays and shadow_states go together");
+    PtShadeArgs a;
     a.rays = (const float4*)rays;
     a.hits = (const int4*)hits;
     a.states = (PtState*)states;
     a.rays_out = (float4*)rays_out;
-    a.count = count;
-    a.count_dev = count_dev;
-    a.tri_normals = tri_normals;
-    a.tri_material = tri_materials;
-    a.n_tris = tri_count;
-    a.albedo = albedo;
-    a.emission = emission;
-    a.n_materials = material_count;
-    a.max_path_len = max_path_len;
-    a.rr_start = rr_start;
-    a.offset = offset;
+    a.shadow_rays = (float4*)shadow_rays;
+    a.shadow_states = (ShadowState*)shadow_states;
     a.image = image;
+    a.sh = *sh;
     bool sync;
     hipStream_t s = resolve_stream(stream, &sync);
-    hipLaunchKernelGGL(k_bounce, dim3(ceil_div(count, 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_pt_shade, dim3(ceil_div(capacity, 256)), dim3(256), 0, s, a, capacity, count_dev);
     IMBA_HIP_TRY(hipGetLastError());
     if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
     return IMBA_OK;
 }
 
-int imba_pt_bounce(const ImbaRay* rays, const ImbaHit* hits, void* states, int count, const float* tri_normals,
-                   const int* tri_materials, int tri_count, const float* albedo, const float* emission,
-                   int material_count, int max_path_len, int rr_start, float offset, float* image, ImbaRay* rays_out,
-                   void* stream) {
-    return pt_bounce(rays, hits, states, count, nullptr, tri_normals, tri_materials, tri_count, albedo, emission,
-                     material_count, max_path_len, rr_start, offset, image, rays_out, stream);
-}
-
-int imba_pt_bounce_dev(const ImbaRay* rays, const ImbaHit* hits, void* states, int capacity, const int* count_dev,
-                       const float* tri_normals, const int* tri_materials, int tri_count, const float* albedo,
-                       const float* emission, int material_count, int max_path_len, int rr_start, float offset,
-                       float* image, ImbaRay* rays_out, void* stream) {
-    IMBA_CHECK_ARG(count_dev, "count_dev is NULL");
-    return pt_bounce(rays, hits, states, capacity, count_dev, tri_normals, tri_materials, tri_count, albedo, emission,
-                     material_count, max_path_len, rr_start, offset, image, rays_out, stream);
+int imba_pt_shadow(const ImbaHit* shadow_hits, const void* shadow_states, int capacity, const int* count_dev,
+                   float* image, void* stream) {
+    IMBA_CHECK_ARG(capacity >= 0, "capacity < 0");
+    if (capacity == 0) return IMBA_OK;
+    IMBA_CHECK_ARG(shadow_hits && shadow_states && image, "NULL argument");
+    bool sync;
+    hipStream_t s = resolve_stream(stream, &sync);
+    hipLaunchKernelGGL(k_pt_shadow, dim3(ceil_div(capacity, 256)), dim3(256), 0, s, (const int4*)shadow_hits,
+                       (const ShadowState*)shadow_states, image, capacity, count_dev);
+    IMBA_HIP_TRY(hipGetLastError());
+    if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
+    return IMBA_OK;
 }
 
 }  // extern "C"

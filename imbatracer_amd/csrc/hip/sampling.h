@@ -35,15 +35,16 @@ __device__ __forceinline__ Rng seeded(uint32_t seed_base, int index, int pixel) 
     return r;
 }
 
-// Path state of the ray producers (32 bytes; pixel_id first so compaction can test it).
-struct PtState {
+// Path state of the photon-mapping producers (32 bytes; pixel_id first so compaction can test
+// it).  The path tracer's state (PtState, 40 bytes) is in pt_shade.h.
+struct PathState {
     int pixel_id;
     int sample_id;
     uint64_t rng;
     float tp[3];
     int bounces;
 };
-static_assert(sizeof(PtState) == 32, "PtState layout");
+static_assert(sizeof(PathState) == 32, "PathState layout");
 
 struct Vec3 {
     float x, y, z;

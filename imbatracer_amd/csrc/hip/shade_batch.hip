@@ -1,21 +1,25 @@
-// Batched Lambertian bounce and stable queue compaction over many ray queues.
+// Batched path-tracing shading step and stable queue compaction over many ray queues.
 //
 // The reference shades and compacts every queue on the host after its traversal
 // (render/scheduling/queue_scheduler.h:177-275, ray_queue.h:188-207).  Here one step of many
 // queues -- any scenes, any sizes, sizes held in device memory -- takes five launches:
 //   k_shade_prefix    one block: each queue's entry count (its device count, clamped to the
-//                     capacity), and exclusive prefixes of its bounce chunks (256 entries) and
+//                     capacity), and exclusive prefixes of its shading chunks (256 entries) and
 //                     compaction tiles (2048 entries);
 //   k_bounce_batch    one block per chunk: the chunk's queue by binary search over the chunk
-//                     prefix, then bounce_entry (the single-queue bounce) on each entry;
+//                     prefix, then pt_shade_entry (the single-queue shading step: emitter
+//                     contribution, next-event shadow ray into the queue's shadow slots,
+//                     Lambertian bounce) on each entry;
 //   k_ccount_batch    one block per tile: survivors (pixel_id >= 0) in the tile;
 //   k_cscan_batch     one block per queue: exclusive scan of its tile counts, the total to the
 //                     queue's next device count;
 //   k_cscatter_batch  one block per tile: each survivor's rank (ballot, mbcnt, wave counts in
 //                     LDS) and its ray and state moved to the queue's output.
-// Survivors keep their order, so every queue's result equals its own imba_pt_bounce_dev +
-// imba_compact_rays_dev.  Nothing here allocates or sets memory: a bounce loop over the batch
-// can be captured in a graph.
+// Survivors keep their order, so every queue's result equals its own imba_pt_shade +
+// imba_compact_rays_dev.  After the shadow queues' (batched, any-hit) traversal,
+// k_shadow_batch adds the unoccluded shadow contributions of every queue (imba_pt_shadow).
+// Nothing here allocates or sets memory: a bounce loop over the batch can be captured in a
+// graph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,7 +27,7 @@
 #include <vector>
 
 #include "common.h"
-#include "pt_bounce.h"
+#include "pt_shade.h"
 
 struct ImbaShadeBatch {
     int device = 0;
@@ -38,6 +42,7 @@ struct ImbaShadeBatch {
     ImbaShadeItem* staging[2] = {nullptr, nullptr};  // pinned host copies of the table
     hipEvent_t staged[2] = {nullptr, nullptr};
     int next_staging = 0;
+    bool launched = false;  // k_shade_prefix has run at least once (its info feeds k_shadow_batch)
     bool captured = false;  // launched into a graph: the device tables may no longer move
     long long captured_chunks = 0;  // grid sizes recorded into graphs (the largest)
     long long captured_tiles = 0;
@@ -51,7 +56,7 @@ constexpr int kCBlock = 256;
 constexpr int kCItems = 8;
 constexpr int kCTile = kCBlock * kCItems;   // entries per compaction tile
 constexpr int kCWaves = kCBlock / 64;
-constexpr int kStateDw = 8;                 // 32-byte states
+constexpr int kStateDw = sizeof(PtState) / 4;
 
 __device__ __forceinline__ int item_entries(const ImbaShadeItem& it) {
     return it.count_dev ? min(max(*it.count_dev, 0), it.capacity) : it.capacity;
@@ -109,8 +114,7 @@ __global__ __launch_bounds__(1024) void k_shade_prefix(const ImbaShadeItem* item
     }
 }
 
-__global__ __launch_bounds__(kChunk) void k_bounce_batch(const ImbaShadeItem* items, int n_items, const int* info,
-                                                         int max_path_len, int rr_start) {
+__global__ __launch_bounds__(kChunk) void k_bounce_batch(const ImbaShadeItem* items, int n_items, const int* info) {
     const int* ent = info;
     const int* cb = info + (n_items + 1);
     const int c = blockIdx.x;
@@ -119,24 +123,31 @@ __global__ __launch_bounds__(kChunk) void k_bounce_batch(const ImbaShadeItem* it
     const int i = (c - cb[q]) * kChunk + (int)threadIdx.x;
     if (i >= ent[q]) return;
     const ImbaShadeItem& it = items[q];
-    BounceArgs a;
+    PtShadeArgs a;
     a.rays = (const float4*)it.rays;
     a.hits = (const int4*)it.hits;
     a.states = (PtState*)it.states;
     a.rays_out = (float4*)it.rays;
-    a.count = ent[q];
-    a.count_dev = nullptr;
-    a.tri_normals = it.tri_normals;
-    a.tri_material = it.tri_materials;
-    a.n_tris = it.tri_count;
-    a.albedo = it.albedo;
-    a.emission = it.emission;
-    a.n_materials = it.material_count;
-    a.max_path_len = max_path_len;
-    a.rr_start = rr_start;
-    a.offset = it.offset;
+    a.shadow_rays = (float4*)it.shadow_rays;
+    a.shadow_states = (ShadowState*)it.shadow_states;
     a.image = it.image;
-    bounce_entry(a, i);
+    a.sh = it.shading;
+    pt_shade_entry(a, i);
+}
+
+// The shadow contributions of every item with shadow slots: the same chunks as k_bounce_batch
+// (the entry counts k_shade_prefix read for the preceding shading launch).
+__global__ __launch_bounds__(kChunk) void k_shadow_batch(const ImbaShadeItem* items, int n_items, const int* info) {
+    const int* ent = info;
+    const int* cb = info + (n_items + 1);
+    const int c = blockIdx.x;
+    if (c >= cb[n_items]) return;
+    const int q = find_item(cb, n_items, c);
+    const int i = (c - cb[q]) * kChunk + (int)threadIdx.x;
+    if (i >= ent[q]) return;
+    const ImbaShadeItem& it = items[q];
+    if (!it.shadow_states || !it.shadow_hits || !it.image) return;
+    pt_shadow_entry((const int4*)it.shadow_hits, (const ShadowState*)it.shadow_states, it.image, i);
 }
 
 __device__ __forceinline__ bool survives(const ImbaShadeItem& it, int i) {
@@ -215,8 +226,8 @@ __global__ __launch_bounds__(kCBlock) void k_cscatter_batch(const ImbaShadeItem*
     const int wave = threadIdx.x >> 6;
     const float4* rin = (const float4*)it.rays;
     float4* rout = (float4*)it.rays_out;
-    const int4* s_in = (const int4*)it.states;
-    int4* s_out = (int4*)it.states_out;
+    const PtState* s_in = (const PtState*)it.states;
+    PtState* s_out = (PtState*)it.states_out;
     for (int k = 0; k < kCItems; k++) {
         const int i = base + k * kCBlock + (int)threadIdx.x;
         const bool f = i < n && survives(it, i);
@@ -235,8 +246,7 @@ __global__ __launch_bounds__(kCBlock) void k_cscatter_batch(const ImbaShadeItem*
             const size_t d = (size_t)(kept_before + woff + r);
             rout[2 * d] = rin[2 * (size_t)i];
             rout[2 * d + 1] = rin[2 * (size_t)i + 1];
-            s_out[2 * d] = s_in[2 * (size_t)i];
-            s_out[2 * d + 1] = s_in[2 * (size_t)i + 1];
+            s_out[d] = s_in[i];
         }
         kept_before += chunk_kept;
         __syncthreads();
@@ -248,9 +258,14 @@ int set_items(ImbaShadeBatch* b, const ImbaShadeItem* items, int n, hipStream_t 
     for (int i = 0; i < n; i++) {
         const ImbaShadeItem& it = items[i];
         if (it.capacity < 0) return fail(IMBA_ERR_INVALID_ARG, "item %d: capacity < 0", i);
-        if (it.capacity > 0 && (!it.rays || !it.hits || !it.states || !it.rays_out || !it.states_out ||
-                                !it.tri_normals || !it.tri_materials || !it.albedo || !it.emission))
+        if (it.capacity > 0 && (!it.rays || !it.hits || !it.states || !it.rays_out || !it.states_out))
             return fail(IMBA_ERR_INVALID_ARG, "item %d: NULL buffer", i);
+        if (it.capacity > 0) {
+            const int r = check_shading(&it.shading);
+            if (r) return r;
+        }
+        if ((it.shadow_rays == nullptr) != (it.shadow_states == nullptr))
+            return fail(IMBA_ERR_INVALID_ARG, "item %d: shadow_rays and shadow_states go together", i);
         if (it.capacity > 0 && (it.rays_out == it.rays || it.states_out == it.states))
             return fail(IMBA_ERR_INVALID_ARG, "item %d: the compaction output must be other buffers", i);
         chunks += ceil_div(it.capacity, kChunk);
@@ -359,7 +374,7 @@ int imba_shade_batch_update(ImbaShadeBatch* b, const ImbaShadeItem* items, int c
     return set_items(b, items, count, (hipStream_t)stream);
 }
 
-int imba_shade_batch_launch(ImbaShadeBatch* b, int max_path_len, int rr_start, void* stream) {
+int imba_shade_batch_launch(ImbaShadeBatch* b, void* stream) {
     IMBA_CHECK_ARG(b, "batch is NULL");
     if (b->n == 0) return IMBA_OK;
     bool sync;
@@ -372,10 +387,11 @@ int imba_shade_batch_launch(ImbaShadeBatch* b, int max_path_len, int rr_start, v
         b->captured_tiles = std::max(b->captured_tiles, b->max_tiles);
     }
     const int n = b->n;
+    b->launched = true;
     hipLaunchKernelGGL(k_shade_prefix, dim3(1), dim3(1024), 0, s, (const ImbaShadeItem*)b->d_items, n, b->d_info);
     if (b->max_chunks > 0)
         hipLaunchKernelGGL(k_bounce_batch, dim3((unsigned)b->max_chunks), dim3(kChunk), 0, s,
-                           (const ImbaShadeItem*)b->d_items, n, (const int*)b->d_info, max_path_len, rr_start);
+                           (const ImbaShadeItem*)b->d_items, n, (const int*)b->d_info);
     if (b->max_tiles > 0)
         hipLaunchKernelGGL(k_ccount_batch, dim3((unsigned)b->max_tiles), dim3(kCBlock), 0, s,
                            (const ImbaShadeItem*)b->d_items, n, (const int*)b->d_info, b->d_tile_counts);
@@ -384,6 +400,19 @@ int imba_shade_batch_launch(ImbaShadeBatch* b, int max_path_len, int rr_start, v
     if (b->max_tiles > 0)
         hipLaunchKernelGGL(k_cscatter_batch, dim3((unsigned)b->max_tiles), dim3(kCBlock), 0, s,
                            (const ImbaShadeItem*)b->d_items, n, (const int*)b->d_info, (const int*)b->d_tile_counts);
+    IMBA_HIP_TRY(hipGetLastError());
+    if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
+    return IMBA_OK;
+}
+
+int imba_shade_batch_shadow(ImbaShadeBatch* b, void* stream) {
+    IMBA_CHECK_ARG(b, "batch is NULL");
+    if (b->n == 0 || b->max_chunks == 0) return IMBA_OK;
+    if (!b->launched) return fail(IMBA_ERR_INVALID_ARG, "imba_shade_batch_shadow needs a preceding imba_shade_batch_launch");
+    bool sync;
+    hipStream_t s = resolve_stream(stream, &sync);
+    hipLaunchKernelGGL(k_shadow_batch, dim3((unsigned)b->max_chunks), dim3(kChunk), 0, s,
+                       (const ImbaShadeItem*)b->d_items, b->n, (const int*)b->d_info);
     IMBA_HIP_TRY(hipGetLastError());
     if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
     return IMBA_OK;

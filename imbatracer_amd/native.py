@@ -92,13 +92,20 @@ class TraceItem(C.Structure):
                 ("count_dev", C.c_void_p), ("any_hit", C.c_int)]
 
 
+class PtShading(C.Structure):
+    """ImbaPtShading: a scene's shading data for the path tracer (device pointers)."""
+    _fields_ = [("tri_normals", C.c_void_p), ("tri_materials", C.c_void_p), ("tri_count", C.c_int),
+                ("albedo", C.c_void_p), ("emission", C.c_void_p), ("emitter_area", C.c_void_p),
+                ("material_count", C.c_int), ("lights", C.c_void_p), ("light_count", C.c_int),
+                ("max_path_len", C.c_int), ("rr_start", C.c_int)]
+
+
 class ShadeItem(C.Structure):
-    """ImbaShadeItem: one queue of a batched bounce + compaction."""
+    """ImbaShadeItem: one queue of a batched shading step + compaction."""
     _fields_ = [("rays", C.c_void_p), ("hits", C.c_void_p), ("states", C.c_void_p), ("capacity", C.c_int),
                 ("count_dev", C.c_void_p), ("rays_out", C.c_void_p), ("states_out", C.c_void_p),
-                ("next_count_dev", C.c_void_p), ("tri_normals", C.c_void_p), ("tri_materials", C.c_void_p),
-                ("tri_count", C.c_int), ("albedo", C.c_void_p), ("emission", C.c_void_p),
-                ("material_count", C.c_int), ("offset", C.c_float), ("image", C.c_void_p)]
+                ("next_count_dev", C.c_void_p), ("shadow_rays", C.c_void_p), ("shadow_states", C.c_void_p),
+                ("shadow_hits", C.c_void_p), ("image", C.c_void_p), ("shading", PtShading)]
 
 
 class HashGridInfo(C.Structure):
@@ -207,11 +214,12 @@ _sig("imba_gen_icosphere", i32, i32, f32, f32, f32, f32, C.c_uint32, vp, vp)
 _sig("imba_gen_camera_rays", i32, vp, vp, vp, f32, i32, i32, vp)
 _sig("imba_gen_camera_matrix", i32, vp, vp, vp, f32, i32, i32, vp)
 _sig("imba_pt_camera", i32, vp, vp, i32, i32, i32, C.c_uint32, i32, i32, vp, vp, vp)
-_sig("imba_pt_bounce", i32, vp, vp, vp, i32, vp, vp, i32, vp, vp, i32, i32, i32, f32, vp, vp, vp)
-_sig("imba_pt_bounce_dev", i32, vp, vp, vp, i32, vp, vp, vp, i32, vp, vp, i32, i32, i32, f32, vp, vp, vp)
+_sig("imba_pt_shade", i32, C.POINTER(PtShading), vp, vp, vp, i32, vp, vp, vp, vp, vp, vp)
+_sig("imba_pt_shadow", i32, vp, vp, i32, vp, vp, vp)
 _sig("imba_shade_batch_create", i32, vp, i32, C.POINTER(vp))
 _sig("imba_shade_batch_update", i32, vp, vp, i32, vp)
-_sig("imba_shade_batch_launch", i32, vp, i32, i32, vp)
+_sig("imba_shade_batch_launch", i32, vp, vp)
+_sig("imba_shade_batch_shadow", i32, vp, vp)
 _sig("imba_shade_batch_release", i32, vp)
 _sig("imba_ppm_light", i32, vp, vp, C.c_uint32, i32, i32, vp, vp, vp)
 _sig("imba_ppm_light_bounce", i32, vp, vp, vp, i32, vp, vp, vp, i32, vp, vp, i32, i32, i32, vp, vp, vp, vp)

@@ -364,8 +364,8 @@ def _p(x):
 
 
 class ShadeBatch:
-    """Many queues bounced (Lambertian) and compacted together: items are N.ShadeItem or dicts
-    with its field names (DeviceBuffer / pointer values)."""
+    """Many queues shaded (path-tracing step) and compacted together: items are N.ShadeItem or
+    dicts with its field names (DeviceBuffer / pointer values; "shading" an N.PtShading)."""
 
     def __init__(self, items):
         arr = self._items(items)
@@ -381,10 +381,10 @@ class ShadeBatch:
             e = N.ShadeItem()
             for name, _ in N.ShadeItem._fields_:
                 v = it.get(name)
-                if name in ("capacity", "tri_count", "material_count"):
+                if name == "capacity":
                     setattr(e, name, int(v or 0))
-                elif name == "offset":
-                    setattr(e, name, float(v or 0.0))
+                elif name == "shading":
+                    e.shading = v
                 else:
                     setattr(e, name, _p(v))
             arr[k] = e
@@ -394,8 +394,11 @@ class ShadeBatch:
         arr = self._items(items)
         check(N.lib.imba_shade_batch_update(self.h, arr, len(items), N._s(stream)), "imba_shade_batch_update")
 
-    def launch(self, max_path_len: int, rr_start: int, stream=None):
-        check(N.lib.imba_shade_batch_launch(self.h, max_path_len, rr_start, N._s(stream)), "imba_shade_batch_launch")
+    def launch(self, stream=None):
+        check(N.lib.imba_shade_batch_launch(self.h, N._s(stream)), "imba_shade_batch_launch")
+
+    def shadow(self, stream=None):
+        check(N.lib.imba_shade_batch_shadow(self.h, N._s(stream)), "imba_shade_batch_shadow")
 
     def __del__(self):
         try:

@@ -9,8 +9,10 @@ of the loop:
     the short ones together in one batched launch (imba_trace_batch_update +
     imba_trace_batch_launch), so requests that arrived at different times share launches, and
     each queue long enough to fill the GPU alone in a launch of its own;
-  * the requests' Lambertian bounces and compactions follow in one batched launch
-    (imba_shade_batch_*) with the queue sizes kept on the device, and one read-back of all the
+  * the requests' shading steps (emitter hits, one next-event shadow ray per surface hit,
+    Lambertian bounce) and compactions follow in one batched launch (imba_shade_batch_*) with
+    the queue sizes kept on the device, then the shadow rays' any-hit traversal (batched the
+    same way) and their contributions (imba_shade_batch_shadow), and one read-back of all the
     requests' queue sizes ends the step.
 A request completes when its queue is empty or its paths reach the maximum length; its latency
 is completion minus arrival.  The plain per-request path (`batched=False`) runs the same
@@ -30,8 +32,8 @@ import numpy as np
 
 from . import native as N
 from . import scene as S
+from .workloads import PPM_STATE_BYTES, PT_STATE_BYTES, SHADOW_STATE_BYTES, PtShadingData
 
-STATE_BYTES = 32
 ALIGN = 256
 
 
@@ -70,12 +72,15 @@ class DeviceArena:
 
 
 class ServedScene:
-    """A resident scene with the shading data of the PT bounce (per-triangle normals and
-    materials, per-material albedo and emission)."""
+    """A resident scene with the shading data of the path tracer (per-triangle normals and
+    materials, per-material albedo, emission and emitter area, the triangle lights) and of the
+    photon mapper."""
 
     def __init__(self, scene, cfg: N.TraceConfig | None = None):
         self.s = scene
         self.gpu = S.Scene(scene.desc)
+        self.shading = PtShadingData(scene.tri_normals, scene.tri_materials, scene.albedo, scene.emission,
+                                     scene.emitter_area, scene.lights)
         self.tri_n = N.DeviceBuffer.from_array(scene.tri_normals)
         self.tri_m = N.DeviceBuffer.from_array(scene.tri_materials)
         self.albedo = N.DeviceBuffer.from_array(scene.albedo)
@@ -126,18 +131,20 @@ def first_rays(req: Request) -> int:
 
 
 def footprint(req: Request) -> int:
-    """Device bytes a request holds while active.  PT: two ray and state buffers, hits, image and
-    the compaction workspace.  PPM: the same for max(light paths, samples) entries, the photon
-    slots and the compacted photons, the merge queries and their results."""
+    """Device bytes a request holds while active.  PT: two ray and state buffers, hits, the
+    shadow slots (ray, state, hit), image and the compaction workspace.  PPM: two ray and state
+    buffers and hits for max(light paths, samples) entries, the photon slots and the compacted
+    photons, the merge queries and their results, image and workspace."""
     n = req.samples
     if req.integrator == "pt":
         ws = int(N.lib.imba_compact_workspace_size(n))
-        return 2 * 32 * n + 2 * STATE_BYTES * n + 16 * n + 12 * req.width * req.height + ws + 8 * ALIGN
+        return (2 * 32 * n + 2 * PT_STATE_BYTES * n + 16 * n + (32 + SHADOW_STATE_BYTES + 16) * n +
+                12 * req.width * req.height + ws + 12 * ALIGN)
     p = req.width * req.height
     cap = max(p, n)
     slots = PPM_DEPTH * p
     ws = int(N.lib.imba_compact_workspace_size(max(slots, cap)))
-    return (2 * 32 * cap + 2 * STATE_BYTES * cap + 16 * cap + 2 * 48 * slots + 64 * n + 8 * PPM_K * n + 4 * n +
+    return (2 * 32 * cap + 2 * PPM_STATE_BYTES * cap + 16 * cap + 2 * 48 * slots + 64 * n + 8 * PPM_K * n + 4 * n +
             12 * req.width * req.height + ws + 16 * ALIGN)
 
 
@@ -155,8 +162,9 @@ class _Job:
             return p
 
         self.rays = [take(32 * n), take(32 * n)]
-        self.states = [take(STATE_BYTES * n), take(STATE_BYTES * n)]
+        self.states = [take(PT_STATE_BYTES * n), take(PT_STATE_BYTES * n)]
         self.hits = take(16 * n)
+        self.shadow_rays, self.shadow_states, self.shadow_hits = take(32 * n), take(SHADOW_STATE_BYTES * n), take(16 * n)
         self.image = take(12 * req.width * req.height)
         self.workspace = take(int(N.lib.imba_compact_workspace_size(n)))
         self.bounce = 0
@@ -196,7 +204,7 @@ class _PPMJob:
 
         cap, slots = self.cap, self.L * self.P
         self.rays = [take(32 * cap), take(32 * cap)]
-        self.states = [take(STATE_BYTES * cap), take(STATE_BYTES * cap)]
+        self.states = [take(PPM_STATE_BYTES * cap), take(PPM_STATE_BYTES * cap)]
         self.hits = take(16 * cap)
         self.slot_rays, self.slot_states = take(32 * slots), take(16 * slots)
         self.photon_rays, self.photon_states = take(32 * slots), take(16 * slots)
@@ -273,7 +281,7 @@ class _PPMJob:
                                                 C.c_void_p(self.slot_rays.value + 32 * b * self.P),
                                                 C.c_void_p(self.slot_states.value + 16 * b * self.P), self.rays[cur],
                                                 st), "ppm_light_bounce")
-            N.check(N.lib.imba_compact_rays_dev(self.rays[cur], self.states[cur], STATE_BYTES, self.P, cnt,
+            N.check(N.lib.imba_compact_rays_dev(self.rays[cur], self.states[cur], PPM_STATE_BYTES, self.P, cnt,
                                                 self.rays[1 - cur], self.states[1 - cur], self._light(srv, b + 1),
                                                 self.workspace, st), "compact")
             self.bounce, self.cur = b + 1, 1 - cur
@@ -289,7 +297,7 @@ class _PPMJob:
                                                  sc.tri_m.ptr, sc.s.tri_count, sc.albedo.ptr, sc.emission.ptr,
                                                  sc.kinds.ptr, len(sc.s.albedo), self.L, srv.rr_start, self.req.spp,
                                                  self.queries, self.image, self.rays[cur], st), "ppm_camera_bounce")
-            N.check(N.lib.imba_compact_rays_dev(self.rays[cur], self.states[cur], STATE_BYTES, self.n, cnt,
+            N.check(N.lib.imba_compact_rays_dev(self.rays[cur], self.states[cur], PPM_STATE_BYTES, self.n, cnt,
                                                 self.rays[1 - cur], self.states[1 - cur], self._camera(srv, b + 1),
                                                 self.workspace, st), "compact")
             self.bounce, self.cur = b + 1, 1 - cur
@@ -345,18 +353,20 @@ class Server:
         self.active: list[_Job] = []
         self.done: list[Request] = []
         self.batch = S.TraceBatch([])
+        self.shadow_batch = S.TraceBatch([])
         self.shade = S.ShadeBatch([])
         self.grids: list = []  # hash grids of finished photon-mapping requests, for reuse
         self.steps = 0
         self.t0 = None
 
     def _shade_item(self, j) -> N.ShadeItem:
-        sc = j.sc
         return N.ShadeItem(j.rays[j.cur].value, j.hits.value, j.states[j.cur].value, j.n,
                            self.counts_ptr(j.slot, j.bounce).value, j.rays[1 - j.cur].value,
-                           j.states[1 - j.cur].value, self.counts_ptr(j.slot, j.bounce + 1).value, sc.tri_n.ptr.value,
-                           sc.tri_m.ptr.value, sc.s.tri_count, sc.albedo.ptr.value, sc.emission.ptr.value,
-                           len(sc.s.albedo), sc.s.offset, j.image.value)
+                           j.states[1 - j.cur].value, self.counts_ptr(j.slot, j.bounce + 1).value, j.shadow_rays.value,
+                           j.shadow_states.value, j.shadow_hits.value, j.image.value, self._shading(j.sc))
+
+    def _shading(self, sc) -> N.PtShading:
+        return sc.shading.struct(self.max_path_len, self.rr_start)
 
     def take_grid(self):
         """A hash grid with no table-size history (reused: no allocation once warm)."""
@@ -427,18 +437,32 @@ class Server:
             self.batch.update(shared, self.stream)
             self.batch.launch(self.cfg, self.stream)
         if self.batched and pt_jobs:
-            # every PT request's bounce and compaction in one batched launch
+            # every PT request's shading step and compaction in one batched launch, then every
+            # request's shadow rays (batched as the closest-hit queues are) and their contributions
             self.shade.update([self._shade_item(j) for j in pt_jobs], self.stream)
-            self.shade.launch(self.max_path_len, self.rr_start, self.stream)
+            self.shade.launch(self.stream)
+            sitems = [(j.sc.gpu, j.shadow_rays, j.shadow_hits, j.n, self.counts_ptr(j.slot, j.bounce), True, j.queued)
+                      for j in pt_jobs]
+            for sc, rays, hits, cap, cnt, _, q in sitems:
+                if q >= self.OWN_LAUNCH_RAYS:
+                    N.check(N.lib.imba_occluded_dev(sc.h, rays, hits, cap, cnt, C.byref(self.cfg) if self.cfg else None,
+                                                    st), "occluded_dev")
+            shared_s = [it[:6] for it in sitems if it[6] < self.OWN_LAUNCH_RAYS]
+            if shared_s:
+                self.shadow_batch.update(shared_s, self.stream)
+                self.shadow_batch.launch(self.cfg, self.stream)
+            self.shade.shadow(self.stream)
         elif pt_jobs:
             for j in pt_jobs:
                 sc = j.sc
                 cnt = self.counts_ptr(j.slot, j.bounce)
-                N.check(N.lib.imba_pt_bounce_dev(j.rays[j.cur], j.hits, j.states[j.cur], j.n, cnt, sc.tri_n.ptr,
-                                                 sc.tri_m.ptr, sc.s.tri_count, sc.albedo.ptr, sc.emission.ptr,
-                                                 len(sc.s.albedo), self.max_path_len, self.rr_start, sc.s.offset,
-                                                 j.image, j.rays[j.cur], st), "bounce_dev")
-                N.check(N.lib.imba_compact_rays_dev(j.rays[j.cur], j.states[j.cur], STATE_BYTES, j.n, cnt,
+                sh = self._shading(sc)
+                N.check(N.lib.imba_pt_shade(C.byref(sh), j.rays[j.cur], j.hits, j.states[j.cur], j.n, cnt, j.image,
+                                            j.rays[j.cur], j.shadow_rays, j.shadow_states, st), "pt_shade")
+                N.check(N.lib.imba_occluded_dev(sc.gpu.h, j.shadow_rays, j.shadow_hits, j.n, cnt,
+                                                C.byref(self.cfg) if self.cfg else None, st), "occluded_dev")
+                N.check(N.lib.imba_pt_shadow(j.shadow_hits, j.shadow_states, j.n, cnt, j.image, st), "pt_shadow")
+                N.check(N.lib.imba_compact_rays_dev(j.rays[j.cur], j.states[j.cur], PT_STATE_BYTES, j.n, cnt,
                                                     j.rays[1 - j.cur], j.states[1 - j.cur],
                                                     self.counts_ptr(j.slot, j.bounce + 1), j.workspace, st),
                         "compact_dev")

@@ -7,9 +7,11 @@ sphere_grid_scene: an open box of 12 triangles, a 32 x 32 grid of level-5 icosph
     (20,480 triangles each, radius 0.4, spacing 1.0; 20,971,520 triangles in all), one
     triangular area light; every sphere is its own mesh with its own Lambertian albedo drawn
     uniformly from [0.2, 0.8]^3; SBVH per mesh; 1,026 identity instances.
-PathTracer: a device-side frame loop over that scene: camera rays, closest-hit traversal,
-    Lambertian bounce, queue compaction, up to max_path_len - 1 bounces, Russian roulette
-    from bounce `rr_start` on.
+PathTracer: a device-side frame loop over that scene: camera rays, then per bounce closest-hit
+    traversal, the shading step of PathTracer::process_primary_rays (emitter hits with their MIS
+    weight, one next-event shadow ray per non-emitting hit, Lambertian bounce), any-hit
+    traversal of the shadow rays and their contributions, queue compaction; up to
+    max_path_len - 1 bounces, Russian roulette from bounce `rr_start` on.
 """
 from __future__ import annotations
 
@@ -20,9 +22,17 @@ import numpy as np
 from . import native as N
 from . import scene as S
 
-STATE_BYTES = 32
-# the 32-byte path state of the ray producers (pixel / path, sample, rng, throughput, bounces)
-PT_STATE_DTYPE = np.dtype([("pixel", "<i4"), ("sample", "<i4"), ("rng", "<u8"), ("tp", "<f4", 3), ("bounces", "<i4")])
+# the 40-byte path-tracer state (PTState: pixel, sample, rng, throughput, bounces, last pdf,
+# last specular) and the 16-byte shadow-ray state (pixel, contribution; pixel -1 = empty slot)
+PT_STATE_BYTES = 40
+PT_STATE_DTYPE = np.dtype([("pixel", "<i4"), ("sample", "<i4"), ("rng", "<u8"), ("tp", "<f4", 3), ("bounces", "<i4"),
+                           ("last_pdf", "<f4"), ("last_specular", "<i4")])
+SHADOW_STATE_BYTES = 16
+SHADOW_STATE_DTYPE = np.dtype([("pixel", "<i4"), ("tp", "<f4", 3)])
+# the 32-byte path state of the photon-mapping producers (path / pixel, sample, rng, throughput,
+# bounces)
+PPM_STATE_BYTES = 32
+PPM_STATE_DTYPE = np.dtype([("pixel", "<i4"), ("sample", "<i4"), ("rng", "<u8"), ("tp", "<f4", 3), ("bounces", "<i4")])
 
 C1_CAMERA = ((0.5, 0.5, -1.5), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0), 60.0)
 C1_LIGHT = np.array([0.5, 2.0, 0.5], np.float32)
@@ -140,6 +150,37 @@ def open_box(lo, hi):
     return v, i
 
 
+def triangle_area(p0, p1, p2) -> np.float32:
+    """Tri::area (core/tri.h:20) in float32: length(cross(v1 - v0, v2 - v0)) / 2."""
+    F = np.float32
+    a = np.asarray(p1, F) - np.asarray(p0, F)
+    b = np.asarray(p2, F) - np.asarray(p0, F)
+    c = np.array([a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]], F)
+    return F(np.sqrt((c[0] * c[0] + c[1] * c[1]) + c[2] * c[2]) / F(2))
+
+
+class PtShadingData:
+    """A scene's path-tracing shading data in device memory (N.PtShading): per-triangle
+    geometric normals and materials, per-material albedo, emission and emitter area, and the
+    triangle lights (12 floats each: p0, p1, p2, intensity)."""
+
+    def __init__(self, tri_normals, tri_materials, albedo, emission, emitter_area, lights):
+        F = np.float32
+        self.tri_n = N.DeviceBuffer.from_array(np.ascontiguousarray(tri_normals, F))
+        self.tri_m = N.DeviceBuffer.from_array(np.ascontiguousarray(tri_materials, np.int32))
+        self.albedo = N.DeviceBuffer.from_array(np.ascontiguousarray(albedo, F))
+        self.emission = N.DeviceBuffer.from_array(np.ascontiguousarray(emission, F))
+        self.emitter_area = N.DeviceBuffer.from_array(np.ascontiguousarray(emitter_area, F))
+        lights = np.ascontiguousarray(lights, F).reshape(-1, 12)
+        self.lights = N.DeviceBuffer.from_array(lights if len(lights) else np.zeros((1, 12), F))
+        self.tri_count, self.material_count, self.light_count = len(tri_materials), len(albedo), len(lights)
+
+    def struct(self, max_path_len: int, rr_start: int) -> N.PtShading:
+        return N.PtShading(self.tri_n.ptr.value, self.tri_m.ptr.value, self.tri_count, self.albedo.ptr.value,
+                           self.emission.ptr.value, self.emitter_area.ptr.value, self.material_count,
+                           self.lights.ptr.value, self.light_count, max_path_len, rr_start)
+
+
 class SphereGridScene:
     def __init__(self, side: int = 32, level: int = 5, radius: float = 0.4, spacing: float = 1.0,
                  builder: int = N.BUILDER_SBVH, threads: int = 8, seed: int = 0x5EED, specular_fraction: float = 0.0):
@@ -172,10 +213,15 @@ class SphereGridScene:
         light = n_mat - 1
         lv = np.array([[ext * 0.35, 7.9, ext * 0.35, 1], [ext * 0.65, 7.9, ext * 0.35, 1],
                        [ext * 0.5, 7.9, ext * 0.65, 1]], np.float32)
-        self.build.add_mesh(lv, np.array([[0, 2, 1, light]], np.uint32))
+        # the light triangle is wound so that its geometric normal, cross(v1 - v0, v2 - v0),
+        # faces the scene (down): emitters radiate on that side only (render/light.h:19-39)
+        self.build.add_mesh(lv, np.array([[0, 1, 2, light]], np.uint32))
         self.emission[light] = 40.0
-        # the light's vertices in the order whose normal, cross(v1 - v0, v2 - v0), faces the scene
         self.light_tri = np.ascontiguousarray(lv[:, :3])
+        self.emitter_area = np.zeros(n_mat, np.float32)
+        self.emitter_area[light] = triangle_area(*self.light_tri)
+        # TriangleLight(intensity, p0, p1, p2), as the reference creates one per emissive face
+        self.lights = np.concatenate([self.light_tri.reshape(-1), self.emission[light]]).astype(np.float32)[None, :]
         self.build.run(builder, threads)
         self.desc = self.build.desc()
         self.tri_normals, self.tri_materials = self.build.tri_attrs()
@@ -186,17 +232,25 @@ class SphereGridScene:
 
 class PathTracer:
     """Device-resident PT frame loop over a SphereGridScene (or any scene with per-triangle
-    normals and materials)."""
+    normals and materials, per-material albedo / emission / emitter area and triangle lights).
+
+    Per bounce: closest-hit traversal of the queue, the shading step (imba_pt_shade: emitter
+    contributions, one next-event shadow ray per non-emitting hit into the shadow slots,
+    Lambertian bounce in place), any-hit traversal of the shadow slots, their contributions
+    (imba_pt_shadow), stable compaction of the continuing paths."""
 
     def __init__(self, scene: SphereGridScene, width: int, height: int, spp: int = 1, max_path_len: int = 8,
-                 rr_start: int = 3, seed_base: int = 0x1234567, trace_cfg: N.TraceConfig | None = None):
+                 rr_start: int = 3, seed_base: int = 0x1234567, trace_cfg: N.TraceConfig | None = None,
+                 shadow_cfg: N.TraceConfig | None = None):
         self.s = scene
         self.gpu_scene = S.Scene(scene.desc)
         self.w, self.h, self.spp = width, height, spp
         self.n = width * height * spp
         self.max_path_len, self.rr_start, self.seed_base = max_path_len, rr_start, seed_base
         self.cfg = trace_cfg
+        self.shadow_cfg = shadow_cfg
         self.bounce_cfg = None  # optional per-bounce launch configurations (see cfg_of)
+        self.shadow_bounce_cfg = None
         pos, d, up, fov = scene.camera
         self.cam_pos = np.asarray(pos, np.float32)
         self.r2w = np.zeros(16, np.float32)
@@ -204,13 +258,15 @@ class PathTracer:
         N.check(N.lib.imba_gen_camera_matrix(N.ptr(self.cam_pos), N.ptr(dd), N.ptr(uu), fov, width, height,
                                              N.ptr(self.r2w)), "camera matrix")
         self.rays = [N.DeviceBuffer(self.n * 32), N.DeviceBuffer(self.n * 32)]
-        self.states = [N.DeviceBuffer(self.n * STATE_BYTES), N.DeviceBuffer(self.n * STATE_BYTES)]
+        self.states = [N.DeviceBuffer(self.n * PT_STATE_BYTES), N.DeviceBuffer(self.n * PT_STATE_BYTES)]
         self.hits = N.DeviceBuffer(self.n * 16)
+        self.shadow_rays = N.DeviceBuffer(self.n * 32)
+        self.shadow_states = N.DeviceBuffer(self.n * SHADOW_STATE_BYTES)
+        self.shadow_hits = N.DeviceBuffer(self.n * 16)
         self.counts = None
-        self.tri_n = N.DeviceBuffer.from_array(scene.tri_normals)
-        self.tri_m = N.DeviceBuffer.from_array(scene.tri_materials)
-        self.albedo = N.DeviceBuffer.from_array(scene.albedo)
-        self.emission = N.DeviceBuffer.from_array(scene.emission)
+        self.shading = PtShadingData(scene.tri_normals, scene.tri_materials, scene.albedo, scene.emission,
+                                     scene.emitter_area, scene.lights)
+        self.sh = self.shading.struct(max_path_len, rr_start)
         self.image = N.DeviceBuffer(width * height * 12)
         self.stream = N.Stream()
         # frames add into the image: it starts at zero (device allocations are not cleared)
@@ -221,8 +277,11 @@ class PathTracer:
 
     def frame(self, stats: bool = False, keep_queues: int = 0, timed: bool = False):
         """Runs one frame. Returns per-bounce ray counts, and with stats=True the per-bounce
-        visit-count sums (nodes, triangles, instances); keep_queues > 0 also returns host
-        copies of the first keep_queues ray queues (their states go to self.kept_states)."""
+        visit-count sums (nodes, triangles, instances) of the closest-hit traversal (those of the
+        shadow traversal, over all its slots, go to self.shadow_visit); keep_queues > 0 also
+        returns host copies of the first keep_queues ray queues (their states go to
+        self.kept_states, their shadow slots and any-hit results to self.kept_shadow).
+        self.shadow_counts holds the shadow rays (non-empty slots) of every bounce."""
         st = self.stream.handle
         n = self.n
         N.check(N.lib.imba_pt_camera(N.ptr(self.r2w), N.ptr(self.cam_pos), self.w, self.h, self.spp, self.seed_base, 0,
@@ -230,13 +289,18 @@ class PathTracer:
         cur = 0
         counts, visit, queues, ev = [], [], [], []
         self.kept_states = []  # with keep_queues: the states of the kept queues
+        self.kept_shadow = []  # with keep_queues: (shadow rays, shadow states, shadow hits) of the kept bounces
+        self.shadow_counts = []
+        self.shadow_visit = []
+        self.shadow_events = []
         if stats and self.counts is None:
             self.counts = N.DeviceBuffer(self.n * 12)
         for bounce in range(self.max_path_len):
             if n == 0:
                 break
             counts.append(n)
-            if len(queues) < keep_queues:
+            keep = len(queues) < keep_queues
+            if keep:
                 queues.append(self.rays[cur].download(N.RAY_DTYPE, n, st))
                 self.kept_states.append(self.states[cur].download(PT_STATE_DTYPE, n, st))
             if stats:
@@ -253,12 +317,33 @@ class PathTracer:
                 if timed:
                     e1.record(self.stream)
                     ev.append((e0, e1, n))
-            N.check(N.lib.imba_pt_bounce(self.rays[cur].ptr, self.hits.ptr, self.states[cur].ptr, n, self.tri_n.ptr,
-                                         self.tri_m.ptr, self.s.tri_count, self.albedo.ptr, self.emission.ptr,
-                                         len(self.s.albedo), self.max_path_len, self.rr_start, self.s.offset,
-                                         self.image.ptr, self.rays[cur].ptr, st), "bounce")
+            N.check(N.lib.imba_pt_shade(C.byref(self.sh), self.rays[cur].ptr, self.hits.ptr, self.states[cur].ptr, n,
+                                        None, self.image.ptr, self.rays[cur].ptr, self.shadow_rays.ptr,
+                                        self.shadow_states.ptr, st), "pt_shade")
+            sst = self.shadow_states.download(SHADOW_STATE_DTYPE, n, st)
+            self.shadow_counts.append(int((sst["pixel"] >= 0).sum()))
+            if stats:
+                # visit counts of every slot of the launch (an empty slot tests the root once)
+                N.check(N.lib.imba_trace_stats(self.gpu_scene.h, self.shadow_rays.ptr, self.shadow_hits.ptr,
+                                               self.counts.ptr, n, 1, self._shadow_cfg_ref(bounce), st), "trace_stats")
+                c = self.counts.download(np.uint32, 3 * n, st).reshape(-1, 3)
+                self.shadow_visit.append(c.astype(np.int64).sum(0))
+            else:
+                if timed:
+                    e0, e1 = N.Event(), N.Event()
+                    e0.record(self.stream)
+                N.check(N.lib.imba_occluded(self.gpu_scene.h, self.shadow_rays.ptr, self.shadow_hits.ptr, n,
+                                            self._shadow_cfg_ref(bounce), st), "occluded")
+                if timed:
+                    e1.record(self.stream)
+                    self.shadow_events.append((e0, e1, n))
+            if keep:
+                self.kept_shadow.append((self.shadow_rays.download(N.RAY_DTYPE, n, st), sst,
+                                         self.shadow_hits.download(N.HIT_DTYPE, n, st)))
+            N.check(N.lib.imba_pt_shadow(self.shadow_hits.ptr, self.shadow_states.ptr, n, None, self.image.ptr, st),
+                    "pt_shadow")
             nn = C.c_int()
-            N.check(N.lib.imba_compact_rays(self.rays[cur].ptr, self.states[cur].ptr, STATE_BYTES, n,
+            N.check(N.lib.imba_compact_rays(self.rays[cur].ptr, self.states[cur].ptr, PT_STATE_BYTES, n,
                                             self.rays[1 - cur].ptr, self.states[1 - cur].ptr, C.byref(nn), st),
                     "compact_rays")
             n = nn.value
@@ -269,6 +354,9 @@ class PathTracer:
     def trace_times_ms(self):
         return [(e0.elapsed_ms(e1), n) for e0, e1, n in self.events]
 
+    def shadow_times_ms(self):
+        return [(e0.elapsed_ms(e1), n) for e0, e1, n in self.shadow_events]
+
     def cfg_of(self, bounce: int):
         """Launch configuration of a bounce's traversal: bounce_cfg[bounce] when set (the queues
         of a frame shrink from bounce to bounce), else cfg."""
@@ -276,8 +364,20 @@ class PathTracer:
             return self.bounce_cfg[bounce]
         return self.cfg
 
+    def shadow_cfg_of(self, bounce: int):
+        """Launch configuration of a bounce's shadow-ray traversal (shadow_bounce_cfg, else
+        shadow_cfg, else the closest-hit configuration)."""
+        if (self.shadow_bounce_cfg is not None and bounce < len(self.shadow_bounce_cfg)
+                and self.shadow_bounce_cfg[bounce] is not None):
+            return self.shadow_bounce_cfg[bounce]
+        return self.shadow_cfg if self.shadow_cfg is not None else self.cfg_of(bounce)
+
     def _cfg_ref(self, bounce: int):
         c = self.cfg_of(bounce)
+        return C.byref(c) if c else None
+
+    def _shadow_cfg_ref(self, bounce: int):
+        c = self.shadow_cfg_of(bounce)
         return C.byref(c) if c else None
 
     # -- device-resident frame: counts stay on the GPU, the frame is one graph launch ----------
@@ -296,11 +396,14 @@ class PathTracer:
             else:
                 N.check(N.lib.imba_intersect_dev(self.gpu_scene.h, self.rays[cur].ptr, self.hits.ptr, n, cnt,
                                                  self._cfg_ref(bounce), st), "intersect_dev")
-            N.check(N.lib.imba_pt_bounce_dev(self.rays[cur].ptr, self.hits.ptr, self.states[cur].ptr, n, cnt,
-                                             self.tri_n.ptr, self.tri_m.ptr, self.s.tri_count, self.albedo.ptr,
-                                             self.emission.ptr, len(self.s.albedo), self.max_path_len, self.rr_start,
-                                             self.s.offset, self.image.ptr, self.rays[cur].ptr, st), "bounce_dev")
-            N.check(N.lib.imba_compact_rays_dev(self.rays[cur].ptr, self.states[cur].ptr, STATE_BYTES, n, cnt,
+            N.check(N.lib.imba_pt_shade(C.byref(self.sh), self.rays[cur].ptr, self.hits.ptr, self.states[cur].ptr, n,
+                                        cnt, self.image.ptr, self.rays[cur].ptr, self.shadow_rays.ptr,
+                                        self.shadow_states.ptr, st), "pt_shade")
+            N.check(N.lib.imba_occluded_dev(self.gpu_scene.h, self.shadow_rays.ptr, self.shadow_hits.ptr, n, cnt,
+                                            self._shadow_cfg_ref(bounce), st), "occluded_dev")
+            N.check(N.lib.imba_pt_shadow(self.shadow_hits.ptr, self.shadow_states.ptr, n, cnt, self.image.ptr, st),
+                    "pt_shadow")
+            N.check(N.lib.imba_compact_rays_dev(self.rays[cur].ptr, self.states[cur].ptr, PT_STATE_BYTES, n, cnt,
                                                 self.rays[1 - cur].ptr, self.states[1 - cur].ptr,
                                                 self.dev_counts.at(4 * (bounce + 1)), self.workspace.ptr, st),
                     "compact_rays_dev")
@@ -395,7 +498,7 @@ class PhotonMapper:
         cap = max(self.P, self.n_cam)
         self.cap = cap
         self.rays = [N.DeviceBuffer(cap * 32), N.DeviceBuffer(cap * 32)]
-        self.states = [N.DeviceBuffer(cap * STATE_BYTES), N.DeviceBuffer(cap * STATE_BYTES)]
+        self.states = [N.DeviceBuffer(cap * PPM_STATE_BYTES), N.DeviceBuffer(cap * PPM_STATE_BYTES)]
         self.hits = N.DeviceBuffer(cap * 16)
         slots = self.L * self.P
         self.slots = slots
@@ -437,7 +540,7 @@ class PhotonMapper:
         self.stream.sync()
         n = int(c[0])
         rays = self.rays[cur].download(N.RAY_DTYPE, n, self.stream)
-        states = self.states[cur].download(PT_STATE_DTYPE, n, self.stream)
+        states = self.states[cur].download(PPM_STATE_DTYPE, n, self.stream)
         self.stream.sync()
         self.kept[which].append((rays, states))
 
@@ -472,7 +575,7 @@ class PhotonMapper:
                                                 self.kinds.ptr, len(self.s.albedo), L, self.rr_start,
                                                 self.slot_rays.at(32 * b * P), self.slot_states.at(16 * b * P),
                                                 self.rays[cur].ptr, st), "ppm_light_bounce")
-            N.check(N.lib.imba_compact_rays_dev(self.rays[cur].ptr, self.states[cur].ptr, STATE_BYTES, P, cnt,
+            N.check(N.lib.imba_compact_rays_dev(self.rays[cur].ptr, self.states[cur].ptr, PPM_STATE_BYTES, P, cnt,
                                                 self.rays[1 - cur].ptr, self.states[1 - cur].ptr, self._cnt(b + 1),
                                                 self.workspace.ptr, st), "compact_rays_dev")
             cur = 1 - cur
@@ -509,7 +612,7 @@ class PhotonMapper:
                                                  self.emission.ptr, self.kinds.ptr, len(self.s.albedo), L,
                                                  self.rr_start, self.spp, self.queries.ptr, self.image.ptr,
                                                  self.rays[cur].ptr, st), "ppm_camera_bounce")
-            N.check(N.lib.imba_compact_rays_dev(self.rays[cur].ptr, self.states[cur].ptr, STATE_BYTES, n, cnt,
+            N.check(N.lib.imba_compact_rays_dev(self.rays[cur].ptr, self.states[cur].ptr, PPM_STATE_BYTES, n, cnt,
                                                 self.rays[1 - cur].ptr, self.states[1 - cur].ptr,
                                                 self._cnt(L + 2 + b), self.workspace.ptr, st), "compact_rays_dev")
             cur = 1 - cur
@@ -588,8 +691,8 @@ class QueueLoop:
     (imba_shade_batch_launch), queue sizes kept on the device.  A step restores every queue's
     first rays and states and runs max_path_len bounces; it is captured once into a graph.
 
-    shading: per scene (tri_normals, tri_materials, albedo, emission, offset); queues: list of
-    (scene index, rays DeviceBuffer, ray count)."""
+    shading: per scene a PtShadingData (scenes without lights: no next-event rays); queues:
+    list of (scene index, rays DeviceBuffer, ray count)."""
 
     def __init__(self, scenes: list, shading: list, queues: list, max_path_len: int = 8, rr_start: int = 3,
                  cfg: N.TraceConfig | None = None, seed: int = 0x5EED):
@@ -598,11 +701,8 @@ class QueueLoop:
         self.L, self.rr_start, self.cfg = max_path_len, rr_start, cfg
         self.stream = N.Stream()
         self.scenes = scenes
-        self.shade_bufs = [(N.DeviceBuffer.from_array(np.ascontiguousarray(n, np.float32)),
-                            N.DeviceBuffer.from_array(np.ascontiguousarray(m, np.int32)), len(m),
-                            N.DeviceBuffer.from_array(np.ascontiguousarray(a, np.float32)),
-                            N.DeviceBuffer.from_array(np.ascontiguousarray(e, np.float32)), len(a), float(off))
-                           for n, m, a, e, off in shading]
+        self.shading = shading
+        self.sh = [d.struct(max_path_len, rr_start) for d in shading]
         self.q = []
         L1 = max_path_len + 1
         self.counts = N.DeviceBuffer(4 * L1 * max(len(queues), 1))
@@ -615,7 +715,7 @@ class QueueLoop:
             st["tp"] = 1.0
             self.q.append({"scene": si, "n": n, "rays0": rays0, "states0": N.DeviceBuffer.from_array(st),
                            "rays": [N.DeviceBuffer(max(n, 1) * 32), N.DeviceBuffer(max(n, 1) * 32)],
-                           "states": [N.DeviceBuffer(max(n, 1) * STATE_BYTES), N.DeviceBuffer(max(n, 1) * STATE_BYTES)],
+                           "states": [N.DeviceBuffer(max(n, 1) * PT_STATE_BYTES), N.DeviceBuffer(max(n, 1) * PT_STATE_BYTES)],
                            "hits": N.DeviceBuffer(max(n, 1) * 16)})
             counts0[k * L1] = n
         self.counts0 = N.DeviceBuffer.from_array(counts0)
@@ -628,11 +728,10 @@ class QueueLoop:
                                            False) for k, e in enumerate(self.q)]))
             items = []
             for k, e in enumerate(self.q):
-                tn, tm, nt, al, em, nm, off = self.shade_bufs[e["scene"]]
                 items.append(N.ShadeItem(e["rays"][cur].ptr.value, e["hits"].ptr.value, e["states"][cur].ptr.value,
                                          e["n"], self._cnt(k, b).value, e["rays"][nxt].ptr.value,
-                                         e["states"][nxt].ptr.value, self._cnt(k, b + 1).value, tn.ptr.value,
-                                         tm.ptr.value, nt, al.ptr.value, em.ptr.value, nm, off, None))
+                                         e["states"][nxt].ptr.value, self._cnt(k, b + 1).value, None, None, None,
+                                         None, self.sh[e["scene"]]))
             self.shade.append(ShadeBatch(items))
         self.graph = None
 
@@ -645,10 +744,10 @@ class QueueLoop:
         for e in self.q:
             if e["n"]:
                 N.check(N.lib.imba_memcpy_d2d(e["rays"][0].ptr, e["rays0"].ptr, 32 * e["n"], st), "rays")
-                N.check(N.lib.imba_memcpy_d2d(e["states"][0].ptr, e["states0"].ptr, STATE_BYTES * e["n"], st), "states")
+                N.check(N.lib.imba_memcpy_d2d(e["states"][0].ptr, e["states0"].ptr, PT_STATE_BYTES * e["n"], st), "states")
         for b in range(self.L):
             self.trace[b].launch(self.cfg, self.stream)
-            self.shade[b].launch(self.L, self.rr_start, self.stream)
+            self.shade[b].launch(self.stream)
 
     def build_graph(self):
         from .scene import Graph

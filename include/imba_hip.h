@@ -439,55 +439,76 @@ int imba_hashgrid_reset(ImbaHashGrid* g);
 int imba_hashgrid_gather(const ImbaHashGrid* g, const void* src, int stride, void* dst, void* stream);
 
 /* ------------------------------ path-tracing ray producers --------------------------------
- * Device-side producers of the queues a path tracer traverses (the shading that the
- * reference runs on the host around RayQueue, render/integrators/pt.cpp).  States are
- * 32-byte records: int pixel_id, int sample_id, uint64 rng, float throughput[3], int bounces.
+ * Device-side producers of the queues a path tracer traverses: the shading that the reference
+ * runs on the host around RayQueue (render/integrators/pt.cpp), for the materials of the
+ * measured scenes (Lambertian surfaces, triangle emitters) and triangle lights.
+ * States are 40-byte records (PTState): int pixel_id, int sample_id, uint64 rng,
+ * float throughput[3], int bounces, float last_pdf, int last_specular.  Shadow states are
+ * 16-byte records: int pixel_id (-1: empty slot), float contribution[3].
  * imba_pt_camera: samples first..first+count-1 of a width x height x spp image; sample i is
  *   pixel i / spp, seeded as render/ray_gen/ray_gen.h:51-72 with seed_base, jittered as
  *   pt.cpp:168-177 and shot through raster_to_world (imba_gen_camera_matrix).
- * imba_pt_bounce: for every ray of the queue and its hit: emission (added to image, may be
- *   NULL) ends the path, otherwise a Lambertian bounce (pt.cpp:46-80) with Russian roulette
- *   from bounce rr_start on; the new ray starts at the hit with tmin = offset.  Ended paths get
- *   pixel_id = -1.  rays_out may equal rays. */
+ * imba_pt_shade: PathTracer::process_primary_rays (pt.cpp:82-160) for every entry of the queue
+ *   and its hit: a miss ends the path; an emitter adds throughput * radiance * MIS weight to
+ *   image and ends the path; otherwise next-event estimation (pt.cpp:17-43) writes the shadow
+ *   ray of entry i to slot i of shadow_rays / shadow_states (both NULL: none), and a Lambertian
+ *   bounce (pt.cpp:46-80) writes the continuation ray to rays_out (may equal rays) with
+ *   tmin = 1e-3 t.  Russian roulette draws its number on every bounce and applies from bounce
+ *   rr_start on (0: every bounce, as the reference).  Ended paths get pixel_id = -1.
+ * imba_pt_shadow: process_shadow_rays (integrator.h:51-66): every non-empty slot whose shadow
+ *   hit has tri_id < 0 adds its contribution to image.
+ * count_dev (nullable): the queue size read on the device when the launch runs (clamped to
+ * [0, capacity]); without it `capacity` entries. */
+typedef struct ImbaPtShading {
+    const float* tri_normals;   /* device, 3 per global triangle: the geometric normal
+                                   normalize(cross(v1 - v0, v2 - v0)), also the shading normal */
+    const int* tri_materials;   /* device, material of each global triangle                  */
+    int tri_count;
+    const float* albedo;        /* device, 3 per material: Lambertian reflectance            */
+    const float* emission;      /* device, 3 per material: emitted radiance (0: no emitter)  */
+    const float* emitter_area;  /* device, 1 per material: area of its emitting triangle     */
+    int material_count;
+    const float* lights;        /* device, 12 per triangle light: p0, p1, p2, intensity
+                                   (TriangleLight, render/light.h:102-170)                  */
+    int light_count;            /* 0: no next-event estimation                               */
+    int max_path_len;
+    int rr_start;
+} ImbaPtShading;
+
 int imba_pt_camera(const float* raster_to_world16, const float* cam_pos3, int width, int height, int spp,
                    uint32_t seed_base, int first, int count, ImbaRay* rays, void* states, void* stream);
-int imba_pt_bounce(const ImbaRay* rays, const ImbaHit* hits, void* states, int count, const float* tri_normals,
-                   const int* tri_materials, int tri_count, const float* albedo, const float* emission,
-                   int material_count, int max_path_len, int rr_start, float offset, float* image,
-                   ImbaRay* rays_out, void* stream);
-/* Same with the queue size read from the device int *count_dev (clamped to [0, capacity])
- * when the launch runs. */
-int imba_pt_bounce_dev(const ImbaRay* rays, const ImbaHit* hits, void* states, int capacity, const int* count_dev,
-                       const float* tri_normals, const int* tri_materials, int tri_count, const float* albedo,
-                       const float* emission, int material_count, int max_path_len, int rr_start, float offset,
-                       float* image, ImbaRay* rays_out, void* stream);
+int imba_pt_shade(const ImbaPtShading* shading, const ImbaRay* rays, const ImbaHit* hits, void* states,
+                  int capacity, const int* count_dev, float* image, ImbaRay* rays_out, ImbaRay* shadow_rays,
+                  void* shadow_states, void* stream);
+int imba_pt_shadow(const ImbaHit* shadow_hits, const void* shadow_states, int capacity, const int* count_dev,
+                   float* image, void* stream);
 
-/* ------------------------------ batched bounce + compaction -------------------------------
- * One Lambertian bounce (imba_pt_bounce) and the stable compaction of the survivors
- * (imba_compact_rays, state stride 32) for many queues at once, in five launches whatever the
+/* ------------------------------ batched shading + compaction ------------------------------
+ * One path-tracing shading step (imba_pt_shade) and the stable compaction of the survivors
+ * (imba_compact_rays, state stride 40) for many queues at once, in five launches whatever the
  * number of queues: the queues' device counts are read when the launch runs, so a bounce loop
  * over many queues and scenes can be captured in one graph (nothing in it allocates or sets
- * memory).  For every item the result equals imba_pt_bounce_dev + imba_compact_rays_dev on that
- * queue alone: rays/states are bounced in place, survivors go to rays_out/states_out in queue
- * order and their number to *next_count_dev (when non-null).  The reference's schedulers shade
- * and compact each queue on the host (queue_scheduler.h:177-275, ray_queue.h:188-207). */
+ * memory).  For every item the result equals imba_pt_shade + imba_compact_rays_dev on that
+ * queue alone: rays/states are shaded in place, survivors go to rays_out/states_out in queue
+ * order and their number to *next_count_dev (when non-null), shadow rays to the item's shadow
+ * slots.  imba_shade_batch_shadow then adds every item's unoccluded shadow contributions
+ * (imba_pt_shadow over shadow_hits, same entry counts as the preceding launch).  The
+ * reference's schedulers shade and compact each queue on the host (queue_scheduler.h:177-275,
+ * ray_queue.h:188-207). */
 typedef struct ImbaShadeItem {
-    ImbaRay* rays;             /* the queue (bounced in place)                                */
+    ImbaRay* rays;             /* the queue (shaded in place)                                 */
     const ImbaHit* hits;
-    void* states;              /* 32-byte path-tracing states                                */
+    void* states;              /* 40-byte path-tracing states                                */
     int capacity;              /* entries the buffers hold                                   */
     const int* count_dev;      /* device count of the queue (NULL: capacity)                 */
     ImbaRay* rays_out;         /* compacted survivors                                        */
     void* states_out;
     int* next_count_dev;       /* device count of the survivors (nullable)                   */
-    const float* tri_normals;  /* shading data of the queue's scene, as imba_pt_bounce       */
-    const int* tri_materials;
-    int tri_count;
-    const float* albedo;
-    const float* emission;
-    int material_count;
-    float offset;
+    ImbaRay* shadow_rays;      /* shadow slots, one per entry (NULL: no next-event rays)     */
+    void* shadow_states;
+    const ImbaHit* shadow_hits;/* the shadow slots' any-hit results (imba_shade_batch_shadow) */
     float* image;              /* nullable                                                   */
+    ImbaPtShading shading;     /* the queue's scene                                          */
 } ImbaShadeItem;
 
 typedef struct ImbaShadeBatch ImbaShadeBatch;
@@ -495,14 +516,15 @@ int imba_shade_batch_create(const ImbaShadeItem* items, int count, ImbaShadeBatc
 /* Replaces the items (table copied through pinned memory on `stream`; the host array may be
  * reused when the call returns). */
 int imba_shade_batch_update(ImbaShadeBatch* b, const ImbaShadeItem* items, int count, void* stream);
-int imba_shade_batch_launch(ImbaShadeBatch* b, int max_path_len, int rr_start, void* stream);
+int imba_shade_batch_launch(ImbaShadeBatch* b, void* stream);
+int imba_shade_batch_shadow(ImbaShadeBatch* b, void* stream);
 int imba_shade_batch_release(ImbaShadeBatch* b);
 
 /* ------------------------------ photon-mapping ray producers ------------------------------
  * The PPM mode of the VCM integrator (render/integrators/vcm.cpp, algo == ALGO_PPM) on the
  * device: light paths deposit photons that HashGrid indexes, camera paths end in merge queries
- * answered by imba_hashgrid_query.  States are the 32-byte path-tracing states above
- * (pixel_id = light path index for light paths).  Material kinds: 0 Lambertian, 1 mirror,
+ * answered by imba_hashgrid_query.  States are 32-byte records: int pixel_id (the light path
+ * index for light paths), int sample_id, uint64 rng, float throughput[3], int bounces.  Material kinds: 0 Lambertian, 1 mirror,
  * 2 glass (index of refraction 1.5).
  * imba_ppm_light: light paths first..first+count-1 from the triangle light_tri9 (3 vertices;
  *   it emits around cross(v1 - v0, v2 - v0)), seeded as LightRayGen

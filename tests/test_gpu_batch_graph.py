@@ -1,16 +1,20 @@
 """Batched multi-queue traversal, device-count launches, graph-captured frames and the launch
 autotuner (GPU).
 
-Every path must give, per queue, exactly the hits of a plain single-queue launch (which the
-other GPU tests pin to the oracle): the batch cuts queues into 64-ray chunks and runs the same
-per-ray steps; the device-count variants read the queue size when they run; a graph replays
-the very launches of an eager frame.
+Every queue's hits are checked against the oracle's traversal of the same rays: closest hits
+bit for bit on every tree (the closest hit does not depend on the traversal order), any hits
+bit for bit on the BVH2 (the oracle walks the BVH2 in the kernel's order) and, on the 4-wide
+trees, whose order differs, the occlusion answer plus bit equality with a single-queue launch
+of the same tree.  The device-count variants read the queue size when they run; a graph
+replays the very launches of an eager frame (whose every bounce the path-tracing image test
+checks against the oracle and the restated integrator).
 """
 import ctypes as C
 
 import numpy as np
 import pytest
 
+import oracle
 from imbatracer_amd import native as N
 from imbatracer_amd import scene as S
 from imbatracer_amd import workloads as W
@@ -26,10 +30,23 @@ def _same(a, b):
 
 
 def _scenes():
+    """(device scene, host arrays) of a random-triangle, an instanced and a masked scene."""
     b0, _, _ = T.random_tri_scene(20000, N.BUILDER_SBVH)
     b1 = T.instanced_scene()
     b2 = T.masked_scene()
-    return [S.Scene(b.desc()) for b in (b0, b1, b2)]
+    return [(S.Scene(b.desc()), b.arrays()) for b in (b0, b1, b2)]
+
+
+def _check(got, rays, arrays, any_hit, sc, cfg):
+    """got = the hits of `rays` under launch configuration cfg, checked against the oracle."""
+    want, bad = oracle.trace(arrays, rays, any_hit=any_hit, threads=8)
+    assert bad == 0
+    wide = cfg is not None and cfg.bvh_width == 4
+    if not any_hit or not wide:
+        return _same(got, want)
+    # any hit on a 4-wide tree: the same occlusion answer, and the single-queue launch's hit
+    return (np.array_equal(got["tri_id"] < 0, want["tri_id"] < 0) and
+            _same(got, sc.trace_host(rays, any_hit=True, cfg=cfg)))
 
 
 def _filled_hits(n):
@@ -54,12 +71,12 @@ def test_batch_matches_single_launches(gpu, cfg):
     sizes = [0, 1, 63, 64, 65, 1000, 4097, 20000, 333, 7000]
     queues = []
     for k, n in enumerate(sizes):
-        sc = scenes[k % 3]
+        sc, arrays = scenes[k % 3]
         any_hit = k % 4 == 3
         rays = T.random_rays(max(n, 1), seed=100 + k, box=(-2.0, 2.0))[:n] if k % 3 == 1 else \
             (T.primary_rays(128, 128)[:n] if k % 3 == 2 else T.random_rays(max(n, 1), seed=100 + k)[:n])
         rb = N.DeviceBuffer.from_array(rays) if n else N.DeviceBuffer(32)
-        queues.append((sc, rb, _filled_hits(n), n, None, any_hit, rays))
+        queues.append((sc, rb, _filled_hits(n), n, None, any_hit, rays, arrays))
     tc = _cfg(cfg)
     single = _cfg(cfg[:4]) if cfg and len(cfg) > 4 else tc  # the same tree without a treelet
     batch = S.TraceBatch([q[:6] for q in queues])
@@ -68,13 +85,12 @@ def test_batch_matches_single_launches(gpu, cfg):
             if q[3]:
                 q[2].upload(np.repeat(SENTINEL, q[3]))
         batch.launch(tc)
-        for sc, rb, hb, n, _, any_hit, rays in queues:
+        for sc, rb, hb, n, _, any_hit, rays, arrays in queues:
             if n == 0:
                 continue
             got = hb.download(N.HIT_DTYPE, n)
-            want = sc.trace_host(rays, any_hit=any_hit, cfg=single)
-            assert _same(got, want), (rep, n, any_hit)
-    for sc in scenes:
+            assert _check(got, rays, arrays, any_hit, sc, single), (rep, n, any_hit)
+    for sc, _ in scenes:
         assert sc.error_count() == 0
 
 
@@ -101,17 +117,19 @@ def test_batch_device_counts_and_many_queues(gpu, cfg):
     batch = S.TraceBatch(items)
     batch.launch(_cfg(cfg))
     single = _cfg(cfg[:4]) if cfg else None
+    arrays = [b.arrays() for b in bs]
     for q, (sc, rb, hb, cap, _, any_hit) in enumerate(items):
         got = hb.download(N.HIT_DTYPE, cap)
         n = int(counts[q])
         if n:
-            assert _same(got[:n], sc.trace_host(expect[q][:n], any_hit=any_hit, cfg=single)), q
+            assert _check(got[:n], expect[q][:n], arrays[q % 8], any_hit, sc, single), q
         assert _same(got[n:], np.repeat(SENTINEL, cap - n)), q
 
 
 def test_intersect_dev_and_compact_dev(gpu):
     b, _, _ = T.random_tri_scene(10000, N.BUILDER_SBVH)
     sc = S.Scene(b.desc())
+    arrays = b.arrays()
     rays = T.random_rays(5000, seed=3)
     rb = N.DeviceBuffer.from_array(rays)
     for any_hit in (False, True):
@@ -120,7 +138,7 @@ def test_intersect_dev_and_compact_dev(gpu):
         fn = N.lib.imba_occluded_dev if any_hit else N.lib.imba_intersect_dev
         N.check(fn(sc.h, rb.ptr, hb.ptr, 5000, cnt.ptr, None, None), "dev")
         got = hb.download(N.HIT_DTYPE, 5000)
-        assert _same(got[:3001], sc.trace_host(rays[:3001], any_hit=any_hit))
+        assert _same(got[:3001], oracle.trace(arrays, rays[:3001], any_hit=any_hit)[0])
         assert _same(got[3001:], np.repeat(SENTINEL, 1999))
     # compaction with device counts equals the host-count version on the first *count entries
     states = np.zeros((5000, 8), np.int32)
@@ -164,7 +182,8 @@ def test_graph_frame_replays_eager_frame(gpu, small_pt, use_batch):
 
     def buffers():
         out = [b.download(np.uint8, b.nbytes) for b in pt.rays + pt.states]
-        return out + [pt.hits.download(np.uint8, pt.hits.nbytes)]
+        return out + [b.download(np.uint8, b.nbytes) for b in (pt.hits, pt.shadow_rays, pt.shadow_states,
+                                                                pt.shadow_hits)]
 
     for rep in range(3):
         # eager work and synchronisation between replays
@@ -194,4 +213,4 @@ def test_autotune_returns_a_supported_shape(gpu):
     cfg, ms = sc.autotune(rb, hb, 50000, reps=2)
     assert (cfg.block_size, cfg.stack_lds) in [(256, 16), (256, 8), (256, 32), (128, 16), (128, 32), (64, 32), (64, 64)]
     assert cfg.persistent in (0, 32) and ms > 0
-    assert _same(sc.trace_host(rays, cfg=cfg), sc.trace_host(rays))
+    assert _same(sc.trace_host(rays, cfg=cfg), oracle.trace(b.arrays(), rays, threads=8)[0])

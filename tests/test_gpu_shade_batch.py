@@ -1,11 +1,15 @@
-"""Batched bounce + compaction (imba_shade_batch_*) against the single-queue calls.
+"""Batched shading + compaction (imba_shade_batch_*) against the single-queue calls.
 
 Several queues over different scenes, of different sizes (empty, smaller than one chunk,
 spanning many compaction tiles), some with their size on the device and below the capacity:
-after one batched launch every queue's bounced rays and states, its compacted survivors, its
-new device count and its image equal those of imba_pt_bounce_dev + imba_compact_rays_dev run
-on that queue alone, bit for bit.  The launch is also replayed from a graph.
+after one batched launch every queue's shaded rays and states, its shadow slots, its compacted
+survivors and its new device count equal those of imba_pt_shade + imba_compact_rays_dev run on
+that queue alone, bit for bit; after the shadow rays' any-hit traversal, imba_shade_batch_shadow
+gives every queue the image of imba_pt_shadow (within 1e-6: a pixel's samples are added with
+float atomics in no fixed order).  The launch is also replayed from a graph.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -15,7 +19,7 @@ from imbatracer_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-STATE = W.STATE_BYTES
+STATE = W.PT_STATE_BYTES
 
 
 class Queue:
@@ -38,10 +42,8 @@ class Queue:
             N.check(N.lib.imba_pt_camera(N.ptr(self.r2w), N.ptr(self.pos), width, height, spp, seed, 0, self.n,
                                          self.rays0.ptr, self.states0.ptr, None), "camera")
             N.check(N.lib.imba_intersect(self.gpu.h, self.rays0.ptr, self.hits.ptr, self.n, None, None), "trace")
-        self.tri_n = N.DeviceBuffer.from_array(scene.tri_normals)
-        self.tri_m = N.DeviceBuffer.from_array(scene.tri_materials)
-        self.albedo = N.DeviceBuffer.from_array(scene.albedo)
-        self.emission = N.DeviceBuffer.from_array(scene.emission)
+        self.shading = W.PtShadingData(scene.tri_normals, scene.tri_materials, scene.albedo, scene.emission,
+                                       scene.emitter_area, scene.lights)
         self.cnt = N.DeviceBuffer(8)
         self.cnt.upload(np.array([count, -1], np.int32))
 
@@ -57,35 +59,52 @@ class Queue:
         N.check(N.lib.imba_memset(img.ptr, 0, img.nbytes, None), "memset")
         nxt = N.DeviceBuffer(4)
         N.check(N.lib.imba_memset(nxt.ptr, 0xFF, 4, None), "memset")
-        return r, s, ro, so, img, nxt
+        sr, ss, shh = N.DeviceBuffer(n * 32), N.DeviceBuffer(n * 16), N.DeviceBuffer(n * 16)
+        return r, s, ro, so, img, nxt, sr, ss, shh
 
-    def item(self, bufs):
-        r, s, ro, so, img, nxt = bufs
+    def item(self, bufs, max_len, rr):
+        r, s, ro, so, img, nxt, sr, ss, shh = bufs
         return {"rays": r, "hits": self.hits, "states": s, "capacity": self.n, "count_dev": self.cnt,
-                "rays_out": ro, "states_out": so, "next_count_dev": nxt, "tri_normals": self.tri_n,
-                "tri_materials": self.tri_m, "tri_count": self.sc.tri_count, "albedo": self.albedo,
-                "emission": self.emission, "material_count": len(self.sc.albedo), "offset": self.sc.offset,
-                "image": img}
+                "rays_out": ro, "states_out": so, "next_count_dev": nxt, "shadow_rays": sr, "shadow_states": ss,
+                "shadow_hits": shh, "image": img, "shading": self.shading.struct(max_len, rr)}
 
     def single(self, bufs, max_len, rr):
-        r, s, ro, so, img, nxt = bufs
+        r, s, ro, so, img, nxt, sr, ss, shh = bufs
         if self.n == 0:
             return
-        N.check(N.lib.imba_pt_bounce_dev(r.ptr, self.hits.ptr, s.ptr, self.n, self.cnt.ptr, self.tri_n.ptr,
-                                         self.tri_m.ptr, self.sc.tri_count, self.albedo.ptr, self.emission.ptr,
-                                         len(self.sc.albedo), max_len, rr, self.sc.offset, img.ptr, r.ptr, None),
-                "bounce")
+        sh = self.shading.struct(max_len, rr)
+        N.check(N.lib.imba_pt_shade(C.byref(sh), r.ptr, self.hits.ptr, s.ptr, self.n, self.cnt.ptr, img.ptr, r.ptr,
+                                    sr.ptr, ss.ptr, None), "shade")
+        self.occluded(bufs)
+        N.check(N.lib.imba_pt_shadow(shh.ptr, ss.ptr, self.n, self.cnt.ptr, img.ptr, None), "shadow")
         ws = N.DeviceBuffer(int(N.lib.imba_compact_workspace_size(self.n)))
         N.check(N.lib.imba_compact_rays_dev(r.ptr, s.ptr, STATE, self.n, self.cnt.ptr, ro.ptr, so.ptr, nxt.ptr,
                                             ws.ptr, None), "compact")
 
+    def occluded(self, bufs, stream=None):
+        r, s, ro, so, img, nxt, sr, ss, shh = bufs
+        if self.n:
+            N.check(N.lib.imba_occluded_dev(self.gpu.h, sr.ptr, shh.ptr, self.n, self.cnt.ptr, None, N._s(stream)),
+                    "occluded")
+
     def host(self, bufs):
-        r, s, ro, so, img, nxt = bufs
+        r, s, ro, so, img, nxt, sr, ss, shh = bufs
         n = max(self.n, 1)
         c = int(nxt.download(np.int32, 1)[0])
+        m = min(max(int(self.cnt.download(np.int32, 1)[0]), 0), self.n)
         return (r.download(np.uint8, n * 32), s.download(np.uint8, n * STATE), c,
                 ro.download(np.uint8, max(c, 0) * 32), so.download(np.uint8, max(c, 0) * STATE),
-                img.download(np.uint8, img.nbytes))
+                sr.download(np.uint8, m * 32), ss.download(np.uint8, m * 16), shh.download(np.uint8, m * 16),
+                img.download(np.float32, img.nbytes // 4))
+
+
+def _same(g, w):
+    """Everything bit for bit but the image (last element), which is compared within 1e-6."""
+    assert g[2] == w[2]
+    for x, y in zip(g[:-1], w[:-1]):
+        if isinstance(x, np.ndarray):
+            assert np.array_equal(x, y)
+    assert np.allclose(g[-1], w[-1], rtol=1e-6, atol=0)
 
 
 @pytest.fixture(scope="module")
@@ -108,34 +127,44 @@ def test_batch_equals_single_queues(queues, max_len, rr):
         q.single(bufs, max_len, rr)
         want.append(q.host(bufs))
     bufs = [q.fresh() for q in queues]
-    batch = S.ShadeBatch([q.item(b) for q, b in zip(queues, bufs)])
-    batch.launch(max_len, rr)
+    batch = S.ShadeBatch([q.item(b, max_len, rr) for q, b in zip(queues, bufs)])
+    batch.launch()
+    for q, b in zip(queues, bufs):
+        q.occluded(b)
+    batch.shadow()
     N.check(N.lib.imba_stream_sync(None), "sync")
+    lit = 0
     for q, b, w in zip(queues, bufs, want):
         g = q.host(b)
         if q.n == 0:
             assert g[2] == 0
             continue
-        assert g[2] == w[2]
-        for x, y in zip(g, w):
-            if isinstance(x, np.ndarray):
-                assert np.array_equal(x, y)
+        _same(g, w)
+        lit += int((g[-1] > 0).sum())
+    assert lit > 100  # the shadow rays and emitter hits did add light
 
 
 def test_graph_replays(queues):
     st = N.Stream()
     bufs = [q.fresh() for q in queues]
-    batch = S.ShadeBatch([q.item(b) for q, b in zip(queues, bufs)])
-    batch.launch(8, 3, st)
+    batch = S.ShadeBatch([q.item(b, 8, 3) for q, b in zip(queues, bufs)])
+
+    def step():
+        batch.launch(st)
+        for q, b in zip(queues, bufs):
+            q.occluded(b, st)
+        batch.shadow(st)
+
+    step()
     st.sync()
     first = [q.host(b) for q, b in zip(queues, bufs)]
     g = S.Graph.begin(st)
     try:
-        batch.launch(8, 3, st)
+        step()
     finally:
         graph = S.Graph.end(st)
     for _ in range(3):
-        # restore the inputs (rays and states were bounced in place), then replay
+        # restore the inputs (rays and states were shaded in place), then replay
         for q, b in zip(queues, bufs):
             if q.n:
                 N.check(N.lib.imba_memcpy_d2d(b[0].ptr, q.rays0.ptr, q.n * 32, None), "copy")
@@ -145,8 +174,5 @@ def test_graph_replays(queues):
         st.sync()
         again = [q.host(b) for q, b in zip(queues, bufs)]
         for x, y in zip(again, first):
-            assert x[2] == y[2]
-            for u, v in zip(x, y):
-                if isinstance(u, np.ndarray):
-                    assert np.array_equal(u, v)
+            _same(x, y)
     del g

@@ -109,28 +109,29 @@ def test_visit_counts_and_batch(gpu):
     rb = N.DeviceBuffer.from_array(rays)
     hb = N.DeviceBuffer(len(rays) * 16)
     c2, c4 = N.DeviceBuffer(len(rays) * 12), N.DeviceBuffer(len(rays) * 12)
+    want, want_counts, _ = oracle.trace(b.arrays(), rays, counts=True)
     sc.trace_stats(rb, hb, c2, len(rays), cfg=N.TraceConfig(0, 0, 0, 0, 2))
     h2 = hb.download(N.HIT_DTYPE, len(rays))
+    assert _same(h2, want)
     sc.trace_stats(rb, hb, c4, len(rays), cfg=WIDE)
     h4 = hb.download(N.HIT_DTYPE, len(rays))
-    assert _same(h2, h4)
+    assert _same(h4, want)
     n2 = c2.download(np.uint32, 3 * len(rays)).reshape(-1, 3).sum(0)
     n4 = c4.download(np.uint32, 3 * len(rays)).reshape(-1, 3).sum(0)
     assert n4[0] < n2[0]          # fewer (wider) nodes visited
     assert n4[2] == n2[2]         # one instance entered per ray
     # the BVH2 counts are the oracle's
-    assert np.array_equal(c2.download(np.uint32, 3 * len(rays)).reshape(-1, 3),
-                          oracle.trace(b.arrays(), rays, counts=True)[1])
-    # the batched kernel on the 4-wide trees
+    assert np.array_equal(c2.download(np.uint32, 3 * len(rays)).reshape(-1, 3), want_counts)
+    # the batched kernel on the 4-wide trees: the oracle's closest hits
     out = N.DeviceBuffer(len(rays) * 16)
     batch = S.TraceBatch([(sc, rb, out, len(rays), None, False)])
     for tree in TREES:
         batch.launch(tree)
-        assert _same(out.download(N.HIT_DTYPE, len(rays)), h2)
+        assert _same(out.download(N.HIT_DTYPE, len(rays)), want)
     # the quantized tree: same hits, about as many node visits as the exact 4-wide tree
     cq = N.DeviceBuffer(len(rays) * 12)
     sc.trace_stats(rb, hb, cq, len(rays), cfg=TREES[1])
-    assert _same(hb.download(N.HIT_DTYPE, len(rays)), h2)
+    assert _same(hb.download(N.HIT_DTYPE, len(rays)), want)
     nq = cq.download(np.uint32, 3 * len(rays)).reshape(-1, 3).sum(0)
     assert 0.9 * n4[0] < nq[0] < 1.5 * n4[0]
 
