@@ -354,6 +354,125 @@ __global__ __launch_bounds__(kBlock) void k_query_wave(const GridParams* gp, con
     if (lane == 0) out_count[q] = count;
 }
 
+// k > 64: the same scan and insertion rule with the k-capped list in LDS, one query per wave
+// and one wave per block (dynamic LDS: k distances, then k photon indices).  Inserting at
+// position pos moves the entries [pos, count - 1) one slot up, 64 at a time from the top: the
+// wave reads 64 entries, then writes them one slot higher, so every entry is read before it is
+// overwritten.  The insertion position is the number of entries <= the candidate, counted over
+// the list by all lanes.
+constexpr int kQueryMaxK = 4096;
+
+__global__ __launch_bounds__(64) void k_query_lds(const GridParams* gp, const int* ends, const float4* pos4,
+                                                  const unsigned char* qs, int qstride, int qoff, int nq, int k,
+                                                  int* out_idx, float* out_dist, int* out_count) {
+    extern __shared__ float list_lds[];
+    float* ld = list_lds;
+    int* li = reinterpret_cast<int*>(list_lds + k);
+    const int q = blockIdx.x;
+    if (q >= nq) return;
+    const int lane = (int)threadIdx.x;
+    const float* qp = photon_pos(qs, qstride, qoff, q);
+    const float qx = qp[0], qy = qp[1], qz = qp[2];
+    const GridParams g = *gp;
+    int count = 0;
+    const bool inside = qx >= g.bmin[0] && qy >= g.bmin[1] && qz >= g.bmin[2] &&
+                        qx <= g.bmax[0] && qy <= g.bmax[1] && qz <= g.bmax[2];
+    if (inside && g.count > 0) {
+        const float cx = g.inv_cell_size * (qx - g.bmin[0]);
+        const float cy = g.inv_cell_size * (qy - g.bmin[1]);
+        const float cz = g.inv_cell_size * (qz - g.bmin[2]);
+        const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
+        const int px = (int)fx, py = (int)fy, pz = (int)fz;
+        const int pxo = px + ((cx - fx) < 0.5f ? -1 : 1);
+        const int pyo = py + ((cy - fy) < 0.5f ? -1 : 1);
+        const int pzo = pz + ((cz - fz) < 0.5f ? -1 : 1);
+        int begin = 0, size = 0;
+        if (lane < 8) {
+            const int x = (lane & 4) ? pxo : px;
+            const int y = (lane & 2) ? pyo : py;
+            const int z = (lane & 1) ? pzo : pz;
+            const int b = bucket_of((uint32_t)x, (uint32_t)y, (uint32_t)z, g.table_size);
+            begin = b == 0 ? 0 : ends[b - 1];
+            size = ends[b] - begin;
+        }
+        int pre = size;
+        for (int o = 1; o < 8; o <<= 1) {
+            const int y = __shfl_up(pre, o, 64);
+            if (lane >= o) pre += y;
+        }
+        pre -= size;
+        int beg_j[8], pre_j[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            beg_j[j] = __builtin_amdgcn_readlane(begin, j);
+            pre_j[j] = __builtin_amdgcn_readlane(pre, j);
+        }
+        const int total = __builtin_amdgcn_readlane(pre + size, 7);
+        float kth = 0.0f;
+        for (int base = 0; base < total; base += 64) {
+            const int c = base + lane;
+            const bool valid = c < total;
+            float d2 = 0.0f;
+            int pid = -1;
+            if (valid) {
+                int slot = 0;
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    if (c >= pre_j[j]) slot = beg_j[j] + (c - pre_j[j]);
+                const float4 p = pos4[slot];
+                const float dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
+                d2 = dx * dx + dy * dy + dz * dz;
+                pid = __float_as_int(p.w);
+            }
+            const float lim = count == k ? kth : g.radius_sqr;
+            unsigned long long m = __ballot(valid && d2 <= lim && d2 <= g.radius_sqr);
+            while (m) {
+                const int src = __builtin_ctzll(m);
+                m &= m - 1;
+                const float dd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d2), src));
+                const int id = __builtin_amdgcn_readlane(pid, src);
+                if (count == k) {
+                    if (kth < dd) continue;
+                } else {
+                    count++;
+                }
+                const int keep = count - 1;  // entries that stay (the last one drops when full)
+                int le = 0;
+                for (int j = lane; j < keep; j += 64) le += ld[j] <= dd ? 1 : 0;
+                for (int o = 32; o > 0; o >>= 1) le += __shfl_xor(le, o, 64);
+                const int pos = le;
+                for (int top = keep - 1; top >= pos; top -= 64) {
+                    const int j = top - lane;
+                    const bool act = j >= pos;
+                    float v = 0.0f;
+                    int w = 0;
+                    if (act) {
+                        v = ld[j];
+                        w = li[j];
+                    }
+                    __syncthreads();
+                    if (act) {
+                        ld[j + 1] = v;
+                        li[j + 1] = w;
+                    }
+                    __syncthreads();
+                }
+                if (lane == 0) {
+                    ld[pos] = dd;
+                    li[pos] = id;
+                }
+                __syncthreads();
+                if (count == k) kth = ld[k - 1];
+            }
+        }
+    }
+    for (int j = lane; j < count; j += 64) {
+        out_idx[(size_t)q * k + j] = li[j];
+        if (out_dist) out_dist[(size_t)q * k + j] = ld[j];
+    }
+    if (lane == 0) out_count[q] = count;
+}
+
 template <typename T>
 int ensure(T*& p, int& cap_now, int want) {
     if (p && cap_now >= want) return IMBA_OK;
@@ -471,7 +590,7 @@ int imba_hashgrid_build(ImbaHashGrid* g, const void* photons, int stride, int po
 int imba_hashgrid_query(const ImbaHashGrid* g, const void* queries, int query_stride, int query_offset, int nq, int k,
                         int* out_idx, float* out_dist, int* out_count, void* stream) {
     IMBA_CHECK_ARG(g, "grid is NULL");
-    IMBA_CHECK_ARG(k >= 1 && k <= 64, "k must be in [1, 64]");
+    if (k < 1 || k > kQueryMaxK) return fail(IMBA_ERR_INVALID_ARG, "k must be in [1, %d]", kQueryMaxK);
     IMBA_CHECK_ARG(nq >= 0, "nq < 0");
     IMBA_CHECK_ARG(query_stride >= 12 && query_stride % 4 == 0 && query_offset >= 0 && query_offset % 4 == 0 &&
                        query_offset + 12 <= query_stride,
@@ -482,8 +601,12 @@ int imba_hashgrid_query(const ImbaHashGrid* g, const void* queries, int query_st
     bool sync;
     hipStream_t s = resolve_stream(stream, &sync);
     const unsigned char* qs = (const unsigned char*)queries;
-    hipLaunchKernelGGL(k_query_wave, dim3(ceil_div(nq, kBlock / 64)), dim3(kBlock), 0, s, g->d_params, g->cell_ends,
-                       g->pos4, qs, query_stride, query_offset, nq, k, out_idx, out_dist, out_count);
+    if (k <= 64)
+        hipLaunchKernelGGL(k_query_wave, dim3(ceil_div(nq, kBlock / 64)), dim3(kBlock), 0, s, g->d_params,
+                           g->cell_ends, g->pos4, qs, query_stride, query_offset, nq, k, out_idx, out_dist, out_count);
+    else
+        hipLaunchKernelGGL(k_query_lds, dim3(nq), dim3(64), (size_t)k * 8, s, g->d_params, g->cell_ends, g->pos4, qs,
+                           query_stride, query_offset, nq, k, out_idx, out_dist, out_count);
     IMBA_HIP_TRY(hipGetLastError());
     if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
     return IMBA_OK;

@@ -185,6 +185,7 @@ class SphereGridScene:
     def __init__(self, side: int = 32, level: int = 5, radius: float = 0.4, spacing: float = 1.0,
                  builder: int = N.BUILDER_SBVH, threads: int = 8, seed: int = 0x5EED, specular_fraction: float = 0.0):
         self.side = side
+        self.seed = seed
         ext = side * spacing
         self.bounds = ((-1.0, 0.0, -12.0), (ext + 1.0, 8.0, ext + 1.0))
         self.build = S.SceneBuild()
@@ -199,12 +200,7 @@ class SphereGridScene:
         self.albedo[1:1 + side * side] = (0.2 + 0.6 * draws).reshape(-1, 3)
         # material kinds (photon mapping): 0 Lambertian, 1 mirror, 2 glass; a fraction of the
         # spheres, half mirrors and half glass, chosen by a second draw per sphere
-        self.kinds = np.zeros(n_mat, np.int32)
-        if specular_fraction > 0.0:
-            u = mwc_uniform(seed ^ 0x5BEC, side * side)
-            sph = self.kinds[1:1 + side * side]
-            sph[u < 0.5 * specular_fraction] = 1
-            sph[(u >= 0.5 * specular_fraction) & (u < specular_fraction)] = 2
+        self.kinds = self._kinds(specular_fraction)
         for k in range(side * side):
             gx, gz = k % side, k // side
             centre = ((gx + 0.5) * spacing, radius, (gz + 0.5) * spacing)
@@ -228,6 +224,26 @@ class SphereGridScene:
         self.tri_count = len(self.tri_materials)
         self.camera = ((ext * 0.5, 6.0, -10.0), (0.0, -6.0, 26.0), (0.0, 1.0, 0.0), 60.0)
         self.offset = 1e-4 * float(np.linalg.norm(np.subtract(self.bounds[1], self.bounds[0])))
+
+
+    def _kinds(self, specular_fraction: float) -> np.ndarray:
+        side = self.side
+        kinds = np.zeros(side * side + 2, np.int32)
+        if specular_fraction > 0.0:
+            u = mwc_uniform(self.seed ^ 0x5BEC, side * side)
+            sph = kinds[1:1 + side * side]
+            sph[u < 0.5 * specular_fraction] = 1
+            sph[(u >= 0.5 * specular_fraction) & (u < specular_fraction)] = 2
+        return kinds
+
+    def with_specular(self, specular_fraction: float) -> "SphereGridScene":
+        """The same geometry (same BVHs, not rebuilt) with the material kinds a scene built with
+        this specular_fraction has."""
+        import copy
+
+        other = copy.copy(self)
+        other.kinds = self._kinds(specular_fraction)
+        return other
 
 
 class PathTracer:
@@ -290,6 +306,7 @@ class PathTracer:
         counts, visit, queues, ev = [], [], [], []
         self.kept_states = []  # with keep_queues: the states of the kept queues
         self.kept_shadow = []  # with keep_queues: (shadow rays, shadow states, shadow hits) of the kept bounces
+        self.kept_hits = []    # with keep_queues: the closest hits of the kept queues
         self.shadow_counts = []
         self.shadow_visit = []
         self.shadow_events = []
@@ -317,6 +334,8 @@ class PathTracer:
                 if timed:
                     e1.record(self.stream)
                     ev.append((e0, e1, n))
+            if keep:
+                self.kept_hits.append(self.hits.download(N.HIT_DTYPE, n, st))
             N.check(N.lib.imba_pt_shade(C.byref(self.sh), self.rays[cur].ptr, self.hits.ptr, self.states[cur].ptr, n,
                                         None, self.image.ptr, self.rays[cur].ptr, self.shadow_rays.ptr,
                                         self.shadow_states.ptr, st), "pt_shade")

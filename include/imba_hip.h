@@ -426,7 +426,8 @@ int imba_hashgrid_build(ImbaHashGrid* g, const void* photons, int stride, int po
 /* For each query position (records of query_stride bytes, position at query_offset): up to
  * k photon indices (out_idx[q*k + j]) sorted by squared distance (out_dist, may be NULL),
  * with the reference's rules for duplicates and ties; out_count[q] = number written.
- * 1 <= k <= 64. */
+ * 1 <= k <= 4096 (k <= 64: the list lives in the lanes of one wave per query; above, in LDS,
+ * k * 8 bytes per query; the reference sizes its list by k, rangesearch.h:98). */
 int imba_hashgrid_query(const ImbaHashGrid* g, const void* queries, int query_stride, int query_offset,
                         int nq, int k, int* out_idx, float* out_dist, int* out_count, void* stream);
 int imba_hashgrid_info(const ImbaHashGrid* g, ImbaHashGridInfo* info);

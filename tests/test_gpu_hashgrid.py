@@ -3,7 +3,8 @@
 Build: bucket table size, grown bounds, bucket ends and the slot order (photon indices by
 bucket, then by index) equal the oracle's exactly.  Query: the photon index lists (with the
 duplicates that coinciding buckets produce, in result order) and the squared distances
-equal the oracle's bit for bit, for k up to 64.
+equal the oracle's bit for bit, for k up to 64 (list in lanes) and above (list in LDS, up to
+4096; the reference sizes its list by k, rangesearch.h:98).
 """
 import numpy as np
 import pytest
@@ -84,6 +85,32 @@ def test_build_and_query(gpu, n, radius, k, clustered):
     nonempty = _check_queries(g, og, queries, k)
     if n > 100:
         assert nonempty > 500
+
+
+@pytest.mark.parametrize("k", [65, 100, 1000, 4096])
+def test_query_large_k(gpu, k):
+    """k above 64: dense clustered photons, so that lists fill up and the k-th distance prunes."""
+    n, radius = 200_000, 0.03
+    photons = _photons(n, seed=77, clustered=True)
+    g, _ = _grid(photons, radius)
+    og = oracle.Grid()
+    og.build(photons[:, :3], radius)
+    rng = np.random.default_rng(2)
+    queries = np.concatenate([photons[rng.integers(0, n, 300), :3] + 0.3 * radius * rng.normal(size=(300, 3)),
+                              rng.uniform(-0.2, 1.2, (30, 3))]).astype(np.float32)
+    assert _check_queries(g, og, queries, k) > 200
+    _, _, cnt = _query(g, queries, k)
+    assert (cnt == k).sum() > 50  # full lists (the k-th distance prunes)
+
+
+def test_query_k_limit(gpu):
+    photons = _photons(100, seed=1)
+    g, _ = _grid(photons, 0.1)
+    qb = N.DeviceBuffer.from_array(photons[:4, :3].copy())
+    out = N.DeviceBuffer(4 * 4097 * 4)
+    cnt = N.DeviceBuffer(16)
+    r = N.lib.imba_hashgrid_query(g.h, qb.ptr, 12, 0, 4, 4097, out.ptr, None, cnt.ptr, None)
+    assert r == 1  # IMBA_ERR_INVALID_ARG: k above 4096
 
 
 def test_table_grows_only_and_rebuild(gpu):
