@@ -756,12 +756,12 @@ def run_c5(args):
     for side, level in grids:
         sg = W.SphereGridScene(side=side, level=level, threads=threads)
         scenes.append(V.ServedScene(sg))
-        centres.append((side * 0.5, 0.4, side * 0.5, max(3.0, 1.6 * side)))
+        centres.append(W.served_view(side))
         tris.append(sg.tri_count)
     log("8 scenes (%s triangles) built in %.1f s" % (tris, time.time() - t0))
     integrators = tuple(args.c5_integrators.split(","))
     reqs = V.poisson_requests(args.c5_requests, args.c5_rate, 16, len(scenes), centres, seed=11,
-                              integrators=integrators)
+                              integrators=integrators, ceiling=W.SERVED_CEILING)
     cfg = fixed_config(args)
     arena = args.c5_arena_gb << 30
 
@@ -771,7 +771,7 @@ def run_c5(args):
             server = V.Server(scenes, arena, max_active=args.c5_max_active, batched=batched, cfg=cfg,
                               max_rays_in_flight=args.c5_max_rays)
             # warm-up on a few requests (kernels loaded, batch tables sized)
-            server.run(V.poisson_requests(4, 1000.0, 1, len(scenes), centres, seed=99))
+            server.run(V.poisson_requests(4, 1000.0, 1, len(scenes), centres, seed=99, ceiling=W.SERVED_CEILING))
             server = V.Server(scenes, arena, max_active=args.c5_max_active, batched=batched, cfg=cfg,
                               max_rays_in_flight=args.c5_max_rays)
             t = time.perf_counter()

@@ -20,39 +20,8 @@
 #include <cfloat>
 
 #include "common.h"
+#include "grid_query.h"
 #include "radix.h"
-
-namespace imba {
-
-struct GridParams {
-    float bmin[3], bmax[3];
-    float radius, radius_sqr, cell_size, inv_cell_size;
-    int table_size;
-    int count;
-};
-
-}  // namespace imba
-
-struct ImbaHashGrid {
-    int device = 0;
-    int table_size = 0;  // grows only (until imba_hashgrid_reset)
-    int table_cap = 0;   // buckets allocated
-    int count = 0;
-    int cap = 0;         // photon capacity of the per-photon arrays
-    int* cell_ends = nullptr;
-    int* order = nullptr;
-    int* keys_a = nullptr;     // sort buffers (photon_count entries each)
-    int* vals_a = nullptr;
-    int* keys_b = nullptr;
-    int* vals_b = nullptr;
-    float4* pos4 = nullptr;
-    imba::GridParams* d_params = nullptr;
-    float* partial = nullptr;  // 6 per bounds block
-    int* scan_tmp = nullptr;   // tile maxima of the ends scan
-    int scan_cap = 0;
-    int* digits = nullptr;     // radix digit counts
-    int digits_cap = 0;
-};
 
 namespace imba {
 namespace {
@@ -139,9 +108,6 @@ __global__ __launch_bounds__(kBlock) void k_zero(int* p, int n) {
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) p[i] = 0;
 }
 
-__device__ __forceinline__ int bucket_of(uint32_t x, uint32_t y, uint32_t z, int table) {
-    return (int)(((x * 73856093u) ^ (y * 19349663u) ^ (z * 83492791u)) % (uint32_t)table);
-}
 
 __global__ __launch_bounds__(kBlock) void k_grid_keys(const unsigned char* ph, int stride, int off, int n,
                                                       const GridParams* gp, int* keys, int* vals) {
@@ -242,16 +208,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_pos(const unsigned char* ph, 
     pos4[s] = make_float4(p[0], p[1], p[2], __int_as_float(i));
 }
 
-constexpr int kDppWaveShr1 = 0x138;  // DPP control: lane i reads lane i - 1 of the wave
-
-// One query per wave.  The eight probed bucket ranges are concatenated in visiting order and
-// scanned 64 candidates at a time, lane l taking candidate base + l: loads within a bucket are
-// contiguous across the wave.  Candidates within the limit at the start of the chunk (the
-// radius, or the k-th distance once the list is full) are then inserted one at a time in
-// visiting order with the reference's rule; the list itself lives in lanes 0..k-1 (distance
-// and photon index per lane), so any k up to 64 fits.  The limit only shrinks, so the
-// candidates it rejects would be rejected by the exact test as well: the results are those of
-// the one-at-a-time scan.
+// One query per wave (the scan and insertion of query_wave_list); the list goes to out_idx /
+// out_dist.
 __global__ __launch_bounds__(kBlock) void k_query_wave(const GridParams* gp, const int* ends, const float4* pos4,
                                                        const unsigned char* qs, int qstride, int qoff, int nq, int k,
                                                        int* out_idx, float* out_dist, int* out_count) {
@@ -259,94 +217,9 @@ __global__ __launch_bounds__(kBlock) void k_query_wave(const GridParams* gp, con
     if (q >= nq) return;  // whole waves leave together
     const int lane = (int)(threadIdx.x & 63);
     const float* qp = photon_pos(qs, qstride, qoff, q);
-    const float qx = qp[0], qy = qp[1], qz = qp[2];
-    const GridParams g = *gp;
-    float my_dist = 0.0f;
-    int my_idx = -1;
-    int count = 0;
-    const bool inside = qx >= g.bmin[0] && qy >= g.bmin[1] && qz >= g.bmin[2] &&
-                        qx <= g.bmax[0] && qy <= g.bmax[1] && qz <= g.bmax[2];
-    if (inside && g.count > 0) {
-        const float cx = g.inv_cell_size * (qx - g.bmin[0]);
-        const float cy = g.inv_cell_size * (qy - g.bmin[1]);
-        const float cz = g.inv_cell_size * (qz - g.bmin[2]);
-        const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
-        const int px = (int)fx, py = (int)fy, pz = (int)fz;
-        const int pxo = px + ((cx - fx) < 0.5f ? -1 : 1);
-        const int pyo = py + ((cy - fy) < 0.5f ? -1 : 1);
-        const int pzo = pz + ((cz - fz) < 0.5f ? -1 : 1);
-        // lane j < 8: probe j's bucket range
-        int begin = 0, size = 0;
-        if (lane < 8) {
-            const int x = (lane & 4) ? pxo : px;
-            const int y = (lane & 2) ? pyo : py;
-            const int z = (lane & 1) ? pzo : pz;
-            const int b = bucket_of((uint32_t)x, (uint32_t)y, (uint32_t)z, g.table_size);
-            begin = b == 0 ? 0 : ends[b - 1];
-            size = ends[b] - begin;
-        }
-        // first candidate number of each probe (exclusive prefix over lanes 0..7)
-        int pre = size;
-        for (int o = 1; o < 8; o <<= 1) {
-            const int y = __shfl_up(pre, o, 64);
-            if (lane >= o) pre += y;
-        }
-        pre -= size;
-        int beg_j[8], pre_j[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            beg_j[j] = __builtin_amdgcn_readlane(begin, j);
-            pre_j[j] = __builtin_amdgcn_readlane(pre, j);
-        }
-        const int total = __builtin_amdgcn_readlane(pre + size, 7);
-        float kth = 0.0f;  // dist[k - 1] once the list is full (wave-uniform)
-        // the photons of candidate c (x = +inf past the end, which no test accepts); the next
-        // chunk's load is issued before the current chunk's insertions
-        auto fetch = [&](int c) {
-            if (c >= total) return make_float4(__int_as_float(0x7f800000), 0.0f, 0.0f, 0.0f);
-            int slot = 0;
-#pragma unroll
-            for (int j = 0; j < 8; j++)
-                if (c >= pre_j[j]) slot = beg_j[j] + (c - pre_j[j]);
-            return pos4[slot];
-        };
-        float4 p_next = fetch(lane);
-        for (int base = 0; base < total; base += 64) {
-            const bool valid = base + lane < total;
-            const float4 p = p_next;
-            p_next = fetch(base + 64 + lane);
-            float d2 = 0.0f;
-            int pid = -1;
-            if (valid) {
-                const float dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
-                d2 = dx * dx + dy * dy + dz * dz;
-                pid = __float_as_int(p.w);
-            }
-            const float lim = count == k ? kth : g.radius_sqr;
-            unsigned long long m = __ballot(valid && d2 <= lim && d2 <= g.radius_sqr);
-            while (m) {
-                const int src = __builtin_ctzll(m);
-                m &= m - 1;
-                const float dd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d2), src));
-                const int id = __builtin_amdgcn_readlane(pid, src);
-                if (count == k) {
-                    if (kth < dd) continue;
-                } else {
-                    count++;
-                }
-                // keep the first count-1 entries, insert dd after every entry <= dd
-                const int keep = count - 1;
-                const int pos = __popcll(__ballot(lane < keep && my_dist <= dd));
-                // the entries one lane down (wave shift right by one lane, DPP)
-                const float up_d = __int_as_float(__builtin_amdgcn_update_dpp(
-                    __float_as_int(my_dist), __float_as_int(my_dist), kDppWaveShr1, 0xF, 0xF, false));
-                const int up_i = __builtin_amdgcn_update_dpp(my_idx, my_idx, kDppWaveShr1, 0xF, 0xF, false);
-                if (lane > pos && lane <= keep) { my_dist = up_d; my_idx = up_i; }
-                if (lane == pos) { my_dist = dd; my_idx = id; }
-                if (count == k) kth = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_dist), k - 1));
-            }
-        }
-    }
+    float my_dist;
+    int my_idx;
+    const int count = query_wave_list<false>(*gp, ends, pos4, qp[0], qp[1], qp[2], k, my_dist, my_idx);
     if (lane < count) {
         out_idx[(size_t)q * k + lane] = my_idx;
         if (out_dist) out_dist[(size_t)q * k + lane] = my_dist;

@@ -538,10 +538,10 @@ class Server:
 
 def poisson_requests(n: int, rate: float, n_sessions: int, n_scenes: int, scene_centres: list, seed: int = 7,
                      resolutions=((256, 256), (640, 480), (1280, 720), (1920, 1080)), spp=(1, 4),
-                     integrators=("pt",)) -> list:
+                     integrators=("pt",), ceiling: float = float("inf")) -> list:
     """n requests with Poisson arrivals at `rate` per second from n_sessions sessions, each for a
     uniformly drawn scene, resolution, spp and integrator and a random camera looking at the
-    scene centre."""
+    scene centre (cx, cy, cz) from `radius` away, no higher than `ceiling`."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -551,7 +551,7 @@ def poisson_requests(n: int, rate: float, n_sessions: int, n_scenes: int, scene_
         w, h = resolutions[int(rng.integers(0, len(resolutions)))]
         cx, cy, cz, radius = scene_centres[sc]
         ang = rng.uniform(0, 2 * np.pi)
-        pos = (cx + radius * np.sin(ang), cy + rng.uniform(0.3, 0.9) * radius, cz - radius * np.cos(ang))
+        pos = (cx + radius * np.sin(ang), min(cy + rng.uniform(0.3, 0.9) * radius, ceiling), cz - radius * np.cos(ang))
         d = (cx - pos[0], cy * 0.3 - pos[1], cz - pos[2])
         req = Request(rid, int(rng.integers(0, n_sessions)), sc, w, h, int(rng.integers(spp[0], spp[1] + 1)),
                       pos, d, int(rng.integers(1, 2 ** 31 - 1)), t)

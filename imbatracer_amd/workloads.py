@@ -181,6 +181,18 @@ class PtShadingData:
                            self.lights.ptr.value, self.light_count, max_path_len, rr_start)
 
 
+# Served requests look at a sphere grid from inside its box (walls at x = -1 and side + 1, the
+# far wall at z = side + 1, the ceiling at y = 8 with the light just below it): from outside,
+# a camera would see only the unlit exterior of the box.
+SERVED_CEILING = 7.5
+
+
+def served_view(side: int, spacing: float = 1.0):
+    """(cx, cy, cz, radius) of the ring the served cameras of a side x side grid sit on."""
+    ext = side * spacing
+    return (ext * 0.5, 0.4, ext * 0.5, 0.5 * ext + 0.5)
+
+
 class SphereGridScene:
     def __init__(self, side: int = 32, level: int = 5, radius: float = 0.4, spacing: float = 1.0,
                  builder: int = N.BUILDER_SBVH, threads: int = 8, seed: int = 0x5EED, specular_fraction: float = 0.0):

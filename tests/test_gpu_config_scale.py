@@ -174,15 +174,15 @@ def test_c5_requests_from_the_serving_mix(grid21m):
     for side in (1, 2, 4, 6, 8, 12, 20, 32):
         sg = big if side == 32 else W.SphereGridScene(side=side, level=5, threads=THREADS)
         scenes.append(V.ServedScene(sg))
-        centres.append((side * 0.5, 0.4, side * 0.5, max(3.0, 1.6 * side)))
+        centres.append(W.served_view(side))
     reqs = V.poisson_requests(10, 200.0, 16, len(scenes), centres, seed=11, integrators=("pt", "ppm"),
-                              resolutions=((256, 256), (640, 480)))
+                              resolutions=((256, 256), (640, 480)), ceiling=W.SERVED_CEILING)
     # every scene size class is represented: force the first requests onto the largest scenes
     for r, sc in zip(reqs, (7, 6, 5)):
         r.scene = sc
         cx, cy, cz, radius = centres[sc]
-        r.cam_pos = (cx, cy + 0.6 * radius, cz - radius)
-        r.cam_dir = (0.0, -0.6 * radius, radius)
+        r.cam_pos = (cx, min(cy + 0.6 * radius, W.SERVED_CEILING), cz - radius)
+        r.cam_dir = (0.0, cy - r.cam_pos[1], radius)
 
     def run(batched):
         srv = V.Server(scenes, 24 << 30, max_active=16, batched=batched)

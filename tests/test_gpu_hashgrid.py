@@ -89,8 +89,9 @@ def test_build_and_query(gpu, n, radius, k, clustered):
 
 @pytest.mark.parametrize("k", [65, 100, 1000, 4096])
 def test_query_large_k(gpu, k):
-    """k above 64: dense clustered photons, so that lists fill up and the k-th distance prunes."""
-    n, radius = 200_000, 0.03
+    """k above 64: dense clustered photons, so that lists fill up and the k-th distance prunes
+    (a larger radius for the largest k)."""
+    n, radius = 200_000, 0.03 if k <= 1000 else 0.06
     photons = _photons(n, seed=77, clustered=True)
     g, _ = _grid(photons, radius)
     og = oracle.Grid()
