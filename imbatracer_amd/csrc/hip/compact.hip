@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "common.h"
 
@@ -164,10 +165,40 @@ __global__ __launch_bounds__(kBlock) void k_scatter(CompactArgs a, int nblocks) 
 
 thread_local Scratch t_offsets[64];
 
+// Block offsets of the allocation-free compaction (imba_compact_hits_noalloc): one array per
+// device for the life of the process, kNoAllocMaxBlocks + 1 ints (256 KiB + 4 B), enough for
+// 134,217,728 entries.  Host code never passes this name to a kernel (on the host it is only a
+// stand-in for the symbol); it uses the device address hipGetSymbolAddress gave for the
+// current device.
+constexpr int kNoAllocMaxBlocks = 65536;
+static_assert((long long)kNoAllocMaxBlocks * kTile == IMBA_COMPACT_NOALLOC_MAX_COUNT, "header limit");
+static __device__ __align__(256) int scratch[kNoAllocMaxBlocks + 1];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch`, looked up once per device.
+int scratch_for_device(int dev, int** out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(IMBA_ERR_INVALID_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        IMBA_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(IMBA_ERR_HIP, "hipGetSymbolAddress returned no address for the compaction scratch");
+        g_scratch_ptr[dev] = (int*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return IMBA_OK;
+}
+
 template <int MODE>
-int run_compact(CompactArgs a, int* new_count, void* stream) {
+int run_compact(CompactArgs a, int* new_count, void* stream, bool static_scratch = false) {
     if (a.n < 0) return fail(IMBA_ERR_INVALID_ARG, "count < 0");
     if (a.stride_dw <= 0) return fail(IMBA_ERR_INVALID_ARG, "state stride must be a positive multiple of 4 bytes");
+    if (static_scratch && a.n > kNoAllocMaxBlocks * kTile)
+        return fail(IMBA_ERR_CAPACITY, "count %d exceeds the %d entries of the allocation-free compaction", a.n,
+                    kNoAllocMaxBlocks * kTile);
     if (a.n == 0) {
         if (new_count) *new_count = 0;
         return IMBA_OK;
@@ -177,12 +208,24 @@ int run_compact(CompactArgs a, int* new_count, void* stream) {
     int dev = 0;
     IMBA_HIP_TRY(hipGetDevice(&dev));
     const int nb = ceil_div(a.n, kTile);
-    Scratch& sc = t_offsets[dev & 63];
-    int r = grow_buffer(sc, sizeof(int) * ((size_t)nb + 1));
-    if (r) return r;
-    a.block_offsets = (int*)sc.ptr;
+    if (static_scratch) {
+        int r = scratch_for_device(dev, &a.block_offsets);
+        if (r) return r;
+    } else {
+        Scratch& sc = t_offsets[dev & 63];
+        int r = grow_buffer(sc, sizeof(int) * ((size_t)nb + 1));
+        if (r) return r;
+        a.block_offsets = (int*)sc.ptr;
+    }
     bool sync;
-    hipStream_t s = resolve_stream(stream, &sync);
+    hipStream_t s;
+    if (static_scratch) {
+        // no per-thread stream is created here: NULL is the device's null stream
+        s = (hipStream_t)stream;
+        sync = stream == nullptr;
+    } else {
+        s = resolve_stream(stream, &sync);
+    }
     hipLaunchKernelGGL(k_count<MODE>, dim3(nb), dim3(kBlock), 0, s, a);
     hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, a.block_offsets, nb, (int*)nullptr);
     hipLaunchKernelGGL(k_scatter<MODE>, dim3(nb), dim3(kBlock), 0, s, a, nb);
@@ -253,6 +296,23 @@ int imba_compact_hits(const ImbaRay* rays_in, const ImbaHit* hits_in, const void
     a.states_out = (int*)states_out;
     return run_compact<kModeHits>(a, new_count, stream);
 }
+
+int imba_compact_hits_noalloc(const ImbaRay* rays_in, const ImbaHit* hits_in, const void* states_in, int state_stride,
+                              int count, ImbaRay* rays_out, ImbaHit* hits_out, void* states_out, int* new_count,
+                              void* stream) {
+    IMBA_CHECK_ARG(state_stride > 0 && state_stride % 4 == 0, "state_stride must be a positive multiple of 4");
+    CompactArgs a{};
+    a.rays_in = (const float4*)rays_in;
+    a.hits_in = (const int4*)hits_in;
+    a.states_in = (const int*)states_in;
+    a.stride_dw = state_stride / 4;
+    a.n = count;
+    a.rays_out = (float4*)rays_out;
+    a.hits_out = (int4*)hits_out;
+    a.states_out = (int*)states_out;
+    return run_compact<kModeHits>(a, new_count, stream, true);
+}
+
 
 int imba_compact_rays(const ImbaRay* rays_in, const void* states_in, int state_stride, int count, ImbaRay* rays_out,
                       void* states_out, int* new_count, void* stream) {

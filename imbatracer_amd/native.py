@@ -198,6 +198,7 @@ _sig("imba_graph_launch", i32, vp, vp)
 _sig("imba_graph_release", i32, vp)
 _sig("imba_compact_hits", i32, vp, vp, vp, i32, i32, vp, vp, vp, ip, vp)
 _sig("imba_compact_rays", i32, vp, vp, i32, i32, vp, vp, ip, vp)
+_sig("imba_compact_hits_noalloc", i32, vp, vp, vp, i32, i32, vp, vp, vp, ip, vp)
 _sig("imba_compact_workspace_size", sz, i32)
 _sig("imba_compact_rays_dev", i32, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp)
 _sig("imba_sort_by_material", i32, vp, i32, vp, vp, i32, i32, vp, vp, vp)

@@ -383,6 +383,22 @@ int imba_compact_hits(const ImbaRay* rays_in, const ImbaHit* hits_in, const void
                       int* new_count, void* stream);
 int imba_compact_rays(const ImbaRay* rays_in, const void* states_in, int state_stride, int count,
                       ImbaRay* rays_out, void* states_out, int* new_count, void* stream);
+/* Allocation-free compact_hits.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, hosts that forbid
+ * hipMalloc).  Same arguments, results and errors as imba_compact_hits, except:
+ *  - the block offsets live in a library array of 256 KiB + 4 B per device, held until the
+ *    process ends, so count may not exceed IMBA_COMPACT_NOALLOC_MAX_COUNT (134,217,728;
+ *    larger counts return IMBA_ERR_CAPACITY);
+ *  - every call on one device shares that array: calls that overlap in time on one device
+ *    (other host threads, other streams) race and give wrong results;
+ *  - stream NULL is the device's null stream (no per-thread stream is created);
+ *  - new_count may be NULL: nothing is read back and, with a stream given, nothing waits.
+ * It calls no hipMalloc / hipFree. */
+#define IMBA_COMPACT_NOALLOC_MAX_COUNT 134217728
+int imba_compact_hits_noalloc(const ImbaRay* rays_in, const ImbaHit* hits_in, const void* states_in,
+                              int state_stride, int count,
+                              ImbaRay* rays_out, ImbaHit* hits_out, void* states_out,
+                              int* new_count, void* stream);
 /* Device-count variant of compact_rays for graph-captured loops: the queue holds *count_dev
  * (clamped to [0, capacity]) entries when the launch runs and the new size is written to the
  * device int *new_count_dev; nothing is read back to the host.  `workspace` is device memory
