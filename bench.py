@@ -12,8 +12,10 @@ timed region).
 
 --workload c1 runs the CPU-runnable configuration instead: 65,536 random triangles
 (binned SAH), 1,048,576 primary closest-hit rays plus 1,048,576 occlusion rays per step.
---workload c3: photon mapping (the PPM mode of VCM) at 1280x720 with 921,600 light paths: light
-and camera traversal, photon compaction, hash-grid build and k-nearest queries per frame.
+--workload c3: one VCM iteration at 1280x720 with 921,600 light paths: light and camera
+traversal with next-event, camera-connection and vertex-connection shadow rays, the light vertex
+cache, hash-grid build and a k-nearest merge at every Lambertian camera vertex per frame
+(--workload c3-ppm: the integrator's PPM mode, one merge query per pixel).
 --workload c4: 256 queues over 8 scenes, about 35M closest-hit rays per step in one batched
 launch, against the same queues launched one at a time.
 --workload c5: serving; Poisson request arrivals over 8 resident scenes, the request loop with
@@ -332,6 +334,7 @@ def run_pt(args):
     log("uploaded %.2f GB" % (pt.gpu_scene.footprint() / 1e9))
     # the queues of a frame (closest hits, hence the queues, do not depend on the configuration)
     _, _, queues = pt.frame(keep_queues=8)
+    shadow_queues = [sr[ss["pixel"] >= 0] for sr, ss, _ in pt.kept_shadow]
     bounce_cfg = shadow_bounce_cfg = None
     if cfg is None and not args.no_autotune:
         cfg, bounce_cfg = frame_tune(pt, queues)
@@ -377,27 +380,45 @@ def run_pt(args):
     per_bounce_ms /= args.steps
     per_bounce_shadow_ms /= args.steps
 
-    # the measured frames
+    # the measured frames: graph launches, `frames_in_flight` frame loops (each with its own
+    # queues, stream and graph over the one uploaded scene) taking turns, so that one frame's
+    # short last bounces overlap the next frame's first bounces on the device (progressive
+    # rendering renders frame after frame, frontend/render_window.cpp:50-91)
+    frame_ms, rays_total = [], 0
     if args.mode == "graph":
-        pt.build_graph(use_batch=args.trace_kernel == "batch")
-        for _ in range(args.warmup):
-            pt.frame_graph()
-        frame = pt.frame_graph
+        loops = [pt] + [W.PathTracer(scene, args.width, args.height, spp=1, max_path_len=8, rr_start=3, trace_cfg=cfg,
+                                     shadow_cfg=pt.shadow_cfg, gpu_scene=pt.gpu_scene)
+                        for _ in range(args.frames_in_flight - 1)]
+        for lp in loops:
+            lp.bounce_cfg, lp.shadow_bounce_cfg = pt.bounce_cfg, pt.shadow_bounce_cfg
+            lp.build_graph(use_batch=args.trace_kernel == "batch")
+            for _ in range(args.warmup):
+                lp.frame_graph()
+        started = {}
+        t_start = time.perf_counter()
+        for step in range(args.steps + len(loops)):
+            lp = loops[step % len(loops)]
+            if id(lp) in started:
+                c = lp.finish_graph()
+                frame_ms.append((time.perf_counter() - started.pop(id(lp))) * 1e3)
+                if c != counts:
+                    raise RuntimeError("frame ray counts changed between frames: %s vs %s" % (c, counts))
+                rays_total += sum(c)
+            if step < args.steps:
+                lp.launch_graph()
+                started[id(lp)] = time.perf_counter()
+        wall = time.perf_counter() - t_start
     else:
-        def frame():
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            f0 = time.perf_counter()
             c, _, _ = pt.frame()
             pt.stream.sync()
-            return c
-    frame_ms, rays_total = [], 0
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        f0 = time.perf_counter()
-        c = frame()
-        frame_ms.append((time.perf_counter() - f0) * 1e3)
-        if c != counts:
-            raise RuntimeError("frame ray counts changed between frames: %s vs %s" % (c, counts))
-        rays_total += sum(c)
-    wall = time.perf_counter() - t_start
+            frame_ms.append((time.perf_counter() - f0) * 1e3)
+            if c != counts:
+                raise RuntimeError("frame ray counts changed between frames: %s vs %s" % (c, counts))
+            rays_total += sum(c)
+        wall = time.perf_counter() - t_start
     if pt.gpu_scene.error_count():
         raise RuntimeError("traversal reported a malformed scene")
     achieved = bytes_total / (trace_ms * 1e-3) / 1e9
@@ -405,7 +426,24 @@ def run_pt(args):
     workload = "pt-%dx%d-1spp-depth8-rr3-nee-sphere-grid-%dtri-sbvh" % (args.width, args.height, scene.tri_count)
     traffic, traffic_src = measured_traffic(workload, kernel_label(cfg))
     shadow_frame = sum(shadow_counts) * args.steps
+    # stack use per bounce (the stats variant on the same queues): rays whose deepest stack entry
+    # lies beyond the 16 entries kept in LDS spill to the global stack buffer
+    depth = []
+    for b, rays in enumerate(queues):
+        h = pt.gpu_scene.stack_depths(rays, cfg=pt.cfg_of(b))
+        hs = pt.gpu_scene.stack_depths(shadow_queues[b], any_hit=True, cfg=pt.shadow_cfg_of(b)) \
+            if b < len(shadow_queues) else np.zeros(64, np.uint32)
+        depth.append((h, hs))
+
+    def depth_summary(h):
+        n = max(int(h.sum()), 1)
+        nz = np.nonzero(h)[0]
+        return {"max": int(nz.max()) if len(nz) else 0, "mean": float((np.arange(64) * h).sum() / n),
+                "frac_beyond_16": float(h[17:].sum() / n), "frac_beyond_8": float(h[9:].sum() / n)}
+
     table = [{"bounce": b, "rays": int(n), "closest_ms": float(per_bounce_ms[b]),
+              "closest_stack_depth": depth_summary(depth[b][0]) if b < len(depth) else None,
+              "occluded_stack_depth": depth_summary(depth[b][1]) if b < len(depth) else None,
               "closest_mrays_s": n / (per_bounce_ms[b] * 1e-3) / 1e6,
               "closest_cfg": cfg_dict(pt.cfg_of(b)),
               "shadow_rays": int(shadow_counts[b]), "occluded_ms": float(per_bounce_shadow_ms[b]),
@@ -424,8 +462,10 @@ def run_pt(args):
                    "rays_per_frame": int(sum(counts)), "rays_per_bounce": counts,
                    "shadow_rays_per_frame": int(sum(shadow_counts)), "shadow_rays_per_bounce": shadow_counts,
                    "triangles": scene.tri_count, "nodes": scene.desc.node_count,
-                   "frame": "one graph launch per frame (queue sizes on the device)" if args.mode == "graph"
+                   "frame": ("one graph launch per frame (queue sizes on the device), %d frames in flight on "
+                             "their own streams" % args.frames_in_flight) if args.mode == "graph"
                             else "eager launches, host reads each bounce's queue size",
+                   "frames_in_flight": args.frames_in_flight if args.mode == "graph" else 1,
                    "trace_kernel": args.trace_kernel,
                    "trace_config": cfg_dict(cfg), "autotuned": cfg is not None and not args.block,
                    "trace_config_per_bounce": [cfg_dict(c) for c in bounce_cfg] if bounce_cfg else None,
@@ -780,6 +820,7 @@ def run_c5(args):
             st = V.latency_stats(done)
             st["steps"] = server.steps
             st["wall_s"] = wall
+            st["gpu_busy_fraction"] = server.gpu_ms * 1e-3 / wall
             out[name] = st
             log("%s %s: %s" % (label, name, st))
         return out
@@ -848,7 +889,7 @@ def grid_candidates(info, qpos):
     return out
 
 
-def run_c3(args):
+def run_c3_ppm(args):
     """Photon mapping (the PPM mode of the VCM integrator) at 1280x720 over the sphere-grid scene
     with 30% mirror and glass spheres: 921,600 light paths of up to 6 hits deposit photons, a
     hash grid is built over them (r = 0.005 x the scene diagonal), one merge query per pixel
@@ -997,12 +1038,168 @@ def run_c3(args):
     return res
 
 
+def run_c3(args):
+    """One VCM iteration (VCMIntegrator<ALGO_VCM>::render, render/integrators/vcm.cpp:24-56) at
+    1280x720 over the sphere-grid scene with 30% mirror and glass spheres: 921,600 light paths of
+    up to 6 hits store light vertices and connect them to the camera, the hash grid is built over
+    the cache (r = 0.005 x the scene diagonal), camera paths of up to 6 vertices add emitter hits,
+    next-event estimation, one connection to a cached light vertex and a k = 10 merge at every
+    Lambertian vertex.  One step = one iteration; value = closest-hit rays (light + camera) per
+    second of frame time."""
+    import oracle
+    from imbatracer_amd import native as N
+    from imbatracer_amd import vcm as VC
+    from imbatracer_amd import workloads as W
+
+    t0 = time.time()
+    threads = host_cores()
+    scene = W.SphereGridScene(side=args.side, level=5, builder=N.BUILDER_SBVH, threads=threads,
+                              specular_fraction=0.3)
+    log("scene: %d triangles, built in %.1f s" % (scene.tri_count, time.time() - t0))
+    cfg = fixed_config(args)
+    vr = VC.VcmRenderer(scene, args.width, args.height, light_paths=args.c3_paths, max_path_len=6, light_max_depth=6,
+                        rr_start=0, radius_factor=0.005, k=10, trace_cfg=cfg,
+                        shadow_cfg=N.TraceConfig(256, 16, 0, 0, 4, 1, 0))
+    if cfg is None and not args.no_autotune:
+        best = None
+        for c in (N.TraceConfig(256, 16, 0, 0, 4, 1, 0), N.TraceConfig(256, 16, 0, 0, 4, 1, 64),
+                  N.TraceConfig(256, 8, 0, 0, 4, 1, 0), N.TraceConfig(128, 16, 0, 0, 4, 1, 0)):
+            vr.cfg = c
+            vr.frame()
+            t = 0.0
+            for _ in range(2):
+                vr.frame(timed=True)
+                ph = vr.phase_ms()
+                t += ph["light"] + ph["camera"]
+            log("frame tune: %s %.3f ms of light + camera passes per frame" % (cfg_dict(c), t / 2))
+            if best is None or t < best[0]:
+                best = (t, c)
+        cfg = best[1]
+        vr.cfg = cfg
+    light, cam, nv = vr.frame()
+    log("light queues %s, camera queues %s, light vertices %d" % (light, cam, nv))
+    for _ in range(args.warmup):
+        vr.frame()
+    phases, frame_ms, rays_total = [], [], 0
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        f0 = time.perf_counter()
+        l, c, n = vr.frame(timed=True)
+        frame_ms.append((time.perf_counter() - f0) * 1e3)
+        if (l, c, n) != (light, cam, nv):
+            raise RuntimeError("frame queue sizes changed between frames")
+        rays_total += sum(l) + sum(c)
+        phases.append(vr.phase_ms())
+    wall = time.perf_counter() - t_start
+    if vr.gpu_scene.error_count():
+        raise RuntimeError("traversal reported a malformed scene")
+    phase = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+    log("phases (ms): %s" % {k: round(v, 3) for k, v in phase.items()})
+    # kernel time per kind of launch (events around every launch, separate frames)
+    vr.time_kernels = True
+    kms = []
+    for _ in range(3):
+        vr.frame()
+        kms.append(vr.kernel_ms())
+    vr.time_kernels = False
+    kernel_ms = {k: float(np.mean([d[k] for d in kms])) for k in kms[0]}
+    log("kernel ms per frame: %s" % {k: round(v, 3) for k, v in kernel_ms.items()})
+
+    # one more iteration with every camera bounce's merge queries read back: a query sample
+    # against the oracle grid over the frame's light vertex cache, and the merge launches'
+    # algorithmic bytes
+    vr.keep_queues = True
+    vr.frame()
+    vr.keep_queues = False
+    a, _ = vr.cache_host()
+    og = oracle.Grid()
+    tb = time.perf_counter()
+    og.build(np.ascontiguousarray(a["pos"]), vr.radius)
+    cpu_build_s = time.perf_counter() - tb
+    info = vr.grid.info()
+    qall = np.concatenate([k["queries"] for k in vr.kept["camera"]])
+    vr.kept = {"light": [], "camera": []}
+    valid = np.nonzero(qall["pixel"] >= 0)[0]
+    rng = np.random.default_rng(3)
+    sample = np.sort(rng.choice(valid, size=min(len(valid), args.c3_check), replace=False))
+    qs = np.ascontiguousarray(qall["pos"][sample])
+    qb = N.DeviceBuffer.from_array(qs)
+    oi, od, oc = N.DeviceBuffer(len(qs) * 40), N.DeviceBuffer(len(qs) * 40), N.DeviceBuffer(len(qs) * 4)
+    vr.grid.query(qb, 12, 0, len(qs), vr.k, oi, oc, od)
+    g_idx = oi.download(np.int32, len(qs) * vr.k).reshape(-1, vr.k)
+    g_dist = od.download(np.float32, len(qs) * vr.k).reshape(-1, vr.k)
+    g_cnt = oc.download(np.int32, len(qs))
+    tq = time.perf_counter()
+    o_idx, o_dist, o_cnt = og.query_many(qs, vr.k)
+    cpu_query_s = time.perf_counter() - tq
+    bad = 0
+    for r in range(len(qs)):
+        c = g_cnt[r]
+        if c != o_cnt[r] or not np.array_equal(g_idx[r, :c], o_idx[r, :c]) or \
+                not np.array_equal(g_dist[r, :c].view(np.int32), o_dist[r, :c].view(np.int32)):
+            bad += 1
+    if bad:
+        raise RuntimeError("%d of %d sampled merge queries differ from the oracle grid" % (bad, len(qs)))
+    cand = grid_candidates(info, np.ascontiguousarray(qall["pos"][valid]))
+    found = np.minimum(cand, vr.k)  # an upper bound on the list length; the exact one is below
+    ig, dg, cg = og.query_many(np.ascontiguousarray(qall["pos"][valid]), vr.k)
+    found = cg.astype(np.int64)
+    # fused query + merge, per valid query: 64 B query record, 8 x 8 B bucket ends, 16 B per
+    # photon visited (slot-ordered position), 32 B merge record per result, 12 B image add;
+    # 4 B per empty query slot (its pixel)
+    m_bytes = float(np.sum(64 + 8 * 8 + cand * 16 + found * 32 + 12) + (len(qall) - len(valid)) * 16)
+    m_ms = kernel_ms["merge"]
+    m_achieved = m_bytes / (m_ms * 1e-3) / 1e9
+    workload = "c3-vcm-%dx%d-%d-light-paths-depth6-sphere-grid-%dtri-30pct-specular" % (
+        args.width, args.height, vr.P, scene.tri_count)
+    res = {
+        "metric": METRIC, "value": rays_total / wall / 1e6, "unit": "Mrays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded procedural scene, light and camera)",
+        "config": {"workload": workload, "width": args.width, "height": args.height, "light_paths": vr.P,
+                   "light_max_depth": 6, "max_path_len": 6, "rr_start": 0, "k": vr.k, "num_connections": vr.C,
+                   "radius": vr.radius, "light_vertices": nv,
+                   "light_rays_per_bounce": light, "camera_rays_per_bounce": cam,
+                   "trace_config": cfg_dict(cfg),
+                   "frame": "eager launches with device-side queue sizes; one host read (cache size) per frame"},
+        "phase_ms": phase,
+        "kernel_ms_per_frame": kernel_ms,
+        "frame_latency_ms": {"p50": float(np.percentile(frame_ms, 50)), "p99": float(np.percentile(frame_ms, 99))},
+        "merge": {"queries": int(len(valid)), "ms_per_frame": m_ms, "photons_visited_per_query": float(cand.mean()) if len(cand) else 0.0,
+                  "results_per_query": float(found.mean()) if len(found) else 0.0,
+                  "algorithmic_bytes_per_frame": m_bytes,
+                  "parity": {"queries_checked": int(len(qs)), "mismatches": bad,
+                             "rule": "photon index lists and squared distances of the grid query bit-exact against "
+                                     "the oracle grid built over the frame's light vertex cache"}},
+        "cpu_grid": {"build_ms": cpu_build_s * 1e3, "mqueries_s": len(qs) / cpu_query_s / 1e6, "cores": 1,
+                     "kind": "port", "sample": "oracle/hashgrid.c building over the frame's %d light vertices and "
+                                               "answering %d sampled merge queries (k = 10, one C loop)"
+                                               % (nv, len(qs))},
+        "roofline": {"bound": "hbm", "achieved": m_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": m_achieved / HBM_PEAK_GBS, "kernel": "k_vcm_merge (fused grid query + merge)",
+                     "traffic": measured_traffic(workload, "k_vcm_merge")[0],
+                     "algorithmic_bytes_per_frame": m_bytes,
+                     "bytes_rule": "per merge query 64 (record) + 8 x 8 (bucket ends) + 16 per light vertex "
+                                   "visited (slot-ordered position) + 32 per result (slot-ordered merge record) "
+                                   "+ 12 (image); 16 per empty query slot"},
+    }
+    if not args.no_cpu:
+        lr, ls = N.DeviceBuffer(vr.P * 32), N.DeviceBuffer(vr.P * 48)
+        N.check(N.lib.imba_vcm_light(C.byref(vr.frame_args), vr.light_seed, 0, vr.P, lr.ptr, ls.ptr, None),
+                "vcm_light")
+        light_rays = lr.download(N.RAY_DTYPE, vr.P)
+        res["cpu_baseline"] = cpu_baseline(scene.build, N.BUILDER_SBVH, threads, [light_rays], [False],
+                                           budget_s=args.cpu_seconds)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pt", choices=["pt", "c1", "c3", "c4", "c5"])
+    ap.add_argument("--workload", default="pt", choices=["pt", "c1", "c3", "c3-ppm", "c4", "c5"])
     ap.add_argument("--c5-requests", type=int, default=300, help="c5: requests in the run")
     ap.add_argument("--c5-rate", type=float, default=200.0, help="c5: mean arrivals per second")
     ap.add_argument("--c5-arena-gb", type=int, default=48, help="c5: device memory for request buffers")
@@ -1024,6 +1221,8 @@ def main():
     ap.add_argument("--stack", type=int, default=0)
     ap.add_argument("--persistent", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--frames-in-flight", type=int, default=2,
+                    help="pt graph mode: frame loops taking turns on their own streams")
     ap.add_argument("--mode", default="graph", choices=["graph", "eager"],
                     help="pt: frames as one captured graph (default) or as eager launches")
     ap.add_argument("--trace-kernel", default="plain", choices=["plain", "batch"],
@@ -1054,7 +1253,8 @@ def main():
         N.check(N.lib.imba_memcpy_d2d(b.ptr, a.ptr, nbytes, None), "copy")
         print(json.dumps({"calibrate_copy_bytes": nbytes}))
         return
-    res = {"pt": run_pt, "c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args)
+    res = {"pt": run_pt, "c1": run_c1, "c3": run_c3, "c3-ppm": run_c3_ppm, "c4": run_c4,
+           "c5": run_c5}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

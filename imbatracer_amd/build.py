@@ -27,7 +27,8 @@ ARCH = "gfx950"
 HOST_SOURCES = ["host/vec.cpp", "host/status.cpp", "host/scene_build.cpp", "host/gen.cpp", "host/bvh_emit.cpp",
                 "host/abi_check.cpp", "host/bvh_cache.cpp", "host/wide_bvh.cpp"]
 HIP_SOURCES = ["hip/runtime.hip", "hip/traverse.hip", "hip/compact.hip", "hip/hashgrid.hip", "hip/pt.hip",
-               "hip/sort.hip", "hip/photon.hip", "hip/shade_batch.hip", "hip/probe.hip"]
+               "hip/sort.hip", "hip/photon.hip", "hip/shade_batch.hip", "hip/probe.hip",
+               "hip/vcm.hip"]
 
 HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-I" + INCLUDE, "-pthread"]
 HIP_FLAGS = [

@@ -39,7 +39,9 @@ struct TraceArgs {
     const int* masks;     int n_masks;       // 3 ints per mask: width, height, offset
     const unsigned char* mask_buf; int mask_buf_size;
     const float4* rays;   int4* hits; int count; int root;
-    uint32_t* counts;     // 3 per ray (stats variant)
+    uint32_t* counts;     // 3 per ray (stats variant), or null
+    uint32_t* depth_hist; // stats variant: rays by the deepest stack entry they used (64 bins,
+                          // the last one for 63 and deeper), or null
     int* err;             // incremented once per ray that met a malformed structure
     int* spill;           // global stack entries beyond the LDS part
     int spill_depth;      // spill entries per thread
@@ -169,6 +171,17 @@ __device__ __forceinline__ IMBA_AS(1) T* gview_w(T* p) {
     return (IMBA_AS(1) T*)p;
 }
 
+// Per-lane traversal stack: entry 0..sp-1 in LDS ([depth][lane], conflict free), then in the
+// global spill buffer.  With IMBA_STACK_TOP_REG=1 the top entry lives in a register instead: a
+// pop returns it at once and issues the load that refills it, which has a whole node step to
+// arrive; a push writes the previous top out.  Measured on the 21M-triangle PT frame (quantized
+// 4-wide, 256 x 16): 970 / 973 against 975 Mrays/s with every entry in LDS (one more VGPR, 4
+// more bytes of scratch per lane in the 7-wave variants), so the pops' LDS latency is hidden
+// already and the option stays off.
+#ifndef IMBA_STACK_TOP_REG
+#define IMBA_STACK_TOP_REG 0
+#endif
+
 template <int BLOCK, int STACK_LDS>
 struct LaneStack {
     int* lds;     // [STACK_LDS][BLOCK]
@@ -178,23 +191,37 @@ struct LaneStack {
     int spill_slot;  // global thread index
     int spill_stride;
     int sp;
+    int top;          // entry sp - 1 (IMBA_STACK_TOP_REG)
     const uint4* tl;  // LDS copy of the treelet nodes (64 B each, as in memory), or null
 
+    __device__ __forceinline__ void store(int k, int v) {
+        if (k < STACK_LDS) lds_view(lds)[k * BLOCK + lane_slot] = v;
+        else spill[(size_t)(k - STACK_LDS) * spill_stride + spill_slot] = v;
+    }
+    __device__ __forceinline__ int load(int k) {
+        if (k < STACK_LDS) return lds_view(lds)[k * BLOCK + lane_slot];
+        return spill[(size_t)(k - STACK_LDS) * spill_stride + spill_slot];
+    }
     __device__ __forceinline__ bool push(int v) {
-        if (sp < STACK_LDS) {
-            lds_view(lds)[sp * BLOCK + lane_slot] = v;
-        } else {
-            const int k = sp - STACK_LDS;
-            if (k >= spill_depth) return false;
-            spill[(size_t)k * spill_stride + spill_slot] = v;
-        }
+        if (sp - STACK_LDS >= spill_depth) return false;
+#if IMBA_STACK_TOP_REG
+        if (sp > 0) store(sp - 1, top);
+        top = v;
+#else
+        store(sp, v);
+#endif
         sp++;
         return true;
     }
     __device__ __forceinline__ int pop() {
         sp--;
-        if (sp < STACK_LDS) return lds_view(lds)[sp * BLOCK + lane_slot];
-        return spill[(size_t)(sp - STACK_LDS) * spill_stride + spill_slot];
+#if IMBA_STACK_TOP_REG
+        const int v = top;
+        if (sp > 0) top = load(sp - 1);
+        return v;
+#else
+        return load(sp);
+#endif
     }
 };
 
@@ -217,6 +244,7 @@ struct TraceState {
     int inst, inst_id, inst_sp;  // current instance, its id, stack depth when it was entered
     int flags;
     uint32_t cn, ct, ci;  // visit counts (stats variant)
+    uint32_t cd;          // deepest stack entry used (stats variant)
 };
 
 // The current space becomes world space (the reciprocals kept from the start: the values
@@ -248,7 +276,7 @@ __device__ __forceinline__ void state_init(TraceState& s, const float4& ro, cons
     s.inst_id = -1;
     s.inst_sp = -1;
     s.flags = kTop | kInstLast;
-    s.cn = s.ct = s.ci = 0;
+    s.cn = s.ct = s.ci = s.cd = 0;
 }
 
 // Enters instance i of the current top-level leaf: the ray goes to its object space.
@@ -329,6 +357,7 @@ __device__ __forceinline__ void node_step2(const TraceArgs& a, TraceState& s, La
     if (hl && hr) {
         const bool left_first = tl <= tr;
         if (!st.push(left_first ? n3.y : n3.x)) s.flags |= kBad;
+        if (STATS) s.cd = max(s.cd, (uint32_t)st.sp);
         s.node = left_first ? n3.x : n3.y;
     } else {
         s.node = hl ? n3.x : (hr ? n3.y : kNoNode);
@@ -388,6 +417,7 @@ __device__ __forceinline__ void node_step4(const TraceArgs& a, TraceState& s, La
         if (cnt >= 4) ok &= st.push(c3);
         if (cnt >= 3) ok &= st.push(c2);
         if (cnt >= 2) ok &= st.push(c1);
+        if (STATS) s.cd = max(s.cd, (uint32_t)st.sp);
         if (!ok) s.flags |= kBad;
         s.node = c0;
     }
@@ -454,6 +484,7 @@ __device__ __forceinline__ void node_step4q(const TraceArgs& a, TraceState& s, L
         if (cnt >= 4) ok &= st.push(c3);
         if (cnt >= 3) ok &= st.push(c2);
         if (cnt >= 2) ok &= st.push(c1);
+        if (STATS) s.cd = max(s.cd, (uint32_t)st.sp);
         if (!ok) s.flags |= kBad;
         s.node = c0;
     }
@@ -565,11 +596,12 @@ __device__ __forceinline__ void treelet_init(const TraceArgs& a, LaneStack<BLOCK
 template <bool STATS>
 __device__ __forceinline__ void write_result(const TraceArgs& a, const TraceState& s, int id) {
     gview_w(a.hits)[id] = make_int4(s.best_tri, s.best_inst, __float_as_int(s.best_t), __float_as_int(s.best_u));
-    if (STATS) {
+    if (STATS && a.counts) {
         a.counts[3 * (size_t)id + 0] = s.cn;
         a.counts[3 * (size_t)id + 1] = s.ct;
         a.counts[3 * (size_t)id + 2] = s.ci;
     }
+    if (STATS && a.depth_hist) atomicAdd(&a.depth_hist[min(s.cd, 63u)], 1u);
     if (s.flags & kBad) atomicAdd(a.err, 1);
 }
 

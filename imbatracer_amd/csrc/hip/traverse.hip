@@ -182,7 +182,8 @@ struct TreeView {
 };
 
 int launch_trace(const TreeView& t, int* d_err, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts, int count,
-                 bool any, bool masked, const ImbaTraceConfig* cfg, void* stream, const int* count_dev = nullptr) {
+                 bool any, bool masked, const ImbaTraceConfig* cfg, void* stream, const int* count_dev = nullptr,
+                 uint32_t* depth_hist = nullptr) {
     if (count < 0) return fail(IMBA_ERR_INVALID_ARG, "count < 0");
     if (count == 0) return IMBA_OK;
     if (!rays || !hits) return fail(IMBA_ERR_INVALID_ARG, "rays or hits is NULL");
@@ -201,12 +202,14 @@ int launch_trace(const TreeView& t, int* d_err, const ImbaRay* rays, ImbaHit* hi
     a.count = count;
     a.root = d.root;
     a.counts = counts;
+    a.depth_hist = depth_hist;
     a.err = d_err;
+    const bool stats = counts || depth_hist;
     if (cfg && cfg->treelet) {
         if (cfg->treelet < 0 || cfg->treelet > kTreeletMaxNodes)
             return fail(IMBA_ERR_INVALID_ARG, "treelet must be 0..%d nodes", kTreeletMaxNodes);
         if (t.tree != 2) return fail(IMBA_ERR_INVALID_ARG, "treelet needs the quantized 4-wide tree");
-        if (!counts) {  // visit counts do not depend on where a node is read from
+        if (!stats) {  // visit counts do not depend on where a node is read from
             a.tl_base = t.info->treelet_base;
             a.tl_nodes = std::min(cfg->treelet, t.info->treelet_nodes);
         }
@@ -215,7 +218,7 @@ int launch_trace(const TreeView& t, int* d_err, const ImbaRay* rays, ImbaHit* hi
     hipStream_t s = resolve_stream(stream, &sync);
     const int ms = t.info->max_stack;
     int r;
-    if (counts) {
+    if (stats) {
         r = t.tree == 2 ? dispatch_flags<true, 2>(a, ms, any, masked, nullptr, s)
           : t.tree == 1 ? dispatch_flags<true, 1>(a, ms, any, masked, nullptr, s)
                         : dispatch_flags<true, 0>(a, ms, any, masked, nullptr, s);
@@ -700,12 +703,14 @@ int imba_trace_batch_release(ImbaTraceBatch* b) {
 }
 
 static int trace_entry(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts, int count, bool any,
-                       const ImbaTraceConfig* cfg, void* stream, const int* count_dev) {
+                       const ImbaTraceConfig* cfg, void* stream, const int* count_dev,
+                       uint32_t* depth_hist = nullptr) {
     IMBA_CHECK_ARG(s, "scene is NULL");
     TreeView t{};
     int r = tree_of(s, cfg, &t);
     if (r) return r;
-    return launch_trace(t, s->d_error, rays, hits, counts, count, any, !s->info.all_opaque, cfg, stream, count_dev);
+    return launch_trace(t, s->d_error, rays, hits, counts, count, any, !s->info.all_opaque, cfg, stream, count_dev,
+                        depth_hist);
 }
 
 int imba_intersect(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int count, const ImbaTraceConfig* cfg,
@@ -748,6 +753,12 @@ int imba_trace_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uin
                      int any_hit, const ImbaTraceConfig* cfg, void* stream) {
     IMBA_CHECK_ARG(counts, "counts is NULL");
     return trace_entry(s, rays, hits, counts, count, any_hit != 0, cfg, stream, nullptr);
+}
+
+int imba_trace_stack_depths(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int count, int any_hit,
+                            const ImbaTraceConfig* cfg, uint32_t* depth_hist, void* stream) {
+    IMBA_CHECK_ARG(depth_hist, "depth_hist is NULL");
+    return trace_entry(s, rays, hits, nullptr, count, any_hit != 0, cfg, stream, nullptr, depth_hist);
 }
 
 int imba_trace_desc(const ImbaSceneDesc* dev, const ImbaRay* rays, ImbaHit* hits, int count, int any_hit,

@@ -171,6 +171,37 @@ int imba_gen_camera_matrix(const float* pos3, const float* dir3, const float* up
     return IMBA_OK;
 }
 
+int imba_gen_camera_frame(const float* pos3, const float* dir3, const float* up3, float fov_deg, int width,
+                          int height, float* out16, float* forward3, float* image_plane_dist) {
+    IMBA_CHECK_ARG(pos3 && dir3 && up3 && out16 && forward3 && image_plane_dist && width > 0 && height > 0,
+                   "bad arguments");
+    const float3 pos(pos3[0], pos3[1], pos3[2]);
+    const float3 dir = normalize(float3(dir3[0], dir3[1], dir3[2]));
+    const float3 right = normalize(cross(dir, float3(up3[0], up3[1], up3[2])));
+    const float3 up = cross(dir, right);
+    const float3 nright(-right.x, -right.y, -right.z), ndir(-dir.x, -dir.y, -dir.z);
+    const float3 lp(dot(up, pos), dot(nright, pos), dot(ndir, pos));
+    Mat4 w2c{};
+    const float3 rows[3] = {up, nright, ndir};
+    const float offs[3] = {-lp.x, -lp.y, -lp.z};
+    for (int r = 0; r < 3; r++) {
+        w2c.m[r][0] = rows[r].x; w2c.m[r][1] = rows[r].y; w2c.m[r][2] = rows[r].z; w2c.m[r][3] = offs[r];
+    }
+    w2c.m[3][3] = 1.0f;
+    const Mat4 w2s = mul(perspective(fov_deg, 1.0f, 0.1f, 10000.0f), w2c);
+    // world_to_raster_ = scale(h / 2, w / 2, 0, 1) * translate(1, 1, 0) * world_to_screen
+    const Mat4 w2r = mul(mul(diag((float)height * 0.5f, (float)width * 0.5f, 0.0f, 1.0f),
+                             translation(1.0f, 1.0f, 0.0f)), w2s);
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) out16[4 * r + c] = w2r.m[r][c];
+    forward3[0] = dir.x;
+    forward3[1] = dir.y;
+    forward3[2] = dir.z;
+    const float tan_half = std::tan(fov_deg * kPi / 360.0f);
+    *image_plane_dist = (float)width / (2.0f * tan_half);
+    return IMBA_OK;
+}
+
 int imba_gen_camera_rays(const float* pos3, const float* dir3, const float* up3, float fov_deg,
                          int width, int height, ImbaRay* rays) {
     IMBA_CHECK_ARG(pos3 && dir3 && up3 && rays && width > 0 && height > 0, "bad arguments");

@@ -178,6 +178,7 @@ _sig("imba_intersect_stats", i32, vp, vp, vp, vp, i32, vp)
 _sig("imba_occluded_stats", i32, vp, vp, vp, vp, i32, vp)
 _sig("imba_trace_error_count", i32, vp, ip)
 _sig("imba_trace_stats", i32, vp, vp, vp, vp, i32, i32, C.POINTER(TraceConfig), vp)
+_sig("imba_trace_stack_depths", i32, vp, vp, vp, i32, i32, C.POINTER(TraceConfig), vp, vp)
 _sig("imba_wide_create", i32, C.POINTER(SceneDesc), C.POINTER(vp))
 _sig("imba_wide_get", i32, vp, C.POINTER(vp), ip, C.POINTER(vp), ip, ip, ip)
 _sig("imba_wide_get_quantized", i32, vp, C.POINTER(vp), ip)
@@ -214,6 +215,13 @@ _sig("imba_gen_icosphere_counts", i32, i32, ip, ip)
 _sig("imba_gen_icosphere", i32, i32, f32, f32, f32, f32, C.c_uint32, vp, vp)
 _sig("imba_gen_camera_rays", i32, vp, vp, vp, f32, i32, i32, vp)
 _sig("imba_gen_camera_matrix", i32, vp, vp, vp, f32, i32, i32, vp)
+_sig("imba_gen_camera_frame", i32, vp, vp, vp, f32, i32, i32, vp, vp, C.POINTER(f32))
+_sig("imba_vcm_light", i32, vp, C.c_uint32, i32, i32, vp, vp, vp)
+_sig("imba_vcm_camera", i32, vp, C.c_uint32, i32, i32, vp, vp, vp)
+_sig("imba_vcm_light_bounce", i32, vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp)
+_sig("imba_vcm_camera_bounce", i32, vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp)
+_sig("imba_vcm_merge_records", i32, vp, vp, vp, vp, vp)
+_sig("imba_vcm_merge", i32, vp, vp, vp, vp, i32, vp, i32, vp, vp)
 _sig("imba_pt_camera", i32, vp, vp, i32, i32, i32, C.c_uint32, i32, i32, vp, vp, vp)
 _sig("imba_pt_shade", i32, C.POINTER(PtShading), vp, vp, vp, i32, vp, vp, vp, vp, vp, vp)
 _sig("imba_pt_shadow", i32, vp, vp, i32, vp, vp, vp)

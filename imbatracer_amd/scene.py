@@ -292,6 +292,16 @@ class Scene:
         check(N.lib.imba_trace_stats(self.h, rays.ptr, hits.ptr, counts.ptr, count, int(any_hit),
                                      C.byref(cfg) if cfg else None, N._s(stream)), "imba_trace_stats")
 
+    def stack_depths(self, rays: np.ndarray, any_hit: bool = False, cfg: N.TraceConfig | None = None) -> np.ndarray:
+        """Rays of the host array `rays` by the deepest traversal stack entry they used (64 bins,
+        the last one for 63 and deeper) on the tree `cfg` selects."""
+        rb = DeviceBuffer.from_array(rays)
+        hb = DeviceBuffer(max(len(rays), 1) * HIT_DTYPE.itemsize)
+        hist = DeviceBuffer.from_array(np.zeros(64, np.uint32))
+        check(N.lib.imba_trace_stack_depths(self.h, rb.ptr, hb.ptr, len(rays), int(any_hit),
+                                            C.byref(cfg) if cfg else None, hist.ptr, None), "imba_trace_stack_depths")
+        return hist.download(np.uint32, 64)
+
     def trace_host(self, rays: np.ndarray, any_hit: bool = False, cfg: N.TraceConfig | None = None) -> np.ndarray:
         """Convenience: upload rays, trace, download hits."""
         rb = DeviceBuffer.from_array(rays)

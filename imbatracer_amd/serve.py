@@ -358,6 +358,11 @@ class Server:
         self.grids: list = []  # hash grids of finished photon-mapping requests, for reuse
         self.steps = 0
         self.t0 = None
+        # device time of every step, from its first launch to the read-back of the queue sizes
+        # (event pair recorded each step): gpu_ms / wall time is the share of the run the GPU had
+        # work queued; the rest it waited for the host
+        self._ev = (N.Event(), N.Event())
+        self.gpu_ms = 0.0
 
     def _shade_item(self, j) -> N.ShadeItem:
         return N.ShadeItem(j.rays[j.cur].value, j.hits.value, j.states[j.cur].value, j.n,
@@ -418,6 +423,7 @@ class Server:
         camera rays, or queries and merging); one batched traversal over every active queue;
         the PT bounces and compactions in one batched launch and the photon-mapping ones; one
         read-back of all queue sizes."""
+        self._ev[0].record(self.stream)
         self._admit()
         if not self.active:
             return
@@ -473,7 +479,9 @@ class Server:
             j.bounce += 1
             j.cur = 1 - j.cur
         allc = self.counts.download(np.int32, self.cs * self.max_active, self.stream)
+        self._ev[1].record(self.stream)
         self.stream.sync()
+        self.gpu_ms += self._ev[0].elapsed_ms(self._ev[1])
         allc = allc.reshape(self.max_active, self.cs)
         self.steps += 1
         still = []

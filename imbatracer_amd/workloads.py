@@ -269,9 +269,10 @@ class PathTracer:
 
     def __init__(self, scene: SphereGridScene, width: int, height: int, spp: int = 1, max_path_len: int = 8,
                  rr_start: int = 3, seed_base: int = 0x1234567, trace_cfg: N.TraceConfig | None = None,
-                 shadow_cfg: N.TraceConfig | None = None):
+                 shadow_cfg: N.TraceConfig | None = None, gpu_scene: S.Scene | None = None):
+        """gpu_scene: an uploaded scene to share (another frame loop over the same scene)."""
         self.s = scene
-        self.gpu_scene = S.Scene(scene.desc)
+        self.gpu_scene = gpu_scene if gpu_scene is not None else S.Scene(scene.desc)
         self.w, self.h, self.spp = width, height, spp
         self.n = width * height * spp
         self.max_path_len, self.rr_start, self.seed_base = max_path_len, rr_start, seed_base
@@ -469,9 +470,22 @@ class PathTracer:
 
     def frame_graph(self):
         """One frame by graph launch; returns the per-bounce ray counts (non-empty bounces)."""
+        self.launch_graph()
+        return self.finish_graph()
+
+    def launch_graph(self):
+        """Enqueues one graph frame and the read-back of its queue sizes on this loop's stream
+        (frames of several loops on their own streams then overlap on the device)."""
         self.graph.launch(self.stream)
-        c = self.dev_counts.download(np.int32, self.max_path_len + 1, self.stream)
+        if getattr(self, "_counts_host", None) is None:
+            self._counts_host = np.zeros(self.max_path_len + 1, np.int32)
+        N.check(N.lib.imba_memcpy_d2h(N.ptr(self._counts_host), self.dev_counts.ptr, self._counts_host.nbytes,
+                                      self.stream.handle), "counts")
+
+    def finish_graph(self):
+        """Waits for the frame launch_graph enqueued; returns its per-bounce ray counts."""
         self.stream.sync()
+        c = self._counts_host
         out = []
         for v in c[:self.max_path_len]:
             if v == 0:

@@ -309,6 +309,12 @@ int imba_occluded_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, 
  * (4-wide nodes for width 4), triangles tested, instances entered. */
 int imba_trace_stats(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, uint32_t* counts, int count,
                      int any_hit, const ImbaTraceConfig* cfg, void* stream);
+/* Stack use of the tree a configuration traverses: adds one to depth_hist[d] (device, 64 uint32
+ * bins, not cleared here) for every ray whose deepest traversal stack entry was d (bin 63: 63
+ * and deeper).  With the 16-entry LDS stack of the default launches, the rays of bins > 16
+ * spill stack entries to global memory.  Same hits as imba_intersect / imba_occluded. */
+int imba_trace_stack_depths(const ImbaScene* s, const ImbaRay* rays, ImbaHit* hits, int count, int any_hit,
+                            const ImbaTraceConfig* cfg, uint32_t* depth_hist, void* stream);
 /* Traversal over caller-owned device arrays described by `dev` (device pointers and
  * element counts; a count of 0 means "as many records as the allocation holding the pointer
  * has room for"), for callers that manage their own TraversalData.  Nothing derived from the
@@ -579,6 +585,88 @@ int imba_ppm_merge(const void* queries, int nq, int k, const int* out_idx, const
                    const void* photon_rays, const void* photon_states, float radius, int light_paths, float* image,
                    void* stream);
 
+/* ------------------------------ vertex connection and merging ---------------------------
+ * One iteration of VCMIntegrator<ALGO_VCM> (render/integrators/vcm.cpp) on the device, for the
+ * materials of the measured scenes (kinds: 0 Lambertian, 1 mirror, 2 glass of index 1.5) and
+ * triangle lights, with the reference's partial MIS weights (dVC, dVCM, dVM) and float
+ * operation order.  States are IMBA_VCM_STATE_BYTES-byte records: int pixel_id (light paths:
+ * the path index; -1 once ended), int sample_id, uint64 rng, float throughput[3],
+ * int path_length, float dVC, dVCM, dVM, int pad.  Shadow states are the 16-byte records of
+ * imba_pt_shadow (int pixel_id, -1 for an empty slot; float contribution[3]), whose
+ * unoccluded contributions imba_pt_shadow adds to the image.
+ * Light vertices (LightPathVertex, light_vertices.h:17-37) are two 32-byte records each:
+ *   vertices_a: float4(position, path_length bits), float4(direction back along the light
+ *               ray, triangle id bits);
+ *   vertices_b: float4(key bits, throughput) with key = path index or -1 for an empty slot,
+ *               float4(dVCM, dVM, dVC, material bits).
+ * imba_vcm_light: light paths first..first+count-1 (trace_light_paths, vcm.cpp:64-96): path i
+ *   belongs to light (runs of light_paths / light_count, the remainder to light 0), seeded as
+ *   LightRayGen (ray_gen.h:138-145), sampled as TriangleLight::sample_emit (light.h:109-141).
+ * imba_vcm_light_bounce: process_light_rays (vcm.cpp:196-256) for every entry: the partial
+ *   weights at the hit, a light vertex into slot i of vertices_a / vertices_b (every other slot
+ *   of the capacity gets key -1) and the camera connection (connect_to_camera :258-318) into
+ *   shadow slot i on a Lambertian hit, then the bounce (:129-193); paths end after
+ *   light_max_depth hits (the reference ends light paths by Russian roulette only).
+ *   The vertex slots of all bounces are stable-compacted with imba_compact_rays (vertices_a as
+ *   the rays, vertices_b as 32-byte states) into the dense light vertex cache.
+ * imba_vcm_camera: camera samples first..first+count-1 (trace_camera_paths :98-127), seeded as
+ *   PixelRayGen, dVCM = light_paths / pdf_cam_w.
+ * imba_vcm_camera_bounce: process_camera_rays (vcm.cpp:320-443) for every entry: emitter hits
+ *   with their MIS weight into image, direct_illum (:445-488) into shadow slot (1 + C) i,
+ *   num_connections = C connections to light vertices drawn from the cache (:490-570) into
+ *   slots (1 + C) i + 1 .. (1 + C) i + C, the merge query of a Lambertian vertex into query
+ *   slot i (4 float4: position | pixel bits, normal | dVCM, direction to the previous vertex |
+ *   dVM, throughput | material bits; an empty slot has position +inf and pixel -1), then the
+ *   bounce.  *shadow_count_dev (nullable) receives (1 + C) * entries.  vertex_count_dev: the
+ *   device count of the cache (NULL: no connections).
+ * imba_vcm_merge_records: 32-byte merge records of the cache in the grid's slot order (the
+ *   grid built over vertices_a positions, stride 32): direction | dVCM, throughput | dVM.
+ * imba_vcm_merge: vertex_merging (vcm.cpp:572-611) for every query slot: the k nearest light
+ *   vertices within the radius (the grid's query, 1 <= k <= 64) and their Epanechnikov- and
+ *   MIS-weighted throughputs into image, in one launch that reads the records by grid slot.
+ * count_dev (nullable): the queue size read on the device when the launch runs. */
+enum { IMBA_VCM_STATE_BYTES = 48 };
+typedef struct ImbaVcmFrame {
+    const float* tri_normals;   /* device, 3 per triangle (as ImbaPtShading)                 */
+    const int* tri_materials;
+    int tri_count;
+    const float* albedo;        /* device, 3 per material                                    */
+    const float* emission;      /* device, 3 per material                                    */
+    const float* emitter_area;  /* device, 1 per material                                    */
+    const int* kinds;           /* device, 1 per material: 0 Lambertian, 1 mirror, 2 glass   */
+    int material_count;
+    const float* lights;        /* device, 12 per triangle light: p0, p1, p2, intensity      */
+    int light_count;            /* >= 1                                                      */
+    float raster_to_world[16];  /* imba_gen_camera_frame                                     */
+    float world_to_raster[16];
+    float cam_pos[3];
+    float cam_dir[3];           /* the camera's unit forward direction                       */
+    float image_plane_dist;
+    int width, height, spp;
+    int light_paths;            /* settings.light_path_count                                 */
+    float radius;               /* the merge radius of this iteration                        */
+    int max_path_len;
+    int light_max_depth;
+    int rr_start;               /* Russian roulette from this bounce on (0: the reference)   */
+    int num_connections;        /* 1..8                                                      */
+} ImbaVcmFrame;
+
+int imba_vcm_light(const ImbaVcmFrame* frame, uint32_t seed_base, int first, int count, ImbaRay* rays, void* states,
+                   void* stream);
+int imba_vcm_light_bounce(const ImbaVcmFrame* frame, const ImbaRay* rays, const ImbaHit* hits, void* states,
+                          int capacity, const int* count_dev, void* vertices_a, void* vertices_b, ImbaRay* rays_out,
+                          ImbaRay* shadow_rays, void* shadow_states, void* stream);
+int imba_vcm_camera(const ImbaVcmFrame* frame, uint32_t seed_base, int first, int count, ImbaRay* rays, void* states,
+                    void* stream);
+int imba_vcm_camera_bounce(const ImbaVcmFrame* frame, const ImbaRay* rays, const ImbaHit* hits, void* states,
+                           int capacity, const int* count_dev, const void* vertices_a, const void* vertices_b,
+                           const int* vertex_count_dev, ImbaRay* rays_out, ImbaRay* shadow_rays, void* shadow_states,
+                           int* shadow_count_dev, void* queries, float* image, void* stream);
+int imba_vcm_merge_records(const ImbaHashGrid* grid, const void* vertices_a, const void* vertices_b, void* records,
+                           void* stream);
+int imba_vcm_merge(const ImbaVcmFrame* frame, const ImbaHashGrid* grid, const void* records, const void* queries,
+                   int capacity, const int* count_dev, int k, float* image, void* stream);
+
 /* ------------------------------ measurement ----------------------------------------------
  * (No counterpart in the reference: a probe for roofline reporting.)  Reads the device buffer
  * of `bytes` bytes `passes` times with 16-byte loads from every CU, each pass in a rotated
@@ -606,6 +694,12 @@ int imba_gen_camera_rays(const float* pos3, const float* dir3, const float* up3,
 /* The camera's raster -> world matrix (row major, applied to (raster y, raster x, 0, 1)). */
 int imba_gen_camera_matrix(const float* pos3, const float* dir3, const float* up3, float fov_deg,
                            int width, int height, float* raster_to_world16);
+/* The rest of PerspectiveCamera (camera.h:27-52, 66-69) for light tracing: world_to_raster
+ * (row major, applied to (x, y, z, 1); raster x = t[1] / t[3], raster y = t[0] / t[3]), the
+ * unit forward direction and the image plane distance width / (2 tan(fov / 2)). */
+int imba_gen_camera_frame(const float* pos3, const float* dir3, const float* up3, float fov_deg,
+                          int width, int height, float* world_to_raster16, float* forward3,
+                          float* image_plane_dist);
 
 #ifdef __cplusplus
 }
