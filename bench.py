@@ -484,6 +484,11 @@ def run_pt(args):
                                   "kernel": "k_trace (any hit) over the shadow slots",
                                   "algorithmic_bytes_per_launch": sum(shadow_bytes) / len(shadow_bytes),
                                   "visits_per_slot": [float(x) / sum(counts) for x in np.sum(shadow_visits, 0)]}},
+        "frame_traversal_bandwidth": {
+            "achieved": (sum(stat_bytes) + sum(shadow_bytes)) / (wall / args.steps) / 1e9, "unit": "GB/s",
+            "frac": (sum(stat_bytes) + sum(shadow_bytes)) / (wall / args.steps) / 1e9 / HBM_PEAK_GBS,
+            "rule": "algorithmic bytes of a frame's closest-hit and shadow traversals over the measured frame "
+                    "time (frames overlapping as configured): the traversal kernels' combined rate"},
         "per_bounce": table,
         "frame_latency_ms": {"p50": float(np.percentile(frame_ms, 50)), "p99": float(np.percentile(frame_ms, 99))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -859,34 +864,61 @@ def run_c5(args):
     return res
 
 
-def grid_candidates(info, qpos):
+def grid_candidates(info, qpos, limit=None):
     """Photons visited per query by HashGrid::query (rangesearch.h:77-129): the sizes of the 8
-    probed buckets (a bucket probed twice counts twice); 0 for queries outside the bounds."""
-    ends = np.empty(info.table_size, np.int32)
+    probed buckets (a bucket probed twice counts twice); 0 for queries outside the bounds.
+    With `limit` (per query: the squared distance of its k-th result when its list is full,
+    else the squared radius) also the photons of the probed buckets whose photon box lies within
+    that final limit: buckets the scan must read whatever the order (it skips a bucket only when
+    the box lies beyond the limit at the bucket's turn, and the limit only shrinks), a lower
+    bound of what the kernel reads."""
     from imbatracer_amd import native as N
 
+    ends = np.empty(info.table_size, np.int32)
     N.check(N.lib.imba_memcpy_d2h(N.ptr(ends), C.c_void_p(info.cell_ends), ends.nbytes, None), "d2h")
     size = np.diff(np.concatenate([[0], ends])).astype(np.int64)
     bmin = np.array(info.bbox_min, np.float32)
     bmax = np.array(info.bbox_max, np.float32)
     q = np.ascontiguousarray(qpos, np.float32)
     inside = np.all((q >= bmin) & (q <= bmax), axis=1)
-    q = q[inside]
-    c = np.float32(info.inv_cell_size) * (q - bmin)
+    qi = q[inside]
+    c = np.float32(info.inv_cell_size) * (qi - bmin)
     f = np.floor(c)
     p = f.astype(np.int64)
     o = p + np.where((c - f) < np.float32(0.5), -1, 1)
-    total = np.zeros(len(q), np.int64)
+    total = np.zeros(len(qi), np.int64)
+    needed = np.zeros(len(qi), np.int64)
+    if limit is not None:
+        pos = np.empty((info.photon_count, 4), np.float32)
+        N.check(N.lib.imba_memcpy_d2h(N.ptr(pos), C.c_void_p(info.positions), pos.nbytes, None), "d2h")
+        nonempty = np.nonzero(size)[0]
+        starts = (ends[nonempty] - size[nonempty]).astype(np.int64)
+        lo = np.full((info.table_size, 3), np.inf, np.float32)
+        hi = np.full((info.table_size, 3), -np.inf, np.float32)
+        if len(nonempty):
+            lo[nonempty] = np.minimum.reduceat(pos[:, :3], starts, axis=0)
+            hi[nonempty] = np.maximum.reduceat(pos[:, :3], starts, axis=0)
+        lim = np.asarray(limit, np.float32)[inside]
     for j in range(8):
         x = o[:, 0] if j & 4 else p[:, 0]
         y = o[:, 1] if j & 2 else p[:, 1]
         z = o[:, 2] if j & 1 else p[:, 2]
         h = ((x.astype(np.uint32) * np.uint32(73856093)) ^ (y.astype(np.uint32) * np.uint32(19349663))
              ^ (z.astype(np.uint32) * np.uint32(83492791))) % np.uint32(info.table_size)
-        total += size[h.astype(np.int64)]
+        h = h.astype(np.int64)
+        total += size[h]
+        if limit is not None:
+            with np.errstate(invalid="ignore"):
+                g = np.maximum(np.maximum(lo[h] - qi, qi - hi[h]), np.float32(0))
+                lb = (g[:, 0] * g[:, 0] + g[:, 1] * g[:, 1]) + g[:, 2] * g[:, 2]
+            needed += np.where(lb <= lim, size[h], 0)
     out = np.zeros(len(qpos), np.int64)
     out[inside] = total
-    return out
+    if limit is None:
+        return out
+    out_needed = np.zeros(len(qpos), np.int64)
+    out_needed[inside] = needed
+    return out, out_needed
 
 
 def run_c3_ppm(args):
@@ -984,9 +1016,11 @@ def run_c3_ppm(args):
     if bad:
         raise RuntimeError("%d of %d sampled photon queries differ from the oracle grid" % (bad, len(sample)))
     info = pm.grid.info()
-    cand = grid_candidates(info, qpos)
     found = cnt.astype(np.int64)
-    q_bytes = float(np.sum(np.where(np.isfinite(qpos[:, 0]), 12 + 8 * 8, 12) + cand * 12 + found * 8 + 4))
+    r2 = np.float32(pm.radius) * np.float32(pm.radius)
+    lim = np.where(found == pm.k, dist[:, pm.k - 1], r2)
+    cand, needed = grid_candidates(info, qpos, lim)
+    q_bytes = float(np.sum(np.where(np.isfinite(qpos[:, 0]), 12 + 8 * 8 + 8 * 24, 12) + needed * 16 + found * 8 + 4))
     q_ms = phase["query"]
     achieved = q_bytes / (q_ms * 1e-3) / 1e9
     n_q = pm.n_cam
@@ -1011,6 +1045,7 @@ def run_c3_ppm(args):
         "grid": {"photons": n_ph, "table_size": info.table_size, "build_ms": phase["grid_build"],
                  "queries": n_q, "query_ms": q_ms, "mqueries_s": n_q / (q_ms * 1e-3) / 1e6,
                  "photons_visited_per_query": float(cand.mean()),
+                 "photons_read_per_query_lower_bound": float(needed.mean()),
                  "results_per_query": float(found.mean()),
                  "parity": {"queries_checked": int(len(sample)), "mismatches": bad,
                             "rule": "photon index lists and squared distances bit-exact against the oracle grid "
@@ -1019,8 +1054,10 @@ def run_c3_ppm(args):
                                     "frac": achieved / HBM_PEAK_GBS, "kernel": "k_query",
                                     "traffic": q_traffic, "traffic_source": q_traffic_src,
                                     "algorithmic_bytes_per_launch": q_bytes,
-                                    "bytes_rule": "12 + 8x8 per query inside the bounds, 12 per photon visited "
-                                                  "(its position), 8 per result, 4 per count"},
+                                    "bytes_rule": "12 + 8x8 (bucket ends) + 8x24 (bucket photon boxes) per query "
+                                                  "inside the bounds, 16 per photon of a bucket the scan must "
+                                                  "read (its box within the query's final limit; slot-ordered "
+                                                  "position), 8 per result, 4 per count"},
                  "build_algorithmic_bytes": b_bytes},
         "cpu_grid": {"build_ms": cpu_build_s * 1e3, "mqueries_s": len(sample) / cpu_query_s / 1e6, "cores": 1,
                      "kind": "port", "sample": "oracle/hashgrid.c (serial restatement of HashGrid) building over "
@@ -1140,14 +1177,19 @@ def run_c3(args):
             bad += 1
     if bad:
         raise RuntimeError("%d of %d sampled merge queries differ from the oracle grid" % (bad, len(qs)))
-    cand = grid_candidates(info, np.ascontiguousarray(qall["pos"][valid]))
-    found = np.minimum(cand, vr.k)  # an upper bound on the list length; the exact one is below
-    ig, dg, cg = og.query_many(np.ascontiguousarray(qall["pos"][valid]), vr.k)
-    found = cg.astype(np.int64)
-    # fused query + merge, per valid query: 64 B query record, 8 x 8 B bucket ends, 16 B per
-    # photon visited (slot-ordered position), 32 B merge record per result, 12 B image add;
-    # 4 B per empty query slot (its pixel)
-    m_bytes = float(np.sum(64 + 8 * 8 + cand * 16 + found * 32 + 12) + (len(qall) - len(valid)) * 16)
+    # every valid query's list on the device (its length and k-th distance bound the scan)
+    qv = np.ascontiguousarray(qall["pos"][valid])
+    qb = N.DeviceBuffer.from_array(qv)
+    oi, od, oc = N.DeviceBuffer(len(qv) * 4 * vr.k), N.DeviceBuffer(len(qv) * 4 * vr.k), N.DeviceBuffer(len(qv) * 4)
+    vr.grid.query(qb, 12, 0, len(qv), vr.k, oi, oc, od)
+    found = oc.download(np.int32, len(qv)).astype(np.int64)
+    kd = od.download(np.float32, len(qv) * vr.k).reshape(-1, vr.k)[:, vr.k - 1]
+    r2 = np.float32(vr.radius) * np.float32(vr.radius)
+    cand, needed = grid_candidates(info, qv, np.where(found == vr.k, kd, r2))
+    # fused query + merge, per valid query: 64 B query record, 8 x 8 B bucket ends, 8 x 24 B
+    # bucket photon boxes, 16 B per photon of a bucket the scan must read (slot-ordered
+    # position), 32 B merge record per result, 12 B image add; 16 B per empty query slot
+    m_bytes = float(np.sum(64 + 8 * 8 + 8 * 24 + needed * 16 + found * 32 + 12) + (len(qall) - len(valid)) * 16)
     m_ms = kernel_ms["merge"]
     m_achieved = m_bytes / (m_ms * 1e-3) / 1e9
     workload = "c3-vcm-%dx%d-%d-light-paths-depth6-sphere-grid-%dtri-30pct-specular" % (
@@ -1167,6 +1209,7 @@ def run_c3(args):
         "kernel_ms_per_frame": kernel_ms,
         "frame_latency_ms": {"p50": float(np.percentile(frame_ms, 50)), "p99": float(np.percentile(frame_ms, 99))},
         "merge": {"queries": int(len(valid)), "ms_per_frame": m_ms, "photons_visited_per_query": float(cand.mean()) if len(cand) else 0.0,
+                  "photons_read_per_query_lower_bound": float(needed.mean()) if len(needed) else 0.0,
                   "results_per_query": float(found.mean()) if len(found) else 0.0,
                   "algorithmic_bytes_per_frame": m_bytes,
                   "parity": {"queries_checked": int(len(qs)), "mismatches": bad,
@@ -1180,9 +1223,10 @@ def run_c3(args):
                      "frac": m_achieved / HBM_PEAK_GBS, "kernel": "k_vcm_merge (fused grid query + merge)",
                      "traffic": measured_traffic(workload, "k_vcm_merge")[0],
                      "algorithmic_bytes_per_frame": m_bytes,
-                     "bytes_rule": "per merge query 64 (record) + 8 x 8 (bucket ends) + 16 per light vertex "
-                                   "visited (slot-ordered position) + 32 per result (slot-ordered merge record) "
-                                   "+ 12 (image); 16 per empty query slot"},
+                     "bytes_rule": "per merge query 64 (record) + 8 x 8 (bucket ends) + 8 x 24 (bucket photon "
+                                   "boxes) + 16 per light vertex of a bucket the scan must read (its box within "
+                                   "the query's final limit; slot-ordered position) + 32 per result (slot-ordered "
+                                   "merge record) + 12 (image); 16 per empty query slot"},
     }
     if not args.no_cpu:
         lr, ls = N.DeviceBuffer(vr.P * 32), N.DeviceBuffer(vr.P * 48)

@@ -208,18 +208,38 @@ __global__ __launch_bounds__(kBlock) void k_gather_pos(const unsigned char* ph, 
     pos4[s] = make_float4(p[0], p[1], p[2], __int_as_float(i));
 }
 
+// Bounds of every non-empty bucket's photons (the queries skip buckets that lie beyond their
+// limit; an empty bucket's entry is never read): the thread of a bucket's first slot in the
+// sorted keys walks the bucket's slots.
+__global__ __launch_bounds__(kBlock) void k_bucket_box(const int* keys_sorted, int n, const float4* pos4, float2* box) {
+    const int s0 = blockIdx.x * kBlock + threadIdx.x;
+    if (s0 >= n) return;
+    const int b = keys_sorted[s0];
+    if (s0 > 0 && keys_sorted[s0 - 1] == b) return;
+    float lx = FLT_MAX, ly = FLT_MAX, lz = FLT_MAX, hx = -FLT_MAX, hy = -FLT_MAX, hz = -FLT_MAX;
+    for (int s = s0; s < n && keys_sorted[s] == b; s++) {
+        const float4 p = pos4[s];
+        lx = fminf(lx, p.x); ly = fminf(ly, p.y); lz = fminf(lz, p.z);
+        hx = fmaxf(hx, p.x); hy = fmaxf(hy, p.y); hz = fmaxf(hz, p.z);
+    }
+    box[3 * (size_t)b] = make_float2(lx, ly);
+    box[3 * (size_t)b + 1] = make_float2(lz, hx);
+    box[3 * (size_t)b + 2] = make_float2(hy, hz);
+}
+
 // One query per wave (the scan and insertion of query_wave_list); the list goes to out_idx /
 // out_dist.
 __global__ __launch_bounds__(kBlock) void k_query_wave(const GridParams* gp, const int* ends, const float4* pos4,
-                                                       const unsigned char* qs, int qstride, int qoff, int nq, int k,
-                                                       int* out_idx, float* out_dist, int* out_count) {
+                                                       const float2* box, const unsigned char* qs, int qstride,
+                                                       int qoff, int nq, int k, int* out_idx, float* out_dist,
+                                                       int* out_count) {
     const int q = blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6);
     if (q >= nq) return;  // whole waves leave together
     const int lane = (int)(threadIdx.x & 63);
     const float* qp = photon_pos(qs, qstride, qoff, q);
     float my_dist;
     int my_idx;
-    const int count = query_wave_list<false>(*gp, ends, pos4, qp[0], qp[1], qp[2], k, my_dist, my_idx);
+    const int count = query_wave_list<false>(*gp, ends, pos4, box, qp[0], qp[1], qp[2], k, my_dist, my_idx);
     if (lane < count) {
         out_idx[(size_t)q * k + lane] = my_idx;
         if (out_dist) out_dist[(size_t)q * k + lane] = my_dist;
@@ -384,7 +404,7 @@ int imba_hashgrid_create(ImbaHashGrid** out) {
 int imba_hashgrid_release(ImbaHashGrid* g) {
     if (!g) return IMBA_OK;
     void* ptrs[] = {g->cell_ends, g->order, g->keys_a, g->vals_a, g->keys_b, g->vals_b, g->pos4, g->d_params,
-                    g->partial, g->scan_tmp, g->digits};
+                    g->partial, g->scan_tmp, g->digits, g->bucket_box};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete g;
@@ -406,6 +426,11 @@ int imba_hashgrid_build(ImbaHashGrid* g, const void* photons, int stride, int po
     if (table > g->table_cap) {
         int r = ensure(g->cell_ends, g->table_cap, table);
         if (r) return r;
+    }
+    if (table > g->box_cap) {  // 3 float2 per bucket
+        int c = 0, r = ensure(g->bucket_box, c, 3 * table);
+        if (r) return r;
+        g->box_cap = table;
     }
     g->table_size = table;
     if (count > g->cap) {
@@ -446,15 +471,16 @@ int imba_hashgrid_build(ImbaHashGrid* g, const void* photons, int stride, int po
                                      &keys_sorted, s);
         hipLaunchKernelGGL(k_mark_ends, dim3(ceil_div(count, kBlock)), dim3(kBlock), 0, s, keys_sorted, count,
                            g->cell_ends);
+        hipLaunchKernelGGL(k_gather_pos, dim3(ceil_div(count, kBlock)), dim3(kBlock), 0, s, ph, stride, pos_offset,
+                           count, g->order, g->pos4);
+        hipLaunchKernelGGL(k_bucket_box, dim3(ceil_div(count, kBlock)), dim3(kBlock), 0, s, keys_sorted, count, g->pos4,
+                           g->bucket_box);
     }
     if (table > 0) {
         hipLaunchKernelGGL(k_max_tiles, dim3(ntiles), dim3(kBlock), 0, s, g->cell_ends, table, g->scan_tmp);
         hipLaunchKernelGGL(k_max_top, dim3(1), dim3(1024), 0, s, g->scan_tmp, ntiles);
         hipLaunchKernelGGL(k_max_apply, dim3(ntiles), dim3(kBlock), 0, s, g->cell_ends, table, g->scan_tmp);
     }
-    if (count > 0)
-        hipLaunchKernelGGL(k_gather_pos, dim3(ceil_div(count, kBlock)), dim3(kBlock), 0, s, ph, stride, pos_offset,
-                           count, g->order, g->pos4);
     IMBA_HIP_TRY(hipGetLastError());
     if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
     return IMBA_OK;
@@ -476,7 +502,8 @@ int imba_hashgrid_query(const ImbaHashGrid* g, const void* queries, int query_st
     const unsigned char* qs = (const unsigned char*)queries;
     if (k <= 64)
         hipLaunchKernelGGL(k_query_wave, dim3(ceil_div(nq, kBlock / 64)), dim3(kBlock), 0, s, g->d_params,
-                           g->cell_ends, g->pos4, qs, query_stride, query_offset, nq, k, out_idx, out_dist, out_count);
+                           g->cell_ends, g->pos4, g->bucket_box, qs, query_stride, query_offset, nq, k, out_idx,
+                           out_dist, out_count);
     else
         hipLaunchKernelGGL(k_query_lds, dim3(nq), dim3(64), (size_t)k * 8, s, g->d_params, g->cell_ends, g->pos4, qs,
                            query_stride, query_offset, nq, k, out_idx, out_dist, out_count);

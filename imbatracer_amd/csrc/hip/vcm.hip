@@ -663,6 +663,7 @@ struct MergeArgs {
     const GridParams* gp;
     const int* ends;
     const float4* pos4;
+    const float2* box;
     const MergeRec* rec;
     const float4* queries;
     int capacity;
@@ -736,7 +737,7 @@ __global__ __launch_bounds__(256) void k_vcm_merge(Frame f, MergeArgs a) {
     if (__float_as_int(q0.w) < 0) return;
     float my_dist;
     int my_slot;
-    const int count = query_wave_list<true>(*a.gp, a.ends, a.pos4, q0.x, q0.y, q0.z, a.k, my_dist, my_slot);
+    const int count = query_wave_list<true>(*a.gp, a.ends, a.pos4, a.box, q0.x, q0.y, q0.z, a.k, my_dist, my_slot);
     if (count == 0) return;
     const float kd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_dist), a.k - 1));
     merge_epilogue(f, a, q, count, my_dist, my_slot, kd);
@@ -914,6 +915,7 @@ int imba_vcm_merge(const ImbaVcmFrame* frame, const ImbaHashGrid* grid, const vo
     a.gp = grid->d_params;
     a.ends = grid->cell_ends;
     a.pos4 = grid->pos4;
+    a.box = grid->bucket_box;
     a.rec = (const MergeRec*)records;
     a.queries = (const float4*)queries;
     a.capacity = capacity;

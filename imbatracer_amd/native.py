@@ -294,6 +294,27 @@ class DeviceBuffer:
             pass
 
 
+class PinnedArray:
+    """A numpy array over pinned host memory (hipHostMalloc): asynchronous copies into it do not
+    block the host."""
+
+    def __init__(self, count: int, dtype):
+        self.dtype = np.dtype(dtype)
+        self.ptr = C.c_void_p()
+        nbytes = max(1, count * self.dtype.itemsize)
+        check(lib.imba_host_alloc(C.byref(self.ptr), nbytes), "imba_host_alloc")
+        buf = (C.c_char * nbytes).from_address(self.ptr.value)
+        self.array = np.frombuffer(buf, dtype=self.dtype, count=count)
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ptr.value:
+                lib.imba_host_free(self.ptr)
+                self.ptr = C.c_void_p()
+        except Exception:
+            pass
+
+
 class Stream:
     def __init__(self):
         self.handle = C.c_void_p()

@@ -478,14 +478,14 @@ class PathTracer:
         (frames of several loops on their own streams then overlap on the device)."""
         self.graph.launch(self.stream)
         if getattr(self, "_counts_host", None) is None:
-            self._counts_host = np.zeros(self.max_path_len + 1, np.int32)
-        N.check(N.lib.imba_memcpy_d2h(N.ptr(self._counts_host), self.dev_counts.ptr, self._counts_host.nbytes,
+            self._counts_host = N.PinnedArray(self.max_path_len + 1, np.int32)  # the copy does not block
+        N.check(N.lib.imba_memcpy_d2h(self._counts_host.ptr, self.dev_counts.ptr, self._counts_host.array.nbytes,
                                       self.stream.handle), "counts")
 
     def finish_graph(self):
         """Waits for the frame launch_graph enqueued; returns its per-bounce ray counts."""
         self.stream.sync()
-        c = self._counts_host
+        c = self._counts_host.array
         out = []
         for v in c[:self.max_path_len]:
             if v == 0:
