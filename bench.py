@@ -385,6 +385,7 @@ def run_pt(args):
     # short last bounces overlap the next frame's first bounces on the device (progressive
     # rendering renders frame after frame, frontend/render_window.cpp:50-91)
     frame_ms, rays_total = [], 0
+    loops = []
     if args.mode == "graph":
         loops = [pt] + [W.PathTracer(scene, args.width, args.height, spp=1, max_path_len=8, rr_start=3, trace_cfg=cfg,
                                      shadow_cfg=pt.shadow_cfg, gpu_scene=pt.gpu_scene)
@@ -498,6 +499,13 @@ def run_pt(args):
                      "bytes_per_ray": bytes_total / max(rays_eager, 1),
                      "visits_per_ray": [float(x) / sum(counts) for x in np.sum(visits, 0)]},
     }
+    if not args.no_serving:
+        # the metric's second half: served vs plain per-request throughput and p99 latency on the
+        # serving mix (the 21M-triangle scene reused), and on small requests
+        loops.clear()  # the extra frame loops' queues are not needed any more
+        c5 = run_c5(args, big_scene=scene, saturated=False)
+        res["serving"] = {k: c5[k] for k in ("config", "served", "plain_per_request", "served_vs_plain",
+                                              "small_requests")}
     if not args.no_cpu:
         sample = sample_queues(queues, 200_000)
         res["cpu_baseline"] = cpu_baseline(scene.build, N.BUILDER_SBVH, threads, sample, [False] * len(sample),
@@ -782,12 +790,12 @@ def run_c4_loop(args, builds, scenes, queues, cfg, rays_first):
     }
 
 
-def run_c5(args):
+def run_c5(args, big_scene=None, saturated: bool = True):
     """Serving: Poisson arrivals from 16 sessions over 8 resident sphere-grid scenes (20K to 21M
     triangles), random cameras, resolutions {256^2, 640x480, 1280x720, 1920x1080}, 1-4 spp, PT or
     photon mapping (the PPM mode of VCM);
     the request loop with cross-session batching against the plain per-request path on the
-    same arrivals."""
+    same arrivals.  big_scene: the 32 x 32 sphere grid when already built."""
     from imbatracer_amd import native as N
     from imbatracer_amd import serve as V
     from imbatracer_amd import workloads as W
@@ -799,7 +807,8 @@ def run_c5(args):
     t0 = time.time()
     scenes, centres, tris = [], [], []
     for side, level in grids:
-        sg = W.SphereGridScene(side=side, level=level, threads=threads)
+        reuse = big_scene is not None and side == big_scene.side and level == 5
+        sg = big_scene if reuse else W.SphereGridScene(side=side, level=level, threads=threads)
         scenes.append(V.ServedScene(sg))
         centres.append(W.served_view(side))
         tris.append(sg.tri_count)
@@ -834,7 +843,7 @@ def run_c5(args):
     b, p = results["batched"], results["plain"]
     # capacity: the same requests all waiting at the start (the loop is never idle), and a mix
     # of small requests (256x256, 1 spp) that leave the GPU underfilled when run one at a time
-    sat = compare([V.Request(**{**r.__dict__, "arrival": 0.0}) for r in reqs], "saturated")
+    sat = compare([V.Request(**{**r.__dict__, "arrival": 0.0}) for r in reqs], "saturated") if saturated else None
     small = [V.Request(**{**r.__dict__, "width": 256, "height": 256, "spp": 1, "arrival": 0.0}) for r in reqs]
     sm = compare(small, "small")
 
@@ -856,7 +865,7 @@ def run_c5(args):
         "served": b, "plain_per_request": p,
         "served_vs_plain": {"throughput": b["mrays_s"] / p["mrays_s"], "p99": b["p99_ms"] / p["p99_ms"]},
         "saturated": {"served": sat["batched"], "plain_per_request": sat["plain"], "served_vs_plain": ratio(sat),
-                      "arrivals": "all %d requests waiting at the start" % len(reqs)},
+                      "arrivals": "all %d requests waiting at the start" % len(reqs)} if sat else None,
         "small_requests": {"served": sm["batched"], "plain_per_request": sm["plain"], "served_vs_plain": ratio(sm),
                            "arrivals": "the same %d requests at 256x256, 1 spp, all waiting at the start" % len(reqs)},
         "frame_latency_ms": {"p50": b["p50_ms"], "p99": b["p99_ms"]},
@@ -1278,6 +1287,7 @@ def main():
                     help="quantized tree: nodes of its top levels each block stages in LDS (0..256)")
     ap.add_argument("--no-autotune", action="store_true", help="use the library's default launch configuration")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serving", action="store_true", help="pt: skip the served-vs-plain comparison")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (reference point for HBM byte counters)")
