@@ -483,6 +483,7 @@ def run_pt(args):
                      "roofline": {"bound": "hbm", "achieved": shadow_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                   "frac": shadow_achieved / HBM_PEAK_GBS,
                                   "kernel": "k_trace (any hit) over the shadow slots",
+                                  "traffic": measured_traffic(workload, kernel_label(pt.shadow_cfg_of(0), any_hit=True))[0],
                                   "algorithmic_bytes_per_launch": sum(shadow_bytes) / len(shadow_bytes),
                                   "visits_per_slot": [float(x) / sum(counts) for x in np.sum(shadow_visits, 0)]}},
         "frame_traversal_bandwidth": {
@@ -1126,21 +1127,55 @@ def run_c3(args):
     log("light queues %s, camera queues %s, light vertices %d" % (light, cam, nv))
     for _ in range(args.warmup):
         vr.frame()
-    phases, frame_ms, rays_total = [], [], 0
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        f0 = time.perf_counter()
-        l, c, n = vr.frame(timed=True)
-        frame_ms.append((time.perf_counter() - f0) * 1e3)
-        if (l, c, n) != (light, cam, nv):
-            raise RuntimeError("frame queue sizes changed between frames")
-        rays_total += sum(l) + sum(c)
+    # phases of one iteration at a time
+    phases = []
+    for _ in range(3):
+        vr.frame(timed=True)
         phases.append(vr.phase_ms())
-    wall = time.perf_counter() - t_start
-    if vr.gpu_scene.error_count():
-        raise RuntimeError("traversal reported a malformed scene")
     phase = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
     log("phases (ms): %s" % {k: round(v, 3) for k, v in phase.items()})
+    # the measured iterations: `frames_in_flight` renderers (own buffers and streams, one scene)
+    # take turns, one's light pass overlapping the other's camera pass
+    loops = [vr] + [VC.VcmRenderer(scene, args.width, args.height, light_paths=args.c3_paths, max_path_len=6,
+                                   light_max_depth=6, rr_start=0, radius_factor=0.005, k=10, trace_cfg=cfg,
+                                   shadow_cfg=vr.shadow_cfg, gpu_scene=vr.gpu_scene)
+                    for _ in range(args.frames_in_flight - 1)]
+    for lp in loops[1:]:
+        for _ in range(max(1, args.warmup)):
+            lp.frame()
+    frame_ms, rays_total = [], 0
+    pending = {}
+
+    def done(lp):
+        l, c, n = lp.finish()
+        frame_ms.append((time.perf_counter() - pending.pop(id(lp))) * 1e3)
+        if (l, c, n) != (light, cam, nv):
+            raise RuntimeError("frame queue sizes changed between frames")
+        return sum(l) + sum(c)
+
+    t_start = time.perf_counter()
+    loops[0].launch_light()
+    pending[id(loops[0])] = time.perf_counter()
+    for step in range(args.steps):
+        lp, other = loops[step % len(loops)], loops[(step + 1) % len(loops)]
+        if step + 1 < args.steps and other is not lp:
+            if id(other) in pending:
+                rays_total += done(other)
+            other.launch_light()
+            pending[id(other)] = time.perf_counter()
+        lp.launch_camera()
+        if len(loops) == 1:
+            rays_total += done(lp)
+            if step + 1 < args.steps:
+                lp.launch_light()
+                pending[id(lp)] = time.perf_counter()
+    for lp in loops:
+        if id(lp) in pending:
+            rays_total += done(lp)
+    wall = time.perf_counter() - t_start
+    loops = loops[:1]
+    if vr.gpu_scene.error_count():
+        raise RuntimeError("traversal reported a malformed scene")
     # kernel time per kind of launch (events around every launch, separate frames)
     vr.time_kernels = True
     kms = []
@@ -1213,7 +1248,9 @@ def run_c3(args):
                    "radius": vr.radius, "light_vertices": nv,
                    "light_rays_per_bounce": light, "camera_rays_per_bounce": cam,
                    "trace_config": cfg_dict(cfg),
-                   "frame": "eager launches with device-side queue sizes; one host read (cache size) per frame"},
+                   "frame": "eager launches with device-side queue sizes; one host read (cache size) per "
+                            "iteration; %d iterations in flight on their own streams" % args.frames_in_flight,
+                   "frames_in_flight": args.frames_in_flight},
         "phase_ms": phase,
         "kernel_ms_per_frame": kernel_ms,
         "frame_latency_ms": {"p50": float(np.percentile(frame_ms, 50)), "p99": float(np.percentile(frame_ms, 99))},
@@ -1275,7 +1312,7 @@ def main():
     ap.add_argument("--persistent", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--frames-in-flight", type=int, default=2,
-                    help="pt graph mode: frame loops taking turns on their own streams")
+                    help="pt (graph mode) and c3: frame loops taking turns on their own streams")
     ap.add_argument("--mode", default="graph", choices=["graph", "eager"],
                     help="pt: frames as one captured graph (default) or as eager launches")
     ap.add_argument("--trace-kernel", default="plain", choices=["plain", "batch"],

@@ -31,8 +31,9 @@ struct ImbaHashGrid {
     int* keys_b = nullptr;
     int* vals_b = nullptr;
     float4* pos4 = nullptr;    // positions in slot order: x, y, z, photon index bits
-    float2* bucket_box = nullptr;  // per bucket: bounds of its photons (lo.xy, lo.z hi.x, hi.yz),
-                                   // written for the non-empty buckets of the last build
+    uint32_t* bucket_box = nullptr;  // per bucket: bounds of its photons (lo xyz, hi xyz) as
+                                     // ordered_bits, written for the non-empty buckets of the last
+                                     // build
     int box_cap = 0;
     imba::GridParams* d_params = nullptr;
     float* partial = nullptr;  // 6 per bounds block
@@ -51,13 +52,25 @@ __device__ __forceinline__ int bucket_of(uint32_t x, uint32_t y, uint32_t z, int
 
 constexpr int kDppWaveShr1 = 0x138;  // DPP control: lane i reads lane i - 1 of the wave
 
+// A float's bits mapped to an unsigned integer of the same order (negative floats reversed
+// below the positive ones), so that integer min / max are float min / max; and back.
+__device__ __forceinline__ uint32_t ordered_bits(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float from_ordered_bits(uint32_t e) {
+    return __uint_as_float((e & 0x80000000u) ? (e & 0x7FFFFFFFu) : ~e);
+}
+
 // Squared distance from (qx, qy, qz) to the box of a bucket's photons: a lower bound of every
 // photon's squared distance as the scan computes it (float subtraction, squaring and addition
 // are monotonic, so a photon inside the box is no nearer than the box).  Only meaningful for a
 // bucket that holds photons.
-__device__ __forceinline__ float box_dist2(const float2* box, int b, float qx, float qy, float qz) {
-    const float2 a = box[3 * (size_t)b], c = box[3 * (size_t)b + 1], e = box[3 * (size_t)b + 2];
-    const float lox = a.x, loy = a.y, loz = c.x, hix = c.y, hiy = e.x, hiz = e.y;
+__device__ __forceinline__ float box_dist2(const uint32_t* box, int b, float qx, float qy, float qz) {
+    const uint2* e = reinterpret_cast<const uint2*>(box + 6 * (size_t)b);
+    const uint2 e0 = e[0], e1 = e[1], e2 = e[2];
+    const float lox = from_ordered_bits(e0.x), loy = from_ordered_bits(e0.y), loz = from_ordered_bits(e1.x);
+    const float hix = from_ordered_bits(e1.y), hiy = from_ordered_bits(e2.x), hiz = from_ordered_bits(e2.y);
     const float gx = fmaxf(fmaxf(lox - qx, qx - hix), 0.0f);
     const float gy = fmaxf(fmaxf(loy - qy, qy - hiy), 0.0f);
     const float gz = fmaxf(fmaxf(loz - qz, qz - hiz), 0.0f);
@@ -77,7 +90,7 @@ __device__ __forceinline__ float box_dist2(const float2* box, int b, float qx, f
 // (wave-uniform); lanes >= count hold (0, -1).  Every lane of the wave must call it.
 template <bool SLOTS>
 __device__ __forceinline__ int query_wave_list(const GridParams& g, const int* ends, const float4* pos4,
-                                               const float2* box, float qx, float qy, float qz, int k,
+                                               const uint32_t* box, float qx, float qy, float qz, int k,
                                                float& my_dist, int& my_id) {
     const int lane = (int)(threadIdx.x & 63);
     my_dist = 0.0f;

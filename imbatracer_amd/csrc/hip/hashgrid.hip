@@ -209,28 +209,58 @@ __global__ __launch_bounds__(kBlock) void k_gather_pos(const unsigned char* ph, 
 }
 
 // Bounds of every non-empty bucket's photons (the queries skip buckets that lie beyond their
-// limit; an empty bucket's entry is never read): the thread of a bucket's first slot in the
-// sorted keys walks the bucket's slots.
-__global__ __launch_bounds__(kBlock) void k_bucket_box(const int* keys_sorted, int n, const float4* pos4, float2* box) {
-    const int s0 = blockIdx.x * kBlock + threadIdx.x;
-    if (s0 >= n) return;
-    const int b = keys_sorted[s0];
-    if (s0 > 0 && keys_sorted[s0 - 1] == b) return;
-    float lx = FLT_MAX, ly = FLT_MAX, lz = FLT_MAX, hx = -FLT_MAX, hy = -FLT_MAX, hz = -FLT_MAX;
-    for (int s = s0; s < n && keys_sorted[s] == b; s++) {
+// limit; an empty bucket's entry is never read), in the order-preserving integer form of
+// grid_query.h so that atomic min / max can combine them.  First the first slot of every bucket
+// run in the sorted keys resets its bucket's entry; then every wave takes 64 consecutive slots,
+// reduces the runs of equal keys within the wave (shift-down doubling: keys are sorted, so equal
+// keys are contiguous) and the first lane of each run combines the run's bounds into the bucket's
+// entry with six atomics.  Hot buckets thus cost one atomic per 64 photons.
+__global__ __launch_bounds__(kBlock) void k_bucket_box_reset(const int* keys_sorted, int n, uint32_t* box) {
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const int b = keys_sorted[s];
+    if (s > 0 && keys_sorted[s - 1] == b) return;
+    uint32_t* e = box + 6 * (size_t)b;
+    e[0] = e[1] = e[2] = 0xFFFFFFFFu;  // min over nothing
+    e[3] = e[4] = e[5] = 0u;           // max over nothing
+}
+
+__global__ __launch_bounds__(kBlock) void k_bucket_box(const int* keys_sorted, int n, const float4* pos4, uint32_t* box) {
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool valid = s < n;
+    const int key = valid ? keys_sorted[s] : -1;
+    uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
+    if (valid) {
         const float4 p = pos4[s];
-        lx = fminf(lx, p.x); ly = fminf(ly, p.y); lz = fminf(lz, p.z);
-        hx = fmaxf(hx, p.x); hy = fmaxf(hy, p.y); hz = fmaxf(hz, p.z);
+        lo[0] = hi[0] = ordered_bits(p.x);
+        lo[1] = hi[1] = ordered_bits(p.y);
+        lo[2] = hi[2] = ordered_bits(p.z);
     }
-    box[3 * (size_t)b] = make_float2(lx, ly);
-    box[3 * (size_t)b + 1] = make_float2(lz, hx);
-    box[3 * (size_t)b + 2] = make_float2(hy, hz);
+    for (int off = 1; off < 64; off <<= 1) {
+        const int k2 = __shfl_down(key, off, 64);
+        const bool take = lane + off < 64 && k2 == key;
+        for (int a = 0; a < 3; a++) {
+            const uint32_t l2 = __shfl_down(lo[a], off, 64), h2 = __shfl_down(hi[a], off, 64);
+            if (take) {
+                lo[a] = min(lo[a], l2);
+                hi[a] = max(hi[a], h2);
+            }
+        }
+    }
+    const int kprev = __shfl_up(key, 1, 64);
+    if (!valid || (lane > 0 && kprev == key)) return;  // not the first lane of its run
+    uint32_t* e = box + 6 * (size_t)key;
+    for (int a = 0; a < 3; a++) {
+        atomicMin(e + a, lo[a]);
+        atomicMax(e + 3 + a, hi[a]);
+    }
 }
 
 // One query per wave (the scan and insertion of query_wave_list); the list goes to out_idx /
 // out_dist.
 __global__ __launch_bounds__(kBlock) void k_query_wave(const GridParams* gp, const int* ends, const float4* pos4,
-                                                       const float2* box, const unsigned char* qs, int qstride,
+                                                       const uint32_t* box, const unsigned char* qs, int qstride,
                                                        int qoff, int nq, int k, int* out_idx, float* out_dist,
                                                        int* out_count) {
     const int q = blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6);
@@ -427,8 +457,8 @@ int imba_hashgrid_build(ImbaHashGrid* g, const void* photons, int stride, int po
         int r = ensure(g->cell_ends, g->table_cap, table);
         if (r) return r;
     }
-    if (table > g->box_cap) {  // 3 float2 per bucket
-        int c = 0, r = ensure(g->bucket_box, c, 3 * table);
+    if (table > g->box_cap) {  // 6 uint32 per bucket
+        int c = 0, r = ensure(g->bucket_box, c, 6 * table);
         if (r) return r;
         g->box_cap = table;
     }
@@ -473,6 +503,8 @@ int imba_hashgrid_build(ImbaHashGrid* g, const void* photons, int stride, int po
                            g->cell_ends);
         hipLaunchKernelGGL(k_gather_pos, dim3(ceil_div(count, kBlock)), dim3(kBlock), 0, s, ph, stride, pos_offset,
                            count, g->order, g->pos4);
+        hipLaunchKernelGGL(k_bucket_box_reset, dim3(ceil_div(count, kBlock)), dim3(kBlock), 0, s, keys_sorted, count,
+                           g->bucket_box);
         hipLaunchKernelGGL(k_bucket_box, dim3(ceil_div(count, kBlock)), dim3(kBlock), 0, s, keys_sorted, count, g->pos4,
                            g->bucket_box);
     }

@@ -382,6 +382,12 @@ int imba_event_record(void* event, void* stream) {
     IMBA_HIP_TRY(hipEventRecord((hipEvent_t)event, s));
     return IMBA_OK;
 }
+int imba_event_sync(void* event) {
+    IMBA_CHECK_ARG(event, "event is NULL");
+    IMBA_HIP_TRY(hipEventSynchronize((hipEvent_t)event));
+    return IMBA_OK;
+}
+
 int imba_event_elapsed_ms(float* ms, void* start, void* end) {
     IMBA_CHECK_ARG(ms && start && end, "NULL argument");
     IMBA_HIP_TRY(hipEventSynchronize((hipEvent_t)end));

@@ -663,7 +663,7 @@ struct MergeArgs {
     const GridParams* gp;
     const int* ends;
     const float4* pos4;
-    const float2* box;
+    const uint32_t* box;
     const MergeRec* rec;
     const float4* queries;
     int capacity;

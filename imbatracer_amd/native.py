@@ -146,6 +146,7 @@ _sig("imba_event_create", i32, C.POINTER(vp))
 _sig("imba_event_destroy", i32, vp)
 _sig("imba_event_record", i32, vp, vp)
 _sig("imba_event_elapsed_ms", i32, C.POINTER(f32), vp, vp)
+_sig("imba_event_sync", i32, vp)
 _sig("imba_build_create", i32, C.POINTER(vp))
 _sig("imba_build_release", i32, vp)
 _sig("imba_build_add_mesh", i32, vp, C.POINTER(MeshDesc), ip)
@@ -338,6 +339,9 @@ class Event:
 
     def record(self, stream=None):
         check(lib.imba_event_record(self.handle, stream.handle if isinstance(stream, Stream) else stream), "record")
+
+    def sync(self):
+        check(lib.imba_event_sync(self.handle), "imba_event_sync")
 
     def elapsed_ms(self, end: "Event") -> float:
         ms = C.c_float()

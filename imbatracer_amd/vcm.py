@@ -68,14 +68,16 @@ class VcmRenderer:
     def __init__(self, scene, width: int, height: int, light_paths: int, spp: int = 1, max_path_len: int = 6,
                  light_max_depth: int = 6, rr_start: int = 0, radius_factor: float = 0.005, k: int = 10,
                  num_connections: int = 1, seed_base: int = 0x1234567, light_seed: int = 0x7654321,
-                 trace_cfg: N.TraceConfig | None = None, shadow_cfg: N.TraceConfig | None = None):
+                 trace_cfg: N.TraceConfig | None = None, shadow_cfg: N.TraceConfig | None = None,
+                 gpu_scene: S.Scene | None = None):
+        """gpu_scene: an uploaded scene to share (another renderer over the same scene)."""
         from .scene import HashGrid
         from .workloads import PtShadingData
 
         if not 1 <= k <= 64:
             raise ValueError("the fused merge keeps its list in one wave: 1 <= k <= 64")
         self.s = scene
-        self.gpu_scene = S.Scene(scene.desc)
+        self.gpu_scene = gpu_scene if gpu_scene is not None else S.Scene(scene.desc)
         self.w, self.h, self.spp = width, height, spp
         self.n_cam = width * height * spp
         self.P = light_paths
@@ -140,6 +142,12 @@ class VcmRenderer:
         # (kernel_ms() sums them per kind)
         self.time_kernels = False
         self.kernel_events = []
+        # host side of the device counts (pinned: copies on the stream do not block the host)
+        self._pinned_counts0 = N.PinnedArray(self.n_counts, np.int32)
+        self._pinned_counts = N.PinnedArray(self.n_counts, np.int32)
+        self._ev_cache = N.Event()
+        self._marks = []
+        self._timed_frame = False
         self.clear_image()
 
     def _timed(self, kind: str, fn):
@@ -190,12 +198,13 @@ class VcmRenderer:
         N.check(N.lib.imba_memset(self.image.ptr, 0, self.w * self.h * 12, self.stream.handle), "memset")
 
     def _reset_counts(self):
-        c = np.zeros(self.n_counts, np.int32)
+        c = self._pinned_counts0.array
+        c[:] = 0
         c[0] = self.P
         c[self.L + 1] = self.n_cam
         c[self.L + self.Lc + 2] = self.slots
-        self._counts0 = c
-        self.counts.upload(c, self.stream)
+        N.check(N.lib.imba_memcpy_h2d(self.counts.ptr, self._pinned_counts0.ptr, 4 * self.n_counts,
+                                      self.stream.handle), "counts")
 
     def light_pass(self):
         st = self.stream.handle
@@ -239,11 +248,27 @@ class VcmRenderer:
             cur = 1 - cur
 
     def build_cache(self):
+        """The light vertex cache: the occupied slots compacted, the cache size read back (the
+        grid build sizes its table from it), the grid and the merge records."""
+        self.compact_cache()
+        self.read_cache_size()
+        self.build_grid()
+
+    def compact_cache(self):
         st = self.stream.handle
         N.check(N.lib.imba_compact_rays_dev(self.slot_a.ptr, self.slot_b.ptr, 32, self.slots, self._slot_cnt(),
                                             self.vtx_a.ptr, self.vtx_b.ptr, self._vtx_cnt(), self.workspace.ptr, st),
                 "compact vertices")
-        self.vertex_count = self._count_host(self.L + self.Lc + 3)
+        # the cache size into pinned memory (the copy does not block the host), and an event after it
+        N.check(N.lib.imba_memcpy_d2h(self._pinned_counts.ptr, self.counts.ptr, 4 * self.n_counts, st), "counts")
+        self._ev_cache.record(self.stream)
+
+    def read_cache_size(self):
+        self._ev_cache.sync()
+        self.vertex_count = int(self._pinned_counts.array[self.L + self.Lc + 3])
+
+    def build_grid(self):
+        st = self.stream.handle
         self.grid.build(self.vtx_a, 32, 0, self.vertex_count, self.radius, self.stream)
         N.check(N.lib.imba_vcm_merge_records(self.grid.h, self.vtx_a.ptr, self.vtx_b.ptr, self.records.ptr, st),
                 "vcm_merge_records")
@@ -298,32 +323,53 @@ class VcmRenderer:
                                                 self._cam_cnt(b + 1), self.workspace.ptr, st), "compact_rays_dev")
             cur = 1 - cur
 
-    def frame(self, timed: bool = False):
-        """One iteration; returns (light queue sizes, camera queue sizes, light vertex count).
-        With timed=True the phases are bracketed by events (phase_ms)."""
-        marks = []
+    def _mark(self, name):
+        if self._timed_frame:
+            e = N.Event()
+            e.record(self.stream)
+            self._marks.append((name, e))
 
-        def mark(name):
-            if timed:
-                e = N.Event()
-                e.record(self.stream)
-                marks.append((name, e))
-
+    def launch_light(self, timed: bool = False):
+        """First half of an iteration: the light pass and the light vertex cache, enqueued; the
+        cache size is copied to the host behind an event."""
+        self._timed_frame = timed
+        self._marks = []
         self.kernel_events = []
         self._reset_counts()
-        mark("start")
+        self._mark("start")
         self.light_pass()
-        mark("light")
-        self.build_cache()
-        mark("cache_grid")
+        self._mark("light")
+        self.compact_cache()
+
+    def launch_camera(self):
+        """Second half: waits for the cache size only, then enqueues the grid, the camera pass
+        and the read-back of every queue size."""
+        self.read_cache_size()
+        self.build_grid()
+        self._mark("cache_grid")
         self.camera_pass()
-        mark("camera")
-        c = self.counts.download(np.int32, self.n_counts, self.stream)
+        self._mark("camera")
+        N.check(N.lib.imba_memcpy_d2h(self._pinned_counts.ptr, self.counts.ptr, 4 * self.n_counts,
+                                      self.stream.handle), "counts")
+
+    def finish(self):
+        """Waits for the iteration; returns (light queue sizes, camera queue sizes, light vertex
+        count)."""
         self.stream.sync()
-        self.events = marks
+        self.events = self._marks
+        c = self._pinned_counts.array
         light = [int(v) for v in c[:self.L] if v > 0]
         cam = [int(v) for v in c[self.L + 1:self.L + 1 + self.Lc] if v > 0]
         return light, cam, self.vertex_count
+
+    def frame(self, timed: bool = False):
+        """One iteration; returns (light queue sizes, camera queue sizes, light vertex count).
+        With timed=True the phases are bracketed by events (phase_ms).  Iterations of several
+        renderers on their own streams overlap when their launch_light / launch_camera /
+        finish calls interleave."""
+        self.launch_light(timed)
+        self.launch_camera()
+        return self.finish()
 
     def phase_ms(self) -> dict:
         out = {}

@@ -109,6 +109,7 @@ int imba_event_create(void** event);
 int imba_event_destroy(void* event);
 int imba_event_record(void* event, void* stream);
 int imba_event_elapsed_ms(float* ms, void* start, void* end);
+int imba_event_sync(void* event);  /* waits until the work before the event's record is done */
 
 /* ------------------------------ host BVH construction ------------------------------------
  * Restatement of the reference builders and GPU layout writers:
