@@ -1274,6 +1274,27 @@ def run_c3(args):
                                    "the query's final limit; slot-ordered position) + 32 per result (slot-ordered "
                                    "merge record) + 12 (image); 16 per empty query slot"},
     }
+    if not args.no_c3_modes:
+        # the integrator's sub-algorithms on the same scene and launch shapes: one iteration at
+        # a time (eager, host waits for each), closest-hit rays per iteration time
+        del loops
+        modes = {}
+        for algo in ("bpt", "lt", "pt"):
+            mr = VC.VcmRenderer(scene, args.width, args.height, light_paths=args.c3_paths, max_path_len=6,
+                                light_max_depth=6, rr_start=0, radius_factor=0.005, k=10, trace_cfg=cfg,
+                                shadow_cfg=vr.shadow_cfg, gpu_scene=vr.gpu_scene, algo=algo)
+            mr.frame()
+            t = []
+            for _ in range(3):
+                t0m = time.perf_counter()
+                ml, mc, mv = mr.frame()
+                t.append(time.perf_counter() - t0m)
+            ms = float(np.median(t)) * 1e3
+            modes[algo] = {"ms_per_iteration": ms, "mrays_s": (sum(ml) + sum(mc)) / ms / 1e3,
+                           "light_rays": int(sum(ml)), "camera_rays": int(sum(mc)), "light_vertices": int(mv)}
+            log("%s iteration: %.2f ms, %.1f Mrays/s" % (algo.upper(), ms, modes[algo]["mrays_s"]))
+            del mr
+        res["sub_algorithms"] = modes
     if not args.no_cpu:
         lr, ls = N.DeviceBuffer(vr.P * 32), N.DeviceBuffer(vr.P * 48)
         N.check(N.lib.imba_vcm_light(C.byref(vr.frame_args), vr.light_seed, 0, vr.P, lr.ptr, ls.ptr, None),
@@ -1301,6 +1322,7 @@ def main():
                     help="c5: integrators drawn per request (pt: path tracing, ppm: photon mapping)")
     ap.add_argument("--c3-paths", type=int, default=921_600, help="c3: light paths per frame")
     ap.add_argument("--c3-check", type=int, default=20_000, help="c3: queries checked against the oracle grid")
+    ap.add_argument("--no-c3-modes", action="store_true", help="c3: skip the BPT / LT / PT iterations")
     ap.add_argument("--c4-bounces", type=int, default=8, help="c4: bounces of the graph-captured loop (0: off)")
     ap.add_argument("--c4-rays", type=int, default=0,
                     help="c4: rescale the 256 log-uniform queue sizes to this many rays (0: as drawn, about 37M)")

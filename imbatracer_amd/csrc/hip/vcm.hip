@@ -1,4 +1,4 @@
-// Vertex connection and merging (VCMIntegrator<ALGO_VCM>, render/integrators/vcm.cpp) on the
+// Vertex connection and merging (VCMIntegrator<algo>, render/integrators/vcm.cpp) on the
 // device, between traversals: every step the reference runs on the host around RayQueue for
 // one iteration, for the materials of the measured scenes (Lambertian, mirror, glass) and
 // triangle lights.  The partial MIS weights dVC / dVCM / dVM follow the VCM technical report
@@ -32,6 +32,10 @@
 //            hit with tmin = 1e-4 t.
 // Merging: the k nearest light vertices within the radius (HashGrid::query) are weighted with
 // the Epanechnikov kernel and the MIS weight 1 / (w_light + 1 + w_camera).
+// The sub-algorithms switch techniques off as the reference's template parameter does: BPT
+// (eta_vm = 0, no merging), LT (no stored vertices, camera connections with weight 1, no camera
+// paths) and the VCM integrator's PT (next event with weight 1, no connections or merging, no
+// light paths); which passes run is the caller's part.
 //
 // Materials: kind 0 Lambertian (albedo; render/materials/brdfs.h:8-20 inside the BSDF of
 // render/materials/bsdf.h:117-231), 1 mirror (perfect reflection with reflectance albedo),
@@ -52,6 +56,7 @@ namespace {
 
 constexpr float kGlassEta = 1.5f;
 enum { kDiffuse = 0, kMirror = 1, kGlass = 2 };
+enum { kAlgoVcm = IMBA_VCM_ALGO_VCM, kAlgoBpt = IMBA_VCM_ALGO_BPT, kAlgoLt = IMBA_VCM_ALGO_LT, kAlgoPt = IMBA_VCM_ALGO_PT };
 
 // VCMState (vcm.h:22-33): RayState + throughput, path_length, finite_light (always true for
 // triangle lights), dVC, dVCM, dVM.  48 bytes; pixel_id (light paths: the path index) first,
@@ -103,6 +108,7 @@ struct Frame {
     int light_max_depth;
     int rr_start;
     int num_connections;
+    int algo;
 };
 
 __device__ __forceinline__ Vec3 sub3(Vec3 a, Vec3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
@@ -368,10 +374,12 @@ __global__ __launch_bounds__(256) void k_vcm_light_bounce(Frame f, LightArgs a) 
                 s.dVM *= ic;
                 const int kind = sc.kinds[mat];
                 if (kind == kDiffuse) {
-                    a.vtx_a[2 * (size_t)i] = make_float4(pos.x, pos.y, pos.z, __int_as_float(s.path_length + 1));
-                    a.vtx_a[2 * (size_t)i + 1] = make_float4(out.x, out.y, out.z, __int_as_float(h.x));
-                    a.vtx_b[2 * (size_t)i] = make_float4(__int_as_float(s.pixel_id), s.tp[0], s.tp[1], s.tp[2]);
-                    a.vtx_b[2 * (size_t)i + 1] = make_float4(s.dVCM, s.dVM, s.dVC, __int_as_float(mat));
+                    if (f.algo != kAlgoLt) {
+                        a.vtx_a[2 * (size_t)i] = make_float4(pos.x, pos.y, pos.z, __int_as_float(s.path_length + 1));
+                        a.vtx_a[2 * (size_t)i + 1] = make_float4(out.x, out.y, out.z, __int_as_float(h.x));
+                        a.vtx_b[2 * (size_t)i] = make_float4(__int_as_float(s.pixel_id), s.tp[0], s.tp[1], s.tp[2]);
+                        a.vtx_b[2 * (size_t)i + 1] = make_float4(s.dVCM, s.dVM, s.dVC, __int_as_float(mat));
+                    }
                     // connect_to_camera (vcm.cpp:258-318)
                     const Vec3 cp = v3(f.cam_pos[0], f.cam_pos[1], f.cam_pos[2]);
                     const Vec3 cd = v3(f.cam_dir[0], f.cam_dir[1], f.cam_dir[2]);
@@ -397,7 +405,7 @@ __global__ __launch_bounds__(256) void k_vcm_light_bounce(Frame f, LightArgs a) 
                                 const float img_to_surf = (ipd * ipd * cos_surf) / (d2 * cos_cam * (cos_cam * cos_cam));
                                 const float wl = img_to_surf / (float)f.light_paths *
                                                  (f.eta_vm + s.dVCM + s.dVC * pdf_rev_w);
-                                const float mis = 1.0f / (wl + 1.0f);
+                                const float mis = f.algo == kAlgoLt ? 1.0f : 1.0f / (wl + 1.0f);
                                 const float alb[3] = {sc.albedo[3 * mat] * (1.0f / kPi),
                                                       sc.albedo[3 * mat + 1] * (1.0f / kPi),
                                                       sc.albedo[3 * mat + 2] * (1.0f / kPi)};
@@ -552,7 +560,7 @@ __global__ __launch_bounds__(256) void k_vcm_camera_bounce(Frame f, CameraArgs a
                         const float wl = pdf_dir_w * pick_inv / pdf_direct_w;
                         const float wc = pdf_emit_w * cos_i / (pdf_direct_w * cos_out) *
                                          (f.eta_vm + s.dVCM + s.dVC * pdf_rev_w);
-                        const float mis = 1.0f / (wc + 1.0f + wl);
+                        const float mis = f.algo == kAlgoPt ? 1.0f : 1.0f / (wc + 1.0f + wl);
                         ShadowState ss;
                         ss.pixel_id = s.pixel_id;
                         for (int c = 0; c < 3; c++) ss.tp[c] = s.tp[c] * mis * alb[c] * cos_i * rad[c] * pick_inv;
@@ -564,7 +572,7 @@ __global__ __launch_bounds__(256) void k_vcm_camera_bounce(Frame f, CameraArgs a
                     }
                 }
             }
-            if (kind == kDiffuse) {
+            if (kind == kDiffuse && (f.algo == kAlgoVcm || f.algo == kAlgoBpt)) {
                 // connect (vcm.cpp:490-570)
                 const int nv = a.vtx_count_dev ? *a.vtx_count_dev : 0;
                 const float vc_weight = (float)nv / ((float)f.light_paths * (float)C);
@@ -618,6 +626,8 @@ __global__ __launch_bounds__(256) void k_vcm_camera_bounce(Frame f, CameraArgs a
                     a.shadow_states[slot] = ss;
                     written |= 2 << c;
                 }
+            }
+            if (kind == kDiffuse && f.algo == kAlgoVcm) {
                 // vertex_merging (vcm.cpp:572-611): the query, answered after this launch
                 if (a.queries) {
                     a.queries[4 * (size_t)i] = make_float4(pos.x, pos.y, pos.z, __int_as_float(s.pixel_id));
@@ -753,6 +763,8 @@ int check_frame(const ImbaVcmFrame* p, Frame* f) {
     IMBA_CHECK_ARG(p->num_connections >= 1 && p->num_connections <= 8, "num_connections must be in [1, 8]");
     IMBA_CHECK_ARG(p->max_path_len >= 1 && p->light_max_depth >= 1, "path lengths must be >= 1");
     IMBA_CHECK_ARG(p->radius > 0.0f, "radius must be > 0");
+    IMBA_CHECK_ARG(p->algo == kAlgoVcm || p->algo == kAlgoBpt || p->algo == kAlgoLt || p->algo == kAlgoPt,
+                   "algo must be IMBA_VCM_ALGO_VCM, _BPT, _LT or _PT (PPM runs through imba_ppm_*)");
     Frame& F = *f;
     F.sc = Scene{p->tri_normals, p->tri_materials, p->tri_count, p->albedo, p->emission, p->emitter_area,
                  p->kinds, p->material_count, p->lights, p->light_count};
@@ -774,13 +786,14 @@ int check_frame(const ImbaVcmFrame* p, Frame* f) {
     const float kPiHost = 3.14159265359f;
     const float eta_vcm = kPiHost * (r * r) * (float)p->light_paths;
     F.eta_vc = 1.0f / eta_vcm;
-    F.eta_vm = eta_vcm;
+    F.eta_vm = p->algo == kAlgoBpt ? 0.0f : eta_vcm;  // mis_eta_vm_ (vcm.cpp:45)
     F.base_radius = r;
     F.radius_sqr = r * r;
     F.max_path_len = p->max_path_len;
     F.light_max_depth = p->light_max_depth;
     F.rr_start = p->rr_start;
     F.num_connections = p->num_connections;
+    F.algo = p->algo;
     return IMBA_OK;
 }
 

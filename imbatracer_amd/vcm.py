@@ -1,5 +1,5 @@
-"""One device-resident iteration of vertex connection and merging (VCMIntegrator<ALGO_VCM>::
-render, render/integrators/vcm.cpp:24-56) over a SphereGridScene:
+"""One device-resident iteration of vertex connection and merging (VCMIntegrator<algo>::render,
+render/integrators/vcm.cpp:24-56) over a SphereGridScene:
 
   light pass   light_paths paths from the triangle lights; per bounce: closest-hit traversal,
                imba_vcm_light_bounce (partial MIS weights, a light vertex per Lambertian hit into
@@ -14,6 +14,12 @@ render, render/integrators/vcm.cpp:24-56) over a SphereGridScene:
                next-event and connection shadow rays, merge queries, the bounce), any-hit
                traversal of the shadow rays and their contributions, the fused grid query +
                merge of the bounce's queries, compaction
+
+The sub-algorithms of the reference's template (vcm.h:37-43, 144-148) run the passes their
+techniques need (vcm.cpp:47-51, 98-99): "vcm" all of them; "bpt" (eta_vm = 0) the light pass,
+the cache and the camera pass without grid or merging; "lt" the light pass alone (no vertices
+stored, camera connections with weight 1); "pt" the camera pass alone (next event with weight
+1, no connections or merging).  PPM runs as workloads.PhotonMapper.
 
 Queue sizes stay on the device; the host reads one number per frame (the cache size the grid
 build needs).  Every launch is a HIP kernel of libimba_hip.so."""
@@ -50,7 +56,10 @@ class VcmFrame(C.Structure):
                 ("cam_pos", C.c_float * 3), ("cam_dir", C.c_float * 3), ("image_plane_dist", C.c_float),
                 ("width", C.c_int), ("height", C.c_int), ("spp", C.c_int), ("light_paths", C.c_int),
                 ("radius", C.c_float), ("max_path_len", C.c_int), ("light_max_depth", C.c_int),
-                ("rr_start", C.c_int), ("num_connections", C.c_int)]
+                ("rr_start", C.c_int), ("num_connections", C.c_int), ("algo", C.c_int)]
+
+
+ALGOS = {"vcm": 0, "bpt": 1, "lt": 3, "pt": 4}  # IMBA_VCM_ALGO_*
 
 
 def camera_frame(pos, d, up, fov, width, height):
@@ -69,13 +78,17 @@ class VcmRenderer:
                  light_max_depth: int = 6, rr_start: int = 0, radius_factor: float = 0.005, k: int = 10,
                  num_connections: int = 1, seed_base: int = 0x1234567, light_seed: int = 0x7654321,
                  trace_cfg: N.TraceConfig | None = None, shadow_cfg: N.TraceConfig | None = None,
-                 gpu_scene: S.Scene | None = None):
-        """gpu_scene: an uploaded scene to share (another renderer over the same scene)."""
+                 gpu_scene: S.Scene | None = None, algo: str = "vcm"):
+        """gpu_scene: an uploaded scene to share (another renderer over the same scene); algo:
+        one of ALGOS."""
         from .scene import HashGrid
         from .workloads import PtShadingData
 
         if not 1 <= k <= 64:
             raise ValueError("the fused merge keeps its list in one wave: 1 <= k <= 64")
+        if algo not in ALGOS:
+            raise ValueError("algo must be one of %s" % sorted(ALGOS))
+        self.algo = algo
         self.s = scene
         self.gpu_scene = gpu_scene if gpu_scene is not None else S.Scene(scene.desc)
         self.w, self.h, self.spp = width, height, spp
@@ -108,6 +121,7 @@ class VcmRenderer:
         f.width, f.height, f.spp, f.light_paths = width, height, spp, light_paths
         f.radius = self.radius
         f.max_path_len, f.light_max_depth, f.rr_start, f.num_connections = max_path_len, light_max_depth, rr_start, num_connections
+        f.algo = ALGOS[algo]
         self.frame_args = f
         cap = max(self.P, self.n_cam)
         self.cap = cap
@@ -200,8 +214,8 @@ class VcmRenderer:
     def _reset_counts(self):
         c = self._pinned_counts0.array
         c[:] = 0
-        c[0] = self.P
-        c[self.L + 1] = self.n_cam
+        c[0] = self.P if self.algo != "pt" else 0
+        c[self.L + 1] = self.n_cam if self.algo != "lt" else 0
         c[self.L + self.Lc + 2] = self.slots
         N.check(N.lib.imba_memcpy_h2d(self.counts.ptr, self._pinned_counts0.ptr, 4 * self.n_counts,
                                       self.stream.handle), "counts")
@@ -307,8 +321,10 @@ class VcmRenderer:
                                          self.image.ptr, st), "pt_shadow")
             if keep is not None:
                 keep["image_shadow"] = self.image.download(np.float32, self.w * self.h * 3, self.stream)
-            self._timed("merge", lambda: N.check(N.lib.imba_vcm_merge(
-                fa, self.grid.h, self.records.ptr, self.queries.ptr, n, cnt, self.k, self.image.ptr, st), "vcm_merge"))
+            if self.algo == "vcm":
+                self._timed("merge", lambda: N.check(N.lib.imba_vcm_merge(
+                    fa, self.grid.h, self.records.ptr, self.queries.ptr, n, cnt, self.k, self.image.ptr, st),
+                    "vcm_merge"))
             if keep is not None:
                 keep["image_merge"] = self.image.download(np.float32, self.w * self.h * 3, self.stream)
                 keep["out_rays"] = self.rays[cur].download(N.RAY_DTYPE, m, self.stream)
@@ -337,17 +353,22 @@ class VcmRenderer:
         self.kernel_events = []
         self._reset_counts()
         self._mark("start")
-        self.light_pass()
+        if self.algo != "pt":
+            self.light_pass()
         self._mark("light")
-        self.compact_cache()
+        if self.algo in ("vcm", "bpt"):
+            self.compact_cache()
 
     def launch_camera(self):
         """Second half: waits for the cache size only, then enqueues the grid, the camera pass
         and the read-back of every queue size."""
-        self.read_cache_size()
-        self.build_grid()
+        if self.algo in ("vcm", "bpt"):
+            self.read_cache_size()
+        if self.algo == "vcm":
+            self.build_grid()
         self._mark("cache_grid")
-        self.camera_pass()
+        if self.algo != "lt":
+            self.camera_pass()
         self._mark("camera")
         N.check(N.lib.imba_memcpy_d2h(self._pinned_counts.ptr, self.counts.ptr, 4 * self.n_counts,
                                       self.stream.handle), "counts")

@@ -587,7 +587,7 @@ int imba_ppm_merge(const void* queries, int nq, int k, const int* out_idx, const
                    void* stream);
 
 /* ------------------------------ vertex connection and merging ---------------------------
- * One iteration of VCMIntegrator<ALGO_VCM> (render/integrators/vcm.cpp) on the device, for the
+ * One iteration of VCMIntegrator<algo> (render/integrators/vcm.cpp) on the device, for the
  * materials of the measured scenes (kinds: 0 Lambertian, 1 mirror, 2 glass of index 1.5) and
  * triangle lights, with the reference's partial MIS weights (dVC, dVCM, dVM) and float
  * operation order.  States are IMBA_VCM_STATE_BYTES-byte records: int pixel_id (light paths:
@@ -625,8 +625,18 @@ int imba_ppm_merge(const void* queries, int nq, int k, const int* out_idx, const
  * imba_vcm_merge: vertex_merging (vcm.cpp:572-611) for every query slot: the k nearest light
  *   vertices within the radius (the grid's query, 1 <= k <= 64) and their Epanechnikov- and
  *   MIS-weighted throughputs into image, in one launch that reads the records by grid slot.
- * count_dev (nullable): the queue size read on the device when the launch runs. */
+ * count_dev (nullable): the queue size read on the device when the launch runs.
+ * algo (VCMSubAlgorithm, vcm.h:37-43) switches techniques as the reference's template does:
+ *   VCM  every technique below;
+ *   BPT  no merging: eta_vm = 0 in every partial weight (vcm.cpp:45), no merge queries (:427);
+ *        the caller builds the cache but no grid (:98-99);
+ *   LT   light paths only (:47-51): no vertex is stored (:238), the camera connections carry
+ *        MIS weight 1 (:301); the caller runs no camera pass;
+ *   PT   camera paths only: next event with MIS weight 1 (:473), no connections (:424), no
+ *        merging; emitter hits keep their MIS weight (:404); the caller runs no light pass.
+ * The PPM mode runs through imba_ppm_* (its camera paths sample only specular scattering). */
 enum { IMBA_VCM_STATE_BYTES = 48 };
+enum { IMBA_VCM_ALGO_VCM = 0, IMBA_VCM_ALGO_BPT = 1, IMBA_VCM_ALGO_LT = 3, IMBA_VCM_ALGO_PT = 4 };
 typedef struct ImbaVcmFrame {
     const float* tri_normals;   /* device, 3 per triangle (as ImbaPtShading)                 */
     const int* tri_materials;
@@ -650,6 +660,7 @@ typedef struct ImbaVcmFrame {
     int light_max_depth;
     int rr_start;               /* Russian roulette from this bounce on (0: the reference)   */
     int num_connections;        /* 1..8                                                      */
+    int algo;                   /* IMBA_VCM_ALGO_*                                           */
 } ImbaVcmFrame;
 
 int imba_vcm_light(const ImbaVcmFrame* frame, uint32_t seed_base, int first, int count, ImbaRay* rays, void* states,

@@ -1,6 +1,6 @@
 """Numpy restatement of one VCM iteration's surface steps — TEST INFRASTRUCTURE ONLY.
 
-VCMIntegrator<ALGO_VCM> (render/integrators/vcm.cpp) for the materials of the measured scenes
+VCMIntegrator<algo> (render/integrators/vcm.cpp) for the materials of the measured scenes
 (Lambertian, mirror, glass of index 1.5) and triangle lights, vectorized over a queue in
 float32 with the reference's operation order, so that everything but sin / cos rounds as the
 device does.  Each function takes one queue (rays, hits, states as the GPU read them back) and
@@ -15,6 +15,11 @@ returns what the step writes:
                  MIS weight, direct_illum (:445-488), connect (:490-570), the merge query
                  (vertex_merging :572-611 is restated in merge()), bounce (non-adjoint)
   merge          vertex_merging over given k-nearest lists
+
+Params.algo picks the sub-algorithm as the reference's template parameter does (vcm.h:37-43):
+"vcm"; "bpt" (mis_eta_vm 0, vcm.cpp:45; no merge queries, :427); "lt" (no stored vertices,
+:238; camera connections with weight 1, :301); "pt" (next event with weight 1, :473; no
+connections, :424, or merge queries).
 
 Only tests import this module.
 """
@@ -40,7 +45,7 @@ class Params:
     """Scene, camera and integrator settings of one iteration (the fields of ImbaVcmFrame)."""
 
     def __init__(self, scene, r2w, w2r, cam_pos, cam_dir, ipd, width, height, spp, light_paths, radius,
-                 max_path_len, light_max_depth, rr_start, num_connections):
+                 max_path_len, light_max_depth, rr_start, num_connections, algo: str = "vcm"):
         self.tri_normals = np.asarray(scene.tri_normals, F)
         self.tri_materials = np.asarray(scene.tri_materials, np.int32)
         self.albedo = np.asarray(scene.albedo, F)
@@ -54,7 +59,9 @@ class Params:
         self.P = light_paths
         r = F(radius)
         eta_vcm = PI * (r * r) * F(light_paths)  # render(): pi * sqr(pm_radius) * light_path_count
-        self.eta_vc, self.eta_vm = F(1) / eta_vcm, eta_vcm
+        self.algo = algo
+        self.eta_vc = F(1) / eta_vcm
+        self.eta_vm = F(0) if algo == "bpt" else eta_vcm
         self.radius, self.radius_sqr = r, r * r
         self.max_path_len, self.light_max_depth, self.rr_start, self.C = (max_path_len, light_max_depth, rr_start,
                                                                            num_connections)
@@ -359,14 +366,15 @@ def light_bounce(p: Params, rays, hits, states):
     va = np.zeros(n, VTX_A_DTYPE)
     vb = np.zeros(n, VTX_B_DTYPE)
     vb["key"] = -1
-    va["pos"][dif] = pos[dif]
-    va["path_length"][dif] = st["path_length"][dif] + 1
-    va["out"][dif] = out[dif]
-    va["tri"][dif] = tri[dif]
-    vb["key"][dif] = st["pixel"][dif]
-    vb["tp"][dif] = st["tp"][dif]
-    vb["dVCM"][dif], vb["dVM"][dif], vb["dVC"][dif] = st["dVCM"][dif], st["dVM"][dif], st["dVC"][dif]
-    vb["mat"][dif] = mat[dif]
+    sv = dif & (p.algo != "lt")  # light tracing stores no vertices
+    va["pos"][sv] = pos[sv]
+    va["path_length"][sv] = st["path_length"][sv] + 1
+    va["out"][sv] = out[sv]
+    va["tri"][sv] = tri[sv]
+    vb["key"][sv] = st["pixel"][sv]
+    vb["tp"][sv] = st["tp"][sv]
+    vb["dVCM"][sv], vb["dVM"][sv], vb["dVC"][sv] = st["dVCM"][sv], st["dVM"][sv], st["dVC"][sv]
+    vb["mat"][sv] = mat[sv]
     # connect_to_camera
     sh_rays, sh_st = _empty_shadow(n)
     cp = np.broadcast_to(p.cam_pos, pos.shape)
@@ -391,7 +399,7 @@ def light_bounce(p: Params, rays, hits, states):
         pdf_rev = np.abs(dot(nrm, out)) / PI
         img_to_surf = ((p.ipd * p.ipd) * cos_surf) / ((d2 * cos_cam) * (cos_cam * cos_cam))
         wl = (img_to_surf / F(p.P)) * ((p.eta_vm + st["dVCM"]) + st["dVC"] * pdf_rev)
-        mis = F(1) / (wl + F(1))
+        mis = F(1) / (wl + F(1)) if p.algo != "lt" else np.ones(n, F)
         alb = p.albedo[mat] * (F(1) / PI)
         tps = st["tp"] * (((mis[:, None] * alb) * img_to_surf[:, None]) / F(p.P))
         off = dist * F(1e-3)
@@ -475,7 +483,7 @@ def camera_bounce(p: Params, rays, hits, states, cache_a, cache_b):
         pick_inv = F(lc)
         wl = (pdf_dir * pick_inv) / pdf_direct_w
         wcn = ((pdf_emit_l * cos_i) / (pdf_direct_w * cos_out)) * ((p.eta_vm + st["dVCM"]) + st["dVC"] * pdf_rev)
-        misn = F(1) / ((wcn + F(1)) + wl)
+        misn = F(1) / ((wcn + F(1)) + wl) if p.algo != "pt" else np.ones(n, F)
         tpn = (((st["tp"] * misn[:, None]) * alb) * cos_i[:, None]) * rad * pick_inv
         offn = F(1e-3) * np.where(dist == FLT_MAX, F(1), dist)
     sl = rows[nee]
@@ -490,7 +498,7 @@ def camera_bounce(p: Params, rays, hits, states, cache_a, cache_b):
     nv = len(cache_a)
     vc_weight = F(nv) / (F(p.P) * F(C))
     for cc in range(C):
-        if nv <= 0:
+        if nv <= 0 or p.algo not in ("vcm", "bpt"):
             break
         vi = np.minimum(rng.integer(dif, 0, nv), nv - 1)
         A, B = cache_a[vi], cache_b[vi]
@@ -530,6 +538,8 @@ def camera_bounce(p: Params, rays, hits, states, cache_a, cache_b):
     q = np.zeros(n, QUERY_DTYPE)
     q["pos"] = np.float32(np.inf)
     q["pixel"] = -1
+    if p.algo != "vcm":
+        dif = dif & False
     q["pos"][dif] = pos[dif]
     q["pixel"][dif] = st["pixel"][dif]
     q["n"][dif] = nrm[dif]

@@ -1,11 +1,13 @@
-"""Vertex connection and merging (VCMIntegrator<ALGO_VCM>, render/integrators/vcm.cpp) on the
+"""Vertex connection and merging (VCMIntegrator<algo>, render/integrators/vcm.cpp) on the
 GPU over a small sphere grid with mirror and glass spheres.
 
   * the light vertex cache is the stable compaction of every light bounce's vertex slots;
   * the merge records are the cache's entries in the grid's slot order;
   * every camera bounce's merging (the fused grid query + merge) equals vertex_merging
     (vcm.cpp:572-611) restated in numpy over the oracle grid's k-nearest lists, within 1e-5;
-  * the frame's image is finite, non-negative and lit, and a second frame repeats the first.
+  * the frame's image is finite, non-negative and lit, and a second frame repeats the first;
+  * every light and camera bounce is restated by oracle/vcm.py on the GPU's own queues, for
+    VCM and for the sub-algorithms BPT, LT and the VCM integrator's PT.
 """
 import numpy as np
 import pytest
@@ -21,14 +23,27 @@ F = np.float32
 
 
 @pytest.fixture(scope="module")
-def vr(gpu):
-    scene = W.SphereGridScene(side=4, level=2, builder=N.BUILDER_SBVH, specular_fraction=0.3)
+def scene(gpu):
+    return W.SphereGridScene(side=4, level=2, builder=N.BUILDER_SBVH, specular_fraction=0.3)
+
+
+def _render(scene, algo):
     r = V.VcmRenderer(scene, 48, 32, light_paths=48 * 32, spp=1, max_path_len=6, light_max_depth=6, rr_start=0,
-                      radius_factor=0.01, k=10)
+                      radius_factor=0.01, k=10, algo=algo)
     r.keep_queues = True
-    print("VCM frame 48x32, %d triangles" % scene.tri_count, flush=True)
+    print("%s frame 48x32, %d triangles" % (algo.upper(), scene.tri_count), flush=True)
     r.counts_frame = r.frame()
     return r
+
+
+@pytest.fixture(scope="module")
+def vr(scene):
+    return _render(scene, "vcm")
+
+
+@pytest.fixture(scope="module", params=["bpt", "lt", "pt"])
+def mode(scene, request):
+    return _render(scene, request.param)
 
 
 def test_frame_runs(vr):
@@ -137,7 +152,7 @@ def _params(vr):
 
     f = vr.frame_args
     return OV.Params(vr.s, vr.r2w, vr.w2r, vr.cam_pos, vr.fwd, vr.ipd, vr.w, vr.h, vr.spp, vr.P, vr.radius,
-                     f.max_path_len, f.light_max_depth, f.rr_start, f.num_connections)
+                     f.max_path_len, f.light_max_depth, f.rr_start, f.num_connections, algo=vr.algo)
 
 
 def _bits(a):
@@ -182,6 +197,10 @@ def test_light_start_restated(vr):
 
 
 def test_light_bounces_restated(vr):
+    _light_bounces_restated(vr)
+
+
+def _light_bounces_restated(vr):
     from oracle import vcm as OV
 
     p = _params(vr)
@@ -204,6 +223,7 @@ def test_light_bounces_restated(vr):
         assert bad == 0
         assert np.array_equal(k["shadow_hits"]["tri_id"][occ] < 0, want["tri_id"] < 0), b
     assert conns > 100
+    return len(vr.kept["light"])
 
 
 def test_camera_start_restated(vr):
@@ -220,6 +240,11 @@ def test_camera_start_restated(vr):
 
 
 def test_camera_bounces_restated(vr):
+    counts = _camera_bounces_restated(vr)
+    assert all(v > 0 for v in counts.values()), counts
+
+
+def _camera_bounces_restated(vr):
     from oracle import vcm as OV
 
     p = _params(vr)
@@ -256,7 +281,7 @@ def test_camera_bounces_restated(vr):
         np.add.at(want, ss[vis], w["shadow_states"]["tp"][vis].astype(np.float64))
         got = (k["image_shadow"].astype(np.float64) - k["image_emit"]).reshape(-1, 3)
         assert np.allclose(got, want, rtol=1e-4, atol=1e-5 * max(1e-30, np.abs(want).max(initial=0))), b
-    assert all(v > 0 for v in counts.values()), counts
+    return counts
 
 
 def test_frame_repeats(vr):
@@ -270,3 +295,33 @@ def test_frame_repeats(vr):
     img2 = vr.image_host()
     on = img > 1e-6
     assert np.mean(np.abs(img2[on] - img[on]) / img[on]) <= 1e-4
+
+
+# ---- the sub-algorithms: the same steps with techniques switched off ----
+
+def test_mode_frame(mode):
+    light, cam, nv = mode.counts_frame
+    assert (len(light) == 0) == (mode.algo == "pt")
+    assert (len(cam) == 0) == (mode.algo == "lt")
+    assert (nv > 0) == (mode.algo == "bpt")  # the cache is built for connections only
+    img = mode.image_host()
+    assert np.all(np.isfinite(img)) and np.all(img >= 0)
+    assert (img.sum(-1) > 0).mean() > 0.3
+
+
+def test_mode_light_bounces_restated(mode):
+    if mode.algo == "pt":
+        assert mode.kept["light"] == []
+        return
+    assert _light_bounces_restated(mode) > 0
+    if mode.algo == "lt":  # light tracing stores no vertices
+        assert all(np.all(k["vtx_b"]["key"] == -1) for k in mode.kept["light"])
+
+
+def test_mode_camera_bounces_restated(mode):
+    if mode.algo == "lt":
+        assert mode.kept["camera"] == []
+        return
+    counts = _camera_bounces_restated(mode)
+    assert counts["nee"] > 0 and counts["emit"] > 0 and counts["query"] == 0
+    assert (counts["conn"] > 0) == (mode.algo == "bpt"), counts
