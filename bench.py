@@ -1268,6 +1268,7 @@ def run_c3(args):
         "roofline": {"bound": "hbm", "achieved": m_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": m_achieved / HBM_PEAK_GBS, "kernel": "k_vcm_merge (fused grid query + merge)",
                      "traffic": measured_traffic(workload, "k_vcm_merge")[0],
+                     "traffic_per": "iteration: the six camera bounces' merge launches together, as achieved",
                      "algorithmic_bytes_per_frame": m_bytes,
                      "bytes_rule": "per merge query 64 (record) + 8 x 8 (bucket ends) + 8 x 24 (bucket photon "
                                    "boxes) + 16 per light vertex of a bucket the scan must read (its box within "
