@@ -794,7 +794,7 @@ def run_c4_loop(args, builds, scenes, queues, cfg, rays_first):
 def run_c5(args, big_scene=None, saturated: bool = True):
     """Serving: Poisson arrivals from 16 sessions over 8 resident sphere-grid scenes (20K to 21M
     triangles), random cameras, resolutions {256^2, 640x480, 1280x720, 1920x1080}, 1-4 spp, PT or
-    photon mapping (the PPM mode of VCM);
+    VCM (photon mapping, the PPM mode of VCM, on request);
     the request loop with cross-session batching against the plain per-request path on the
     same arrivals.  big_scene: the 32 x 32 sphere grid when already built."""
     from imbatracer_amd import native as N
@@ -826,7 +826,8 @@ def run_c5(args, big_scene=None, saturated: bool = True):
             server = V.Server(scenes, arena, max_active=args.c5_max_active, batched=batched, cfg=cfg,
                               max_rays_in_flight=args.c5_max_rays)
             # warm-up on a few requests (kernels loaded, batch tables sized)
-            server.run(V.poisson_requests(4, 1000.0, 1, len(scenes), centres, seed=99, ceiling=W.SERVED_CEILING))
+            server.run(V.poisson_requests(6, 1000.0, 1, len(scenes), centres, seed=99, integrators=integrators,
+                                          ceiling=W.SERVED_CEILING))
             server = V.Server(scenes, arena, max_active=args.c5_max_active, batched=batched, cfg=cfg,
                               max_rays_in_flight=args.c5_max_rays)
             t = time.perf_counter()
@@ -949,7 +950,7 @@ def run_c3_ppm(args):
     log("scene: %d triangles, built in %.1f s" % (scene.tri_count, time.time() - t0))
     cfg = fixed_config(args)
     pm = W.PhotonMapper(scene, args.width, args.height, light_paths=args.c3_paths, max_depth=6, rr_start=3,
-                        radius_factor=0.005, k=10, trace_cfg=cfg)
+                        radius_factor=0.005, k=10, trace_cfg=cfg, merge="fused")
     if cfg is None and not args.no_autotune:
         lr, ls = N.DeviceBuffer(pm.P * 32), N.DeviceBuffer(pm.P * 32)
         N.check(N.lib.imba_ppm_light(N.ptr(pm.light_tri), N.ptr(pm.light_int), pm.light_seed, 0, pm.P, lr.ptr,
@@ -1001,10 +1002,12 @@ def run_c3_ppm(args):
     log("phases (ms): %s" % {k: round(v, 3) for k, v in phase.items()})
 
     # parity sample: the grid over the GPU's photons and a sample of the queries, against the oracle
+    # (the frame's fused merge keeps its lists in the waves: the index query writes them, untimed)
     ph_rays, _ = pm.photons_host()
     pos = np.ascontiguousarray(ph_rays["pos"])
     q = pm.queries_host()
     qpos = np.ascontiguousarray(q["pos"])
+    pm.query()
     idx, dist, cnt = pm.results_host()
     og = oracle.Grid()
     tb = time.perf_counter()
@@ -1030,15 +1033,21 @@ def run_c3_ppm(args):
     r2 = np.float32(pm.radius) * np.float32(pm.radius)
     lim = np.where(found == pm.k, dist[:, pm.k - 1], r2)
     cand, needed = grid_candidates(info, qpos, lim)
-    q_bytes = float(np.sum(np.where(np.isfinite(qpos[:, 0]), 12 + 8 * 8 + 8 * 24, 12) + needed * 16 + found * 8 + 4))
-    q_ms = phase["query"]
+    # fused query + merge: 16 B of every query record, 8 x 8 B bucket ends and 8 x 24 B bucket
+    # photon boxes per query inside the grid bounds, 16 B per photon of a bucket the scan must
+    # read (slot-ordered position), and for a query with results the rest of its record (48 B),
+    # 32 B per result (slot-ordered direction and state) and 12 B of image
+    inside = np.isfinite(qpos[:, 0]) & (q["pixel"] >= 0)
+    q_bytes = float(np.sum(16 + np.where(inside, 8 * 8 + 8 * 24, 0) + needed * 16 +
+                           np.where(found > 0, 48 + 12, 0) + found * 32))
+    q_ms = phase["merge"]
     achieved = q_bytes / (q_ms * 1e-3) / 1e9
     n_q = pm.n_cam
     # build: bounds 12 B, count 12 + 4 B, scan 2 x 4 B per bucket, scatter 16 + 4 + 4 B per photon
     b_bytes = float(n_ph * 12 + n_ph * 16 + 2 * info.table_size * 4 + n_ph * 24)
     workload = "c3-ppm-%dx%d-%d-light-paths-depth6-sphere-grid-%dtri-30pct-specular" % (
         args.width, args.height, pm.P, scene.tri_count)
-    q_traffic, q_traffic_src = measured_traffic(workload, "k_query")
+    q_traffic, q_traffic_src = measured_traffic(workload, "k_merge_grid")
     res = {
         "metric": METRIC, "value": rays_total / wall / 1e6, "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
@@ -1048,12 +1057,13 @@ def run_c3_ppm(args):
                    "max_depth": 6, "rr_start": 3, "k": pm.k, "radius": pm.radius, "photons": n_ph,
                    "light_rays_per_bounce": light, "camera_rays_per_bounce": cam,
                    "trace_config": cfg_dict(cfg),
-                   "frame": "eager launches with device-side queue sizes; one host read (photon count) per frame"},
+                   "frame": "eager launches with device-side queue sizes; one host read (photon count) per frame; "
+                            "grid query and merge fused in one launch"},
         "phase_ms": phase,
         "roofline": None,
         "frame_latency_ms": {"p50": float(np.percentile(frame_ms, 50)), "p99": float(np.percentile(frame_ms, 99))},
         "grid": {"photons": n_ph, "table_size": info.table_size, "build_ms": phase["grid_build"],
-                 "queries": n_q, "query_ms": q_ms, "mqueries_s": n_q / (q_ms * 1e-3) / 1e6,
+                 "queries": n_q, "query_merge_ms": q_ms, "mqueries_s": n_q / (q_ms * 1e-3) / 1e6,
                  "photons_visited_per_query": float(cand.mean()),
                  "photons_read_per_query_lower_bound": float(needed.mean()),
                  "results_per_query": float(found.mean()),
@@ -1061,13 +1071,16 @@ def run_c3_ppm(args):
                             "rule": "photon index lists and squared distances bit-exact against the oracle grid "
                                     "built over the same photons"},
                  "query_roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                    "frac": achieved / HBM_PEAK_GBS, "kernel": "k_query",
+                                    "frac": achieved / HBM_PEAK_GBS,
+                                    "kernel": "k_merge_grid (fused grid query + merge)",
                                     "traffic": q_traffic, "traffic_source": q_traffic_src,
                                     "algorithmic_bytes_per_launch": q_bytes,
-                                    "bytes_rule": "12 + 8x8 (bucket ends) + 8x24 (bucket photon boxes) per query "
-                                                  "inside the bounds, 16 per photon of a bucket the scan must "
-                                                  "read (its box within the query's final limit; slot-ordered "
-                                                  "position), 8 per result, 4 per count"},
+                                    "bytes_rule": "16 per query record; 8x8 (bucket ends) + 8x24 (bucket photon "
+                                                  "boxes) per query inside the bounds; 16 per photon of a bucket "
+                                                  "the scan must read (its box within the query's final limit; "
+                                                  "slot-ordered position); per query with results 48 (rest of "
+                                                  "its record) + 12 (image) + 32 per result (slot-ordered "
+                                                  "direction and state)"},
                  "build_algorithmic_bytes": b_bytes},
         "cpu_grid": {"build_ms": cpu_build_s * 1e3, "mqueries_s": len(sample) / cpu_query_s / 1e6, "cores": 1,
                      "kind": "port", "sample": "oracle/hashgrid.c (serial restatement of HashGrid) building over "
@@ -1319,8 +1332,9 @@ def main():
     ap.add_argument("--c5-max-active", type=int, default=64, help="c5: requests in flight at once (batched loop)")
     ap.add_argument("--c5-max-rays", type=int, default=32 << 20,
                     help="c5: first-traversal rays of the requests in flight at once (batched loop; 0 = no limit)")
-    ap.add_argument("--c5-integrators", default="pt,ppm",
-                    help="c5: integrators drawn per request (pt: path tracing, ppm: photon mapping)")
+    ap.add_argument("--c5-integrators", default="pt,vcm",
+                    help="c5: integrators drawn per request (pt: path tracing, vcm: vertex connection and merging, "
+                         "ppm: photon mapping)")
     ap.add_argument("--c3-paths", type=int, default=921_600, help="c3: light paths per frame")
     ap.add_argument("--c3-check", type=int, default=20_000, help="c3: queries checked against the oracle grid")
     ap.add_argument("--no-c3-modes", action="store_true", help="c3: skip the BPT / LT / PT iterations")

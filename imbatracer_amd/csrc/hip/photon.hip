@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include "common.h"
+#include "grid_query.h"
 #include "sampling.h"
 
 namespace imba {
@@ -303,6 +304,77 @@ __global__ __launch_bounds__(256) void k_merge(MergeArgs a) {
     atomicAdd(&a.image[3 * (size_t)pixel + 2], qw.z * sb * scale);
 }
 
+// Merge records in grid slot order: the direction of photon order[s] and its state (weight
+// in y, z, w), so that the fused merge reads them next to the positions the scan just read.
+__global__ __launch_bounds__(256) void k_merge_records(const float4* photon_rays, const float4* photon_states,
+                                                       const int* order, int n, float4* rec) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= n) return;
+    const int v = order[s];
+    rec[2 * (size_t)s] = photon_rays[2 * (size_t)v + 1];
+    rec[2 * (size_t)s + 1] = photon_states[v];
+}
+
+struct GridMergeArgs {
+    const GridParams* gp;
+    const int* ends;
+    const float4* pos4;
+    const uint32_t* box;
+    const float4* rec;
+    const float4* queries;
+    int nq, k;
+    float radius_sqr;
+    float inv_paths;
+    float* image;
+};
+
+// The grid query and k_merge's sum in one launch, one query per wave: the wave's k-capped list
+// keeps each photon's grid slot, lane j weighs list entry j, and the wave adds the lanes'
+// terms in list order (k_merge's loop order).
+__global__ __launch_bounds__(256) void k_merge_grid(GridMergeArgs a) {
+    const int q = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (q >= a.nq) return;  // whole waves leave together
+    const float4 q0 = a.queries[4 * (size_t)q];
+    const int pixel = __float_as_int(q0.w);
+    if (pixel < 0) return;
+    float my_dist;
+    int my_slot;
+    const int c = query_wave_list<true>(*a.gp, a.ends, a.pos4, a.box, q0.x, q0.y, q0.z, a.k, my_dist, my_slot);
+    if (c <= 0) return;
+    const float kd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_dist), a.k - 1));
+    const float rs = c == a.k ? kd : a.radius_sqr;
+    if (!(rs > 0.0f)) return;
+    const float4 qn = a.queries[4 * (size_t)q + 1];
+    const float4 qo = a.queries[4 * (size_t)q + 2];
+    const float4 qw = a.queries[4 * (size_t)q + 3];
+    const float cos_c = qn.x * qo.x + qn.y * qo.y + qn.z * qo.z;
+    const int lane = (int)(threadIdx.x & 63);
+    float cr = 0.0f, cg = 0.0f, cb = 0.0f;
+    if (lane < c) {
+        const float4 pd = a.rec[2 * (size_t)my_slot];
+        const float cos_p = qn.x * pd.x + qn.y * pd.y + qn.z * pd.z;
+        if (cos_p * cos_c > 0.0f) {
+            const float kern = 1.0f - my_dist / rs;
+            const float4 ps = a.rec[2 * (size_t)my_slot + 1];
+            cr = kern * ps.y;
+            cg = kern * ps.z;
+            cb = kern * ps.w;
+        }
+    }
+    float sr = 0.0f, sg = 0.0f, sb = 0.0f;
+    for (int j = 0; j < c; j++) {
+        sr += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cr), j));
+        sg += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cg), j));
+        sb += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cb), j));
+    }
+    if (lane == 0) {
+        const float scale = 2.0f / (kPi * rs) * a.inv_paths;
+        atomicAdd(&a.image[3 * (size_t)pixel + 0], qw.x * sr * scale);
+        atomicAdd(&a.image[3 * (size_t)pixel + 1], qw.y * sg * scale);
+        atomicAdd(&a.image[3 * (size_t)pixel + 2], qw.z * sb * scale);
+    }
+}
+
 int launch_surface(bool camera, SurfArgs& a, void* stream) {
     if (a.count == 0) return IMBA_OK;
     bool sync;
@@ -428,6 +500,47 @@ int imba_ppm_merge(const void* queries, int nq, int k, const int* out_idx, const
     bool sync;
     hipStream_t s = resolve_stream(stream, &sync);
     hipLaunchKernelGGL(k_merge, dim3(ceil_div(nq, 256)), dim3(256), 0, s, a);
+    IMBA_HIP_TRY(hipGetLastError());
+    if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
+    return IMBA_OK;
+}
+
+int imba_ppm_merge_records(const ImbaHashGrid* grid, const void* photon_rays, const void* photon_states,
+                           void* records, void* stream) {
+    IMBA_CHECK_ARG(grid && photon_rays && photon_states && records, "NULL argument");
+    const int n = grid->count;
+    if (n == 0) return IMBA_OK;
+    bool sync;
+    hipStream_t s = resolve_stream(stream, &sync);
+    hipLaunchKernelGGL(k_merge_records, dim3(ceil_div(n, 256)), dim3(256), 0, s, (const float4*)photon_rays,
+                       (const float4*)photon_states, grid->order, n, (float4*)records);
+    IMBA_HIP_TRY(hipGetLastError());
+    if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
+    return IMBA_OK;
+}
+
+int imba_ppm_merge_grid(const ImbaHashGrid* grid, const void* records, const void* queries, int nq, int k,
+                        float radius, int light_paths, float* image, void* stream) {
+    IMBA_CHECK_ARG(grid, "grid is NULL");
+    IMBA_CHECK_ARG(nq >= 0 && light_paths > 0, "bad sizes");
+    IMBA_CHECK_ARG(k >= 1 && k <= 64, "the fused merge keeps its list in one wave: 1 <= k <= 64");
+    if (nq == 0 || grid->count == 0) return IMBA_OK;
+    IMBA_CHECK_ARG(records && queries && image, "NULL argument");
+    GridMergeArgs a;
+    a.gp = grid->d_params;
+    a.ends = grid->cell_ends;
+    a.pos4 = grid->pos4;
+    a.box = grid->bucket_box;
+    a.rec = (const float4*)records;
+    a.queries = (const float4*)queries;
+    a.nq = nq;
+    a.k = k;
+    a.radius_sqr = radius * radius;
+    a.inv_paths = 1.0f / (float)light_paths;
+    a.image = image;
+    bool sync;
+    hipStream_t s = resolve_stream(stream, &sync);
+    hipLaunchKernelGGL(k_merge_grid, dim3(ceil_div(nq, 4)), dim3(256), 0, s, a);
     IMBA_HIP_TRY(hipGetLastError());
     if (sync) IMBA_HIP_TRY(hipStreamSynchronize(s));
     return IMBA_OK;

@@ -236,6 +236,8 @@ _sig("imba_ppm_light_bounce", i32, vp, vp, vp, i32, vp, vp, vp, i32, vp, vp, i32
 _sig("imba_ppm_camera", i32, vp, vp, i32, i32, i32, C.c_uint32, i32, i32, vp, vp, vp, vp)
 _sig("imba_ppm_camera_bounce", i32, vp, vp, vp, i32, vp, vp, vp, i32, vp, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp)
 _sig("imba_ppm_merge", i32, vp, i32, i32, vp, vp, vp, vp, vp, f32, i32, vp, vp)
+_sig("imba_ppm_merge_records", i32, vp, vp, vp, vp, vp)
+_sig("imba_ppm_merge_grid", i32, vp, vp, vp, i32, i32, f32, i32, vp, vp)
 _sig("intersect_gpu_masked_instanced", None, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32)
 _sig("occluded_gpu_masked_instanced", None, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32)
 

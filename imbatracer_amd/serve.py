@@ -14,6 +14,8 @@ of the loop:
     the queue sizes kept on the device, then the shadow rays' any-hit traversal (batched the
     same way) and their contributions (imba_shade_batch_shadow), and one read-back of all the
     requests' queue sizes ends the step.
+Photon-mapping (PPM) and vertex connection and merging (VCM) requests advance through their
+phases one per step; their light and camera queues join the step's batched traversal.
 A request completes when its queue is empty or its paths reach the maximum length; its latency
 is completion minus arrival.  The plain per-request path (`batched=False`) runs the same
 kernels for one request at a time, in arrival order.
@@ -32,6 +34,7 @@ import numpy as np
 
 from . import native as N
 from . import scene as S
+from . import vcm as VC
 from .workloads import PPM_STATE_BYTES, PT_STATE_BYTES, SHADOW_STATE_BYTES, PtShadingData
 
 ALIGN = 256
@@ -106,7 +109,7 @@ class Request:
     cam_dir: tuple
     seed: int
     arrival: float  # seconds after the start of the run
-    integrator: str = "pt"  # "pt" or "ppm" (photon mapping: the PPM mode of VCM)
+    integrator: str = "pt"  # "pt", "ppm" (photon mapping: the PPM mode of VCM) or "vcm"
     # filled in by the server
     admitted: float = -1.0
     done: float = -1.0
@@ -120,11 +123,15 @@ class Request:
 
 PPM_DEPTH = 6   # light and camera path hits of a photon-mapping request
 PPM_K = 10      # photons per merge query
+VCM_DEPTH = 6   # light path hits and camera path vertices of a VCM request
+VCM_K = 10      # light vertices per merge query
+VCM_C = 1       # vertex connections per camera vertex
+VCM_RR_START = 0  # Russian roulette from the first bounce, as the reference
 
 
 def first_rays(req: Request) -> int:
     """Rays of a request's first traversal: its camera samples (PT), its light paths plus its
-    camera samples (PPM: the light pass, then the camera pass)."""
+    camera samples (PPM, VCM: the light pass, then the camera pass)."""
     if req.integrator == "pt":
         return req.samples
     return req.width * req.height + req.samples
@@ -134,7 +141,9 @@ def footprint(req: Request) -> int:
     """Device bytes a request holds while active.  PT: two ray and state buffers, hits, the
     shadow slots (ray, state, hit), image and the compaction workspace.  PPM: two ray and state
     buffers and hits for max(light paths, samples) entries, the photon slots and the compacted
-    photons, the merge queries and their results, image and workspace."""
+    photons, the merge queries and their results, image and workspace.  VCM: two ray and state
+    buffers, hits and the shadow slots for max(light paths, samples) entries, the light vertex
+    slots, the cache and its merge records, the merge queries, image and workspace."""
     n = req.samples
     if req.integrator == "pt":
         ws = int(N.lib.imba_compact_workspace_size(n))
@@ -142,6 +151,11 @@ def footprint(req: Request) -> int:
                 12 * req.width * req.height + ws + 12 * ALIGN)
     p = req.width * req.height
     cap = max(p, n)
+    if req.integrator == "vcm":
+        slots = VCM_DEPTH * p
+        ws = int(N.lib.imba_compact_workspace_size(max(slots, cap)))
+        return (2 * 32 * cap + 2 * VC.VCM_STATE_DTYPE.itemsize * cap + 16 * cap + (1 + VCM_C) * cap * (32 + 16 + 16) +
+                (4 * 32 + 32) * slots + 64 * cap + 12 * req.width * req.height + ws + 24 * ALIGN)
     slots = PPM_DEPTH * p
     ws = int(N.lib.imba_compact_workspace_size(max(slots, cap)))
     return (2 * 32 * cap + 2 * PPM_STATE_BYTES * cap + 16 * cap + 2 * 48 * slots + 64 * n + 8 * PPM_K * n + 4 * n +
@@ -322,6 +336,160 @@ class _PPMJob:
         return False
 
 
+class _VCMJob:
+    """A VCM request (one iteration of VCMIntegrator<ALGO_VCM>, vcm.cpp, as
+    imbatracer_amd.vcm.VcmRenderer renders it: width x height light paths, VCM_DEPTH hits and
+    vertices, one connection and a k = VCM_K merge per camera vertex) as phases advanced one per
+    step: `light` (VCM_DEPTH bounces: light vertices into slots, camera connections traced and
+    added), `cache` (slots compacted; the cache size is read back), `grid` (hash grid and merge
+    records built, camera rays shot), `camera` (bounces with next event, connections and the
+    fused merge) until every path has ended.  The light and camera queues join the step's
+    batched traversal; the shadow rays of a bounce are traversed in the request's own launch."""
+
+    def __init__(self, req: Request, slot: int, block, scene: ServedScene, arena: DeviceArena):
+        self.req, self.slot, self.block, self.sc = req, slot, block, scene
+        self.P = req.width * req.height
+        self.n = req.samples
+        self.cap = max(self.P, self.n)
+        self.L = VCM_DEPTH
+        off = block[0]
+
+        def take(nbytes):
+            nonlocal off
+            p = arena.ptr(off)
+            off += (nbytes + ALIGN - 1) // ALIGN * ALIGN
+            return p
+
+        cap, slots = self.cap, self.L * self.P
+        sb = VC.VCM_STATE_DTYPE.itemsize
+        self.rays = [take(32 * cap), take(32 * cap)]
+        self.states = [take(sb * cap), take(sb * cap)]
+        self.hits = take(16 * cap)
+        ns = (1 + VCM_C) * cap
+        self.shadow_rays, self.shadow_states, self.shadow_hits = take(32 * ns), take(16 * ns), take(16 * ns)
+        self.slot_a, self.slot_b = take(32 * slots), take(32 * slots)
+        self.vtx_a, self.vtx_b = take(32 * slots), take(32 * slots)
+        self.records = take(32 * slots)
+        self.queries = take(64 * cap)
+        self.image = take(12 * req.width * req.height)
+        self.workspace = take(int(N.lib.imba_compact_workspace_size(max(slots, cap))))
+        self.grid = None  # taken from the server's pool when the cache size is known
+        self.phase = "light"
+        self.bounce = 0
+        self.cur = 0
+        self.queued = self.P
+        self.vertices = 0
+        r2w, w2r, fwd, ipd = VC.camera_frame(req.cam_pos, req.cam_dir, (0.0, 1.0, 0.0), 60.0, req.width, req.height)
+        self.frame = VC.make_frame(scene.shading, scene.kinds, (np.asarray(req.cam_pos, np.float32), r2w, w2r, fwd, ipd),
+                                   req.width, req.height, req.spp, self.P, scene.radius, VCM_DEPTH, VCM_DEPTH,
+                                   VCM_RR_START, VCM_C, "vcm")
+
+    # count slots of the request's row: light bounce b, camera bounce b, vertex slots, cache
+    # size, shadow slots of the camera bounce
+    def _light(self, srv, b):
+        return srv.counts_ptr(self.slot, b)
+
+    def _camera(self, srv, b):
+        return srv.counts_ptr(self.slot, srv.L1 + b)
+
+    def _cache(self, srv):
+        return srv.counts_ptr(self.slot, 2 * srv.L1 + 1)
+
+    def _shadow(self, srv):
+        return srv.counts_ptr(self.slot, 2 * srv.L1 + 2)
+
+    def start(self, srv, st):
+        c0 = np.zeros(srv.cs, np.int32)
+        c0[0] = self.P
+        c0[srv.L1] = self.n
+        c0[2 * srv.L1] = self.L * self.P
+        self.c0 = c0  # kept until the copy has run (the step ends with a stream synchronisation)
+        N.check(N.lib.imba_memcpy_h2d(srv.counts_ptr(self.slot, 0), N.ptr(c0), c0.nbytes, st), "counts")
+        N.check(N.lib.imba_memset(self.image, 0, 12 * self.req.width * self.req.height, st), "image")
+        N.check(N.lib.imba_vcm_light(C.byref(self.frame), (self.req.seed ^ 0x5A5A5A5A) & 0xFFFFFFFF, 0, self.P,
+                                     self.rays[0], self.states[0], st), "vcm_light")
+
+    def tracing(self) -> bool:
+        return self.phase in ("light", "camera")
+
+    def trace_item(self, srv):
+        cnt = self._light(srv, self.bounce) if self.phase == "light" else self._camera(srv, self.bounce)
+        cap = self.P if self.phase == "light" else self.n
+        return (self.sc.gpu, self.rays[self.cur], self.hits, cap, cnt, False)
+
+    def before_trace(self, srv, st):
+        if self.phase != "grid":
+            return
+        self.grid = srv.take_grid()
+        if self.vertices > 0:
+            N.check(N.lib.imba_hashgrid_build(self.grid.h, self.vtx_a, 32, 0, self.vertices, self.sc.radius, st),
+                    "grid build")
+            N.check(N.lib.imba_vcm_merge_records(self.grid.h, self.vtx_a, self.vtx_b, self.records, st),
+                    "vcm_merge_records")
+        N.check(N.lib.imba_vcm_camera(C.byref(self.frame), self.req.seed, 0, self.n, self.rays[0], self.states[0], st),
+                "vcm_camera")
+        self.phase, self.bounce, self.cur = "camera", 0, 0
+
+    def after_trace(self, srv, st):
+        sc, b, cur, fa = self.sc, self.bounce, self.cur, C.byref(self.frame)
+        cfg = C.byref(srv.cfg) if srv.cfg else None
+        sb = VC.VCM_STATE_DTYPE.itemsize
+        if self.phase == "light":
+            cnt = self._light(srv, b)
+            N.check(N.lib.imba_vcm_light_bounce(fa, self.rays[cur], self.hits, self.states[cur], self.P, cnt,
+                                                C.c_void_p(self.slot_a.value + 32 * b * self.P),
+                                                C.c_void_p(self.slot_b.value + 32 * b * self.P), self.rays[cur],
+                                                self.shadow_rays, self.shadow_states, st), "vcm_light_bounce")
+            N.check(N.lib.imba_occluded_dev(sc.gpu.h, self.shadow_rays, self.shadow_hits, self.P, cnt, cfg, st),
+                    "occluded_dev")
+            N.check(N.lib.imba_pt_shadow(self.shadow_hits, self.shadow_states, self.P, cnt, self.image, st),
+                    "pt_shadow")
+            N.check(N.lib.imba_compact_rays_dev(self.rays[cur], self.states[cur], sb, self.P, cnt, self.rays[1 - cur],
+                                                self.states[1 - cur], self._light(srv, b + 1), self.workspace, st),
+                    "compact")
+            self.bounce, self.cur = b + 1, 1 - cur
+            if self.bounce == self.L:
+                N.check(N.lib.imba_compact_rays_dev(self.slot_a, self.slot_b, 32, self.L * self.P,
+                                                    srv.counts_ptr(self.slot, 2 * srv.L1), self.vtx_a, self.vtx_b,
+                                                    self._cache(srv), self.workspace, st), "compact vertices")
+                self.phase = "cache"
+        elif self.phase == "camera":
+            cnt = self._camera(srv, b)
+            ns = (1 + VCM_C) * self.n
+            N.check(N.lib.imba_vcm_camera_bounce(fa, self.rays[cur], self.hits, self.states[cur], self.n, cnt,
+                                                 self.vtx_a, self.vtx_b, self._cache(srv), self.rays[cur],
+                                                 self.shadow_rays, self.shadow_states, self._shadow(srv),
+                                                 self.queries, self.image, st), "vcm_camera_bounce")
+            N.check(N.lib.imba_occluded_dev(sc.gpu.h, self.shadow_rays, self.shadow_hits, ns, self._shadow(srv), cfg,
+                                            st), "occluded_dev")
+            N.check(N.lib.imba_pt_shadow(self.shadow_hits, self.shadow_states, ns, self._shadow(srv), self.image, st),
+                    "pt_shadow")
+            if self.vertices > 0:
+                N.check(N.lib.imba_vcm_merge(fa, self.grid.h, self.records, self.queries, self.n, cnt, VCM_K,
+                                             self.image, st), "vcm_merge")
+            N.check(N.lib.imba_compact_rays_dev(self.rays[cur], self.states[cur], sb, self.n, cnt, self.rays[1 - cur],
+                                                self.states[1 - cur], self._camera(srv, b + 1), self.workspace, st),
+                    "compact")
+            self.bounce, self.cur = b + 1, 1 - cur
+
+    def update(self, srv, row) -> bool:
+        """Advances the phase from the read-back counts; True when the request is complete."""
+        if self.phase == "light":
+            self.queued = int(row[self.bounce])
+        elif self.phase == "cache":
+            self.vertices = int(row[2 * srv.L1 + 1])
+            self.phase = "grid"
+        elif self.phase == "camera":
+            self.queued = int(row[srv.L1 + self.bounce])
+            if self.queued == 0 or self.bounce >= self.L:
+                light = [int(x) for x in row[:self.L]]
+                cam = [int(x) for x in row[srv.L1:srv.L1 + self.L]]
+                self.req.counts = light + cam
+                self.req.rays = int(sum(light) + sum(cam))
+                return True
+        return False
+
+
 class Server:
     # A queue at least this long fills the GPU by itself: it gets its own k_trace launch in a
     # batched step, and the shorter queues share one batched launch.
@@ -343,10 +511,11 @@ class Server:
         self.max_path_len, self.rr_start = max_path_len, rr_start
         self.cfg = cfg
         self.stream = N.Stream()
-        # per request slot: PT bounce counts [0, L]; PPM light counts [0, L], camera counts
-        # [L1, L1 + L], photon slots at 2 L1 and photons at 2 L1 + 1
-        self.L1 = max(max_path_len, PPM_DEPTH) + 1
-        self.cs = 2 * self.L1 + 2
+        # per request slot: PT bounce counts [0, L]; PPM / VCM light counts [0, L], camera
+        # counts [L1, L1 + L], photon (vertex) slots at 2 L1, photons (cache size) at 2 L1 + 1,
+        # VCM camera shadow slots at 2 L1 + 2
+        self.L1 = max(max_path_len, PPM_DEPTH, VCM_DEPTH) + 1
+        self.cs = 2 * self.L1 + 3
         self.counts = N.DeviceBuffer(4 * self.cs * self.max_active)
         self.free_slots = list(range(self.max_active))
         self.pending: list[Request] = []
@@ -402,8 +571,9 @@ class Server:
             self.pending.pop(0)
             req.admitted = self.now()
             self.rays_in_flight += first_rays(req)
-            if req.integrator == "ppm":
-                job = _PPMJob(req, self.free_slots.pop(0), block, self.scenes[req.scene], self.arena)
+            if req.integrator in ("ppm", "vcm"):
+                cls = _PPMJob if req.integrator == "ppm" else _VCMJob
+                job = cls(req, self.free_slots.pop(0), block, self.scenes[req.scene], self.arena)
                 job.start(self, st)
                 self.active.append(job)
                 continue
@@ -429,7 +599,7 @@ class Server:
             return
         st = self.stream.handle
         pt_jobs = [j for j in self.active if isinstance(j, _Job)]
-        ppm_jobs = [j for j in self.active if isinstance(j, _PPMJob)]
+        ppm_jobs = [j for j in self.active if isinstance(j, (_PPMJob, _VCMJob))]  # phased requests
         for j in ppm_jobs:
             j.before_trace(self, st)
         items = [(j.sc.gpu, j.rays[j.cur], j.hits, j.n, self.counts_ptr(j.slot, j.bounce), False, j.queued)
@@ -487,7 +657,7 @@ class Server:
         still = []
         for j in self.active:
             c = allc[j.slot]
-            if isinstance(j, _PPMJob):
+            if isinstance(j, (_PPMJob, _VCMJob)):
                 finished = j.update(self, c)
             else:
                 j.queued = int(c[j.bounce])
@@ -501,7 +671,7 @@ class Server:
                 self.done.append(j.req)
                 self.arena.release(*j.block)
                 self.free_slots.append(j.slot)
-                if isinstance(j, _PPMJob) and j.grid is not None:
+                if isinstance(j, (_PPMJob, _VCMJob)) and j.grid is not None:
                     self.grids.append(j.grid)
                     j.grid = None
             else:

@@ -73,14 +73,38 @@ def camera_frame(pos, d, up, fov, width, height):
     return r2w, w2r, fwd, np.float32(ipd.value)
 
 
+def make_frame(sh, kinds, cam, width: int, height: int, spp: int, light_paths: int, radius: float,
+               max_path_len: int, light_max_depth: int, rr_start: int, num_connections: int, algo: str) -> VcmFrame:
+    """ImbaVcmFrame over a scene's shading data (workloads.PtShadingData), its material kinds (a
+    device buffer) and a camera (pos, raster_to_world16, world_to_raster16, forward3,
+    image_plane_dist from camera_frame)."""
+    pos, r2w, w2r, fwd, ipd = cam
+    f = VcmFrame()
+    f.tri_normals, f.tri_materials, f.tri_count = sh.tri_n.ptr.value, sh.tri_m.ptr.value, sh.tri_count
+    f.albedo, f.emission, f.emitter_area = sh.albedo.ptr.value, sh.emission.ptr.value, sh.emitter_area.ptr.value
+    f.kinds, f.material_count = kinds.ptr.value, sh.material_count
+    f.lights, f.light_count = sh.lights.ptr.value, sh.light_count
+    f.raster_to_world[:] = [float(x) for x in r2w]
+    f.world_to_raster[:] = [float(x) for x in w2r]
+    f.cam_pos[:] = [float(x) for x in pos]
+    f.cam_dir[:] = [float(x) for x in fwd]
+    f.image_plane_dist = float(ipd)
+    f.width, f.height, f.spp, f.light_paths = width, height, spp, light_paths
+    f.radius = radius
+    f.max_path_len, f.light_max_depth, f.rr_start, f.num_connections = (max_path_len, light_max_depth, rr_start,
+                                                                         num_connections)
+    f.algo = ALGOS[algo]
+    return f
+
+
 class VcmRenderer:
     def __init__(self, scene, width: int, height: int, light_paths: int, spp: int = 1, max_path_len: int = 6,
                  light_max_depth: int = 6, rr_start: int = 0, radius_factor: float = 0.005, k: int = 10,
                  num_connections: int = 1, seed_base: int = 0x1234567, light_seed: int = 0x7654321,
                  trace_cfg: N.TraceConfig | None = None, shadow_cfg: N.TraceConfig | None = None,
-                 gpu_scene: S.Scene | None = None, algo: str = "vcm"):
+                 gpu_scene: S.Scene | None = None, algo: str = "vcm", camera=None):
         """gpu_scene: an uploaded scene to share (another renderer over the same scene); algo:
-        one of ALGOS."""
+        one of ALGOS; camera: (position, direction, up, fov in degrees), else the scene's."""
         from .scene import HashGrid
         from .workloads import PtShadingData
 
@@ -102,27 +126,15 @@ class VcmRenderer:
         self.cfg, self.shadow_cfg = trace_cfg, shadow_cfg
         lo, hi = np.asarray(scene.bounds[0], np.float64), np.asarray(scene.bounds[1], np.float64)
         self.radius = float(np.float32(radius_factor * np.linalg.norm(hi - lo)))
-        pos, d, up, fov = scene.camera
+        pos, d, up, fov = camera if camera is not None else scene.camera
         self.r2w, self.w2r, self.fwd, self.ipd = camera_frame(pos, d, up, fov, width, height)
         self.cam_pos = np.asarray(pos, np.float32)
         self.sh = PtShadingData(scene.tri_normals, scene.tri_materials, scene.albedo, scene.emission,
                                 scene.emitter_area, scene.lights)
         self.kinds = N.DeviceBuffer.from_array(np.ascontiguousarray(scene.kinds, np.int32))
-        f = VcmFrame()
-        f.tri_normals, f.tri_materials, f.tri_count = self.sh.tri_n.ptr.value, self.sh.tri_m.ptr.value, self.sh.tri_count
-        f.albedo, f.emission, f.emitter_area = self.sh.albedo.ptr.value, self.sh.emission.ptr.value, self.sh.emitter_area.ptr.value
-        f.kinds, f.material_count = self.kinds.ptr.value, self.sh.material_count
-        f.lights, f.light_count = self.sh.lights.ptr.value, self.sh.light_count
-        f.raster_to_world[:] = [float(x) for x in self.r2w]
-        f.world_to_raster[:] = [float(x) for x in self.w2r]
-        f.cam_pos[:] = [float(x) for x in self.cam_pos]
-        f.cam_dir[:] = [float(x) for x in self.fwd]
-        f.image_plane_dist = float(self.ipd)
-        f.width, f.height, f.spp, f.light_paths = width, height, spp, light_paths
-        f.radius = self.radius
-        f.max_path_len, f.light_max_depth, f.rr_start, f.num_connections = max_path_len, light_max_depth, rr_start, num_connections
-        f.algo = ALGOS[algo]
-        self.frame_args = f
+        self.frame_args = make_frame(self.sh, self.kinds, (self.cam_pos, self.r2w, self.w2r, self.fwd, self.ipd),
+                                     width, height, spp, light_paths, self.radius, max_path_len, light_max_depth,
+                                     rr_start, num_connections, algo)
         cap = max(self.P, self.n_cam)
         self.cap = cap
         self.rays = [N.DeviceBuffer(cap * 32), N.DeviceBuffer(cap * 32)]

@@ -514,14 +514,25 @@ class PhotonMapper:
       query        the k nearest photons within the radius of each query (HashGrid::query)
       merge        vertex_merging into the image
 
+    merge="fused" replaces the last two steps by one launch that keeps each query's list in its
+    wave and reads the photons' directions and states from records gathered into grid slot order
+    when the grid is built (imba_ppm_merge_records / imba_ppm_merge_grid); query() still writes
+    the index lists on request.
+
     Every traversal, producer and compaction runs on the device with queue sizes kept on the
     device; the host reads one number per frame (the photon count the grid build needs)."""
 
     def __init__(self, scene: SphereGridScene, width: int, height: int, light_paths: int, max_depth: int = 6,
                  rr_start: int = 3, radius_factor: float = 0.005, k: int = 10, spp: int = 1,
-                 seed_base: int = 0x1234567, light_seed: int = 0x7654321, trace_cfg: N.TraceConfig | None = None):
+                 seed_base: int = 0x1234567, light_seed: int = 0x7654321, trace_cfg: N.TraceConfig | None = None,
+                 merge: str = "lists"):
         from .scene import HashGrid
 
+        if merge not in ("lists", "fused"):
+            raise ValueError("merge must be 'lists' or 'fused'")
+        if merge == "fused" and not 1 <= k <= 64:
+            raise ValueError("the fused merge keeps its list in one wave: 1 <= k <= 64")
+        self.merge_mode = merge
         self.s = scene
         self.gpu_scene = S.Scene(scene.desc)
         self.w, self.h, self.spp = width, height, spp
@@ -551,6 +562,7 @@ class PhotonMapper:
         self.slot_states = N.DeviceBuffer(slots * 16)
         self.photon_rays = N.DeviceBuffer(slots * 32)
         self.photon_states = N.DeviceBuffer(slots * 16)
+        self.records = N.DeviceBuffer(slots * 32) if merge == "fused" else None
         self.queries = N.DeviceBuffer(self.n_cam * 64)
         self.out_idx = N.DeviceBuffer(self.n_cam * k * 4)
         self.out_dist = N.DeviceBuffer(self.n_cam * k * 4)
@@ -639,6 +651,9 @@ class PhotonMapper:
 
     def build_grid(self):
         self.grid.build(self.photon_rays, 32, 0, self.photon_count, self.radius, self.stream)
+        if self.records is not None:
+            N.check(N.lib.imba_ppm_merge_records(self.grid.h, self.photon_rays.ptr, self.photon_states.ptr,
+                                                 self.records.ptr, self.stream.handle), "ppm_merge_records")
 
     def camera_pass(self):
         st = self.stream.handle
@@ -665,6 +680,11 @@ class PhotonMapper:
     def query(self):
         self.grid.query(self.queries, 64, 0, self.n_cam, self.k, self.out_idx, self.out_count, self.out_dist,
                         self.stream)
+
+    def merge_grid(self):
+        """The grid query and the merge in one launch (merge="fused")."""
+        N.check(N.lib.imba_ppm_merge_grid(self.grid.h, self.records.ptr, self.queries.ptr, self.n_cam, self.k,
+                                          self.radius, self.P, self.image.ptr, self.stream.handle), "ppm_merge_grid")
 
     def merge(self):
         N.check(N.lib.imba_ppm_merge(self.queries.ptr, self.n_cam, self.k, self.out_idx.ptr, self.out_dist.ptr,
@@ -697,9 +717,12 @@ class PhotonMapper:
         mark("grid_build")
         self.camera_pass()
         mark("camera")
-        self.query()
-        mark("query")
-        self.merge()
+        if self.merge_mode == "fused":
+            self.merge_grid()
+        else:
+            self.query()
+            mark("query")
+            self.merge()
         mark("merge")
         c = self.counts.download(np.int32, 2 * self.L + 4, self.stream)
         self.stream.sync()

@@ -568,6 +568,12 @@ int imba_shade_batch_release(ImbaShadeBatch* b);
  *   camera | throughput * albedo / pi) and ends the path; mirror and glass continue it.
  * imba_ppm_merge: vertex_merging (vcm.cpp:572-611, mis weight 1) from imba_hashgrid_query's
  *   output over the queries: Epanechnikov-weighted photon throughput into image.
+ * imba_ppm_merge_records + imba_ppm_merge_grid: the same merge without the index lists: the
+ *   photons' directions and states gathered once into grid slot order (32 bytes per photon,
+ *   the grid built over photon_rays), then the grid query and the merge in one launch that
+ *   reads them by slot (1 <= k <= 64).  The lists, distances and sums are those of
+ *   imba_hashgrid_query + imba_ppm_merge; only the float rounding of the sum may differ (no
+ *   fused multiply-add into the running sum).
  * count_dev (nullable): the queue size read on the device when the launch runs. */
 int imba_ppm_light(const float* light_tri9, const float* intensity3, uint32_t seed_base, int first, int count,
                    ImbaRay* rays, void* states, void* stream);
@@ -585,6 +591,10 @@ int imba_ppm_camera_bounce(const ImbaRay* rays, const ImbaHit* hits, void* state
 int imba_ppm_merge(const void* queries, int nq, int k, const int* out_idx, const float* out_dist, const int* out_count,
                    const void* photon_rays, const void* photon_states, float radius, int light_paths, float* image,
                    void* stream);
+int imba_ppm_merge_records(const ImbaHashGrid* grid, const void* photon_rays, const void* photon_states,
+                           void* records, void* stream);
+int imba_ppm_merge_grid(const ImbaHashGrid* grid, const void* records, const void* queries, int nq, int k,
+                        float radius, int light_paths, float* image, void* stream);
 
 /* ------------------------------ vertex connection and merging ---------------------------
  * One iteration of VCMIntegrator<algo> (render/integrators/vcm.cpp) on the device, for the

@@ -200,6 +200,44 @@ def test_merge_image(pm):
     assert np.allclose(got, want, rtol=1e-5, atol=1e-6 * np.abs(want).max())
 
 
+def test_fused_merge(pm):
+    """imba_ppm_merge_records gathers every photon's direction and state into grid slot order,
+    and imba_ppm_merge_grid (query and merge in one launch, reading those records) adds the
+    image of imba_hashgrid_query + imba_ppm_merge: the same lists and terms, only the running
+    sum's rounding may differ."""
+    n = pm.photon_count
+    rec = N.DeviceBuffer(n * 32)
+    N.check(N.lib.imba_ppm_merge_records(pm.grid.h, pm.photon_rays.ptr, pm.photon_states.ptr, rec.ptr, None),
+            "ppm_merge_records")
+    order = np.empty(n, np.int32)
+    N.check(N.lib.imba_memcpy_d2h(N.ptr(order), pm.grid.info().order, order.nbytes, None), "d2h")
+    ph_rays, ph_states = pm.photons_host()
+    got = rec.download(np.uint8, n * 32).reshape(n, 32)
+    assert np.array_equal(got[:, :16], ph_rays.view(np.uint8).reshape(n, 32)[order, 16:])
+    assert np.array_equal(got[:, 16:], ph_states.view(np.uint8).reshape(n, 16)[order])
+    pm.clear_image()
+    pm.merge()
+    lists = pm.image.download(np.float32, pm.w * pm.h * 3, pm.stream).astype(np.float64)
+    pm.clear_image()
+    N.check(N.lib.imba_ppm_merge_grid(pm.grid.h, rec.ptr, pm.queries.ptr, pm.n_cam, pm.k, pm.radius, pm.P,
+                                      pm.image.ptr, pm.stream.handle), "ppm_merge_grid")
+    fused = pm.image.download(np.float32, pm.w * pm.h * 3, pm.stream).astype(np.float64)
+    assert np.count_nonzero(lists) > 0
+    assert np.allclose(fused, lists, rtol=1e-5, atol=1e-6 * np.abs(lists).max())
+
+
+def test_fused_frame(pm):
+    """A frame with merge="fused" renders the image of the query + merge frame."""
+    pm.clear_image()
+    pm.frame()
+    want = pm.image.download(np.float32, pm.w * pm.h * 3, pm.stream).astype(np.float64)
+    f = W.PhotonMapper(pm.s, pm.w, pm.h, light_paths=pm.P, max_depth=pm.L, rr_start=pm.rr_start,
+                       radius_factor=0.02, k=pm.k, merge="fused")
+    assert f.frame()[2] == pm.photon_count
+    got = f.image.download(np.float32, f.w * f.h * 3, f.stream).astype(np.float64)
+    assert np.allclose(got, want, rtol=1e-5, atol=1e-6 * np.abs(want).max())
+
+
 def test_frames_repeat(pm):
     """A second frame reproduces the photon set and the query results exactly."""
     a = pm.photons_host()[0].copy()

@@ -91,3 +91,59 @@ def test_photon_mapping_requests_alone_and_in_a_mix(gpu, scenes):
             n_ppm_lit += r.integrator == "ppm"
     assert n_ppm_lit >= 2  # the photon-mapping images received photons
     assert mix.arena.free == [(0, mix.arena.size)]
+
+
+def _vcm_requests():
+    reqs = _requests()
+    for k, r in enumerate(reqs):
+        r.integrator = "vcm" if k % 2 == 0 else "pt"
+    return reqs
+
+
+def test_vcm_requests_alone_and_in_a_mix(gpu, scenes):
+    """VCM requests served beside PT requests: each request's counts and image are the same
+    alone and in the mix."""
+    mix = V.Server(scenes, 512 << 20, max_active=8, batched=True)
+    done_mix, img_mix = mix.run(_vcm_requests(), keep_images=True)
+    alone = V.Server(scenes, 512 << 20, batched=False)
+    done_alone, img_alone = alone.run(_vcm_requests(), keep_images=True)
+    assert len(done_mix) == len(done_alone) == 6
+    by_mix = {r.rid: r for r in done_mix}
+    n_vcm_lit = 0
+    for r in done_alone:
+        m = by_mix[r.rid]
+        assert m.counts == r.counts and m.rays == r.rays
+        first = r.width * r.height if r.integrator == "vcm" else r.samples
+        assert r.counts[0] == first
+        a, b = img_alone[r.rid], img_mix[r.rid]
+        assert np.allclose(a, b, rtol=1e-5, atol=1e-6)
+        lit = np.abs(a) > 1e-6
+        if lit.any():
+            assert np.mean(np.abs(a[lit] - b[lit]) / np.abs(a[lit])) <= 1e-4
+            n_vcm_lit += r.integrator == "vcm"
+    assert n_vcm_lit >= 2
+    assert mix.arena.free == [(0, mix.arena.size)]
+
+
+def test_served_vcm_request_is_the_vcm_iteration(gpu, scenes):
+    """A served VCM request renders the iteration imbatracer_amd.vcm.VcmRenderer renders with
+    the same camera, seeds and settings (whose steps tests/test_gpu_vcm.py restates)."""
+    from imbatracer_amd import vcm as VC
+
+    req = _vcm_requests()[2]
+    sc = scenes[req.scene]
+    srv = V.Server(scenes, 512 << 20, batched=False)
+    done, img = srv.run([req], keep_images=True)
+    vr = VC.VcmRenderer(sc.s, req.width, req.height, light_paths=req.width * req.height, spp=req.spp,
+                        max_path_len=V.VCM_DEPTH, light_max_depth=V.VCM_DEPTH, rr_start=V.VCM_RR_START,
+                        radius_factor=0.005, k=V.VCM_K, num_connections=V.VCM_C, seed_base=req.seed,
+                        light_seed=(req.seed ^ 0x5A5A5A5A) & 0xFFFFFFFF, gpu_scene=sc.gpu,
+                        camera=(req.cam_pos, req.cam_dir, (0.0, 1.0, 0.0), 60.0))
+    assert vr.radius == sc.radius
+    light, cam, nv = vr.frame()
+    assert done[0].counts[:len(light)] == light
+    want = vr.image_host().reshape(-1)
+    got = img[req.rid]
+    lit = np.abs(want) > 1e-6
+    assert lit.mean() > 0.2
+    assert np.mean(np.abs(got[lit] - want[lit]) / np.abs(want[lit])) <= 1e-4

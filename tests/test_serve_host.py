@@ -32,6 +32,18 @@ def test_footprint_covers_the_buffers():
     assert V.footprint(r) >= n * (2 * 32 + 2 * 32 + 16) + 640 * 480 * 12
 
 
+def test_vcm_footprint_covers_the_buffers():
+    r = V.Request(0, 0, 0, 640, 480, 2, (0, 0, 0), (0, 0, 1), 1, 0.0, integrator="vcm")
+    p = 640 * 480
+    cap, slots = max(p, 2 * p), V.VCM_DEPTH * p
+    # rays, states, hits, shadow slots (ray, state, hit), vertex slots, cache, merge records,
+    # merge queries, image
+    need = (2 * 32 * cap + 2 * 48 * cap + 16 * cap + (1 + V.VCM_C) * cap * 64 + 5 * 32 * slots + 64 * cap +
+            12 * p)
+    assert V.footprint(r) >= need
+    assert V.first_rays(r) == p + 2 * p
+
+
 def test_poisson_requests():
     reqs = V.poisson_requests(4000, 200.0, 16, 8, [(0.5, 0.4, 0.5, 3.0)] * 8, seed=3)
     t = np.array([r.arrival for r in reqs])
