@@ -344,7 +344,7 @@ class _VCMJob:
     added), `cache` (slots compacted; the cache size is read back), `grid` (hash grid and merge
     records built, camera rays shot), `camera` (bounces with next event, connections and the
     fused merge) until every path has ended.  The light and camera queues join the step's
-    batched traversal; the shadow rays of a bounce are traversed in the request's own launch."""
+    batched closest-hit traversal and the bounce's shadow rays its batched any-hit traversal."""
 
     def __init__(self, req: Request, slot: int, block, scene: ServedScene, arena: DeviceArena):
         self.req, self.slot, self.block, self.sc = req, slot, block, scene
@@ -430,18 +430,38 @@ class _VCMJob:
                 "vcm_camera")
         self.phase, self.bounce, self.cur = "camera", 0, 0
 
-    def after_trace(self, srv, st):
-        sc, b, cur, fa = self.sc, self.bounce, self.cur, C.byref(self.frame)
-        cfg = C.byref(srv.cfg) if srv.cfg else None
+    def bounce_step(self, srv, st):
+        """The bounce kernel of the current phase (after the closest-hit traversal): light
+        vertices and camera connections, or next event, connections and merge queries."""
+        b, cur, fa = self.bounce, self.cur, C.byref(self.frame)
+        if self.phase == "light":
+            N.check(N.lib.imba_vcm_light_bounce(fa, self.rays[cur], self.hits, self.states[cur], self.P,
+                                                self._light(srv, b), C.c_void_p(self.slot_a.value + 32 * b * self.P),
+                                                C.c_void_p(self.slot_b.value + 32 * b * self.P), self.rays[cur],
+                                                self.shadow_rays, self.shadow_states, st), "vcm_light_bounce")
+        else:
+            N.check(N.lib.imba_vcm_camera_bounce(fa, self.rays[cur], self.hits, self.states[cur], self.n,
+                                                 self._camera(srv, b), self.vtx_a, self.vtx_b, self._cache(srv),
+                                                 self.rays[cur], self.shadow_rays, self.shadow_states,
+                                                 self._shadow(srv), self.queries, self.image, st), "vcm_camera_bounce")
+
+    def shadow_item(self, srv):
+        """The bounce's shadow rays as a traversal item (any hit), with the estimate of their
+        number the server uses to choose an own launch."""
+        if self.phase == "light":
+            return (self.sc.gpu, self.shadow_rays, self.shadow_hits, self.P, self._light(srv, self.bounce), True,
+                    self.queued)
+        ns = (1 + VCM_C) * self.n
+        return (self.sc.gpu, self.shadow_rays, self.shadow_hits, ns, self._shadow(srv), True,
+                (1 + VCM_C) * self.queued)
+
+    def finish_step(self, srv, st):
+        """After the shadow traversal: the unoccluded contributions, the merge (camera bounces)
+        and the compaction; the last light bounce also compacts the vertex slots."""
+        b, cur, fa = self.bounce, self.cur, C.byref(self.frame)
         sb = VC.VCM_STATE_DTYPE.itemsize
         if self.phase == "light":
             cnt = self._light(srv, b)
-            N.check(N.lib.imba_vcm_light_bounce(fa, self.rays[cur], self.hits, self.states[cur], self.P, cnt,
-                                                C.c_void_p(self.slot_a.value + 32 * b * self.P),
-                                                C.c_void_p(self.slot_b.value + 32 * b * self.P), self.rays[cur],
-                                                self.shadow_rays, self.shadow_states, st), "vcm_light_bounce")
-            N.check(N.lib.imba_occluded_dev(sc.gpu.h, self.shadow_rays, self.shadow_hits, self.P, cnt, cfg, st),
-                    "occluded_dev")
             N.check(N.lib.imba_pt_shadow(self.shadow_hits, self.shadow_states, self.P, cnt, self.image, st),
                     "pt_shadow")
             N.check(N.lib.imba_compact_rays_dev(self.rays[cur], self.states[cur], sb, self.P, cnt, self.rays[1 - cur],
@@ -456,12 +476,6 @@ class _VCMJob:
         elif self.phase == "camera":
             cnt = self._camera(srv, b)
             ns = (1 + VCM_C) * self.n
-            N.check(N.lib.imba_vcm_camera_bounce(fa, self.rays[cur], self.hits, self.states[cur], self.n, cnt,
-                                                 self.vtx_a, self.vtx_b, self._cache(srv), self.rays[cur],
-                                                 self.shadow_rays, self.shadow_states, self._shadow(srv),
-                                                 self.queries, self.image, st), "vcm_camera_bounce")
-            N.check(N.lib.imba_occluded_dev(sc.gpu.h, self.shadow_rays, self.shadow_hits, ns, self._shadow(srv), cfg,
-                                            st), "occluded_dev")
             N.check(N.lib.imba_pt_shadow(self.shadow_hits, self.shadow_states, ns, self._shadow(srv), self.image, st),
                     "pt_shadow")
             if self.vertices > 0:
@@ -589,10 +603,12 @@ class Server:
             self.active.append(job)
 
     def step(self):
-        """One step: admit; the photon-mapping requests' non-traversal phases (grid build and
-        camera rays, or queries and merging); one batched traversal over every active queue;
-        the PT bounces and compactions in one batched launch and the photon-mapping ones; one
-        read-back of all queue sizes."""
+        """One step: admit; the phased requests' non-traversal phases (grid build and camera
+        rays, or queries and merging); one batched traversal over every active queue; the PT
+        bounces and compactions in one batched launch and the VCM bounce kernels; one batched
+        any-hit traversal over every shadow queue (PT next event, VCM connections and next
+        event); the shadow contributions, VCM merges and compactions and the photon-mapping
+        bounces; one read-back of all queue sizes."""
         self._ev[0].record(self.stream)
         self._admit()
         if not self.active:
@@ -612,21 +628,29 @@ class Server:
         if shared:
             self.batch.update(shared, self.stream)
             self.batch.launch(self.cfg, self.stream)
+        vcm_steps = [j for j in ppm_jobs if isinstance(j, _VCMJob) and j.tracing()]
+        sitems = []
         if self.batched and pt_jobs:
-            # every PT request's shading step and compaction in one batched launch, then every
-            # request's shadow rays (batched as the closest-hit queues are) and their contributions
+            # every PT request's shading step and compaction in one batched launch; their shadow
+            # rays join the step's any-hit traversal below
             self.shade.update([self._shade_item(j) for j in pt_jobs], self.stream)
             self.shade.launch(self.stream)
-            sitems = [(j.sc.gpu, j.shadow_rays, j.shadow_hits, j.n, self.counts_ptr(j.slot, j.bounce), True, j.queued)
-                      for j in pt_jobs]
-            for sc, rays, hits, cap, cnt, _, q in sitems:
-                if q >= self.OWN_LAUNCH_RAYS:
-                    N.check(N.lib.imba_occluded_dev(sc.h, rays, hits, cap, cnt, C.byref(self.cfg) if self.cfg else None,
-                                                    st), "occluded_dev")
-            shared_s = [it[:6] for it in sitems if it[6] < self.OWN_LAUNCH_RAYS]
-            if shared_s:
-                self.shadow_batch.update(shared_s, self.stream)
-                self.shadow_batch.launch(self.cfg, self.stream)
+            sitems += [(j.sc.gpu, j.shadow_rays, j.shadow_hits, j.n, self.counts_ptr(j.slot, j.bounce), True, j.queued)
+                       for j in pt_jobs]
+        for j in vcm_steps:
+            j.bounce_step(self, st)
+            sitems.append(j.shadow_item(self))
+        # every shadow queue of the step (PT next event, VCM camera connections / next event /
+        # vertex connections): long ones in their own launch, the rest in one batched launch
+        for sc, rays, hits, cap, cnt, _, q in sitems:
+            if not self.batched or q >= self.OWN_LAUNCH_RAYS:
+                N.check(N.lib.imba_occluded_dev(sc.h, rays, hits, cap, cnt, C.byref(self.cfg) if self.cfg else None,
+                                                st), "occluded_dev")
+        shared_s = [it[:6] for it in sitems if self.batched and it[6] < self.OWN_LAUNCH_RAYS]
+        if shared_s:
+            self.shadow_batch.update(shared_s, self.stream)
+            self.shadow_batch.launch(self.cfg, self.stream)
+        if self.batched and pt_jobs:
             self.shade.shadow(self.stream)
         elif pt_jobs:
             for j in pt_jobs:
@@ -642,8 +666,10 @@ class Server:
                                                     j.rays[1 - j.cur], j.states[1 - j.cur],
                                                     self.counts_ptr(j.slot, j.bounce + 1), j.workspace, st),
                         "compact_dev")
+        for j in vcm_steps:
+            j.finish_step(self, st)
         for j in ppm_jobs:
-            if j.tracing():
+            if isinstance(j, _PPMJob) and j.tracing():
                 j.after_trace(self, st)
         for j in pt_jobs:
             j.bounce += 1

@@ -571,9 +571,8 @@ int imba_shade_batch_release(ImbaShadeBatch* b);
  * imba_ppm_merge_records + imba_ppm_merge_grid: the same merge without the index lists: the
  *   photons' directions and states gathered once into grid slot order (32 bytes per photon,
  *   the grid built over photon_rays), then the grid query and the merge in one launch that
- *   reads them by slot (1 <= k <= 64).  The lists, distances and sums are those of
- *   imba_hashgrid_query + imba_ppm_merge; only the float rounding of the sum may differ (no
- *   fused multiply-add into the running sum).
+ *   reads them by slot (1 <= k <= 64).  The lists, distances, terms and per-query sums are
+ *   those of imba_hashgrid_query + imba_ppm_merge (same order of operations).
  * count_dev (nullable): the queue size read on the device when the launch runs. */
 int imba_ppm_light(const float* light_tri9, const float* intensity3, uint32_t seed_base, int first, int count,
                    ImbaRay* rays, void* states, void* stream);
