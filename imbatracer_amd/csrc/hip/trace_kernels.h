@@ -615,10 +615,11 @@ __device__ __forceinline__ void write_result(const TraceArgs& a, const TraceStat
 }
 
 // One ray per thread, rays strided over a grid sized to the resident waves.
-// The 4-wide closest-hit variants are held to 7 waves per SIMD (72 VGPRs, a few bytes of
-// scratch per lane): faster than their natural 6 (77 VGPRs) and than 8 (64 VGPRs, more spills).
+// The 4-wide variants are held to 7 waves per SIMD (72 VGPRs, a few bytes of scratch per
+// lane): faster than their natural 6 (77 VGPRs; PT frames 4% faster with the any-hit variant
+// held too) and, for closest hit, than 8 (64 VGPRs, more spills: 9% slower).
 template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS, int TREE>
-__global__ __launch_bounds__(BLOCK, (ANY || MASKED || STATS || TREE == 0) ? 1 : 7) void k_trace(TraceArgs a) {
+__global__ __launch_bounds__(BLOCK, (MASKED || STATS || TREE == 0) ? 1 : 7) void k_trace(TraceArgs a) {
     __shared__ int lds_stack[STACK_LDS * BLOCK];
     LaneStack<BLOCK, STACK_LDS> st;
     stack_init(st, lds_stack, a);
@@ -757,11 +758,11 @@ __global__ __launch_bounds__(1024) void k_batch_prefix(BatchItem* items, BatchGr
 
 // TL (quantized tree, tl_max > 0): every wave keeps its own LDS copy of the first
 // min(tl_max, a.tl_nodes) treelet nodes of the scene of its current chunk (tl_max x 64 B per
-// wave of dynamic LDS), reloaded when a chunk of another scene comes.  The 4-wide closest-hit
-// variants are held to 7 waves per SIMD (72 VGPRs, a few bytes of scratch per lane), faster
-// than their natural 6.
+// wave of dynamic LDS), reloaded when a chunk of another scene comes.  The 4-wide variants
+// are held to 7 waves per SIMD (72 VGPRs, a few bytes of scratch per lane), faster than their
+// natural 6.
 template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, int TREE, bool TL>
-__global__ __launch_bounds__(BLOCK, (ANY || MASKED) ? 1 : (TREE == 0 ? 8 : 7)) void k_trace_batch(const BatchItem* __restrict__ items, int first, int n_items,
+__global__ __launch_bounds__(BLOCK, MASKED ? 1 : (TREE == 0 && !ANY ? 8 : 7)) void k_trace_batch(const BatchItem* __restrict__ items, int first, int n_items,
                                                        const int* __restrict__ total_chunks, int* counter, int* spill,
                                                        int spill_depth, int tl_max) {
     extern __shared__ uint4 treelet_lds[];
