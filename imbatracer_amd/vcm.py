@@ -144,6 +144,11 @@ class VcmRenderer:
         self.shadow_rays = N.DeviceBuffer(ns * 32)
         self.shadow_states = N.DeviceBuffer(ns * 16)
         self.shadow_hits = N.DeviceBuffer(ns * 16)
+        # the occupied shadow slots, stable-compacted before their traversal (a wave then holds
+        # no empty slots beside live rays)
+        self.compact_shadows = True
+        self.shadow_rays_c = N.DeviceBuffer(ns * 32)
+        self.shadow_states_c = N.DeviceBuffer(ns * 16)
         self.slots = self.L * self.P
         self.slot_a = N.DeviceBuffer(self.slots * 32)
         self.slot_b = N.DeviceBuffer(self.slots * 32)
@@ -154,9 +159,9 @@ class VcmRenderer:
         self.image = N.DeviceBuffer(width * height * 12)
         # device counts: light queues [0, L], camera queues [L+1, L+1+Lc], slot count, vertex
         # count, shadow count
-        self.n_counts = self.L + self.Lc + 5
+        self.n_counts = self.L + self.Lc + 6
         self.counts = N.DeviceBuffer(4 * self.n_counts)
-        self.workspace = N.DeviceBuffer(int(N.lib.imba_compact_workspace_size(max(self.slots, cap))))
+        self.workspace = N.DeviceBuffer(int(N.lib.imba_compact_workspace_size(max(self.slots, ns))))
         self.grid = HashGrid()
         self.stream = N.Stream()
         self.vertex_count = 0
@@ -212,6 +217,47 @@ class VcmRenderer:
     def _shadow_cnt(self):
         return self._cnt(self.L + self.Lc + 4)
 
+    def _live_shadow_cnt(self):
+        return self._cnt(self.L + self.Lc + 5)
+
+    def _slot_hits(self, shadow_states):
+        """The any-hit answers of the last shadow traversal in slot order (tests): with
+        compact_shadows, the occupied slots' answers scattered back to their slots (empty
+        slots read tri_id -1)."""
+        n = len(shadow_states)
+        if not self.compact_shadows:
+            return self.shadow_hits.download(N.HIT_DTYPE, n, self.stream)
+        live = np.nonzero(shadow_states["pixel"] >= 0)[0]
+        out = np.zeros(n, N.HIT_DTYPE)
+        out["tri_id"] = -1
+        out["inst_id"] = -1
+        out[live] = self.shadow_hits.download(N.HIT_DTYPE, len(live), self.stream)
+        return out
+
+    def _trace_shadows(self, capacity: int, count_dev):
+        """Any-hit traversal of the shadow slots [0, *count_dev) and the unoccluded ones'
+        contributions.  With compact_shadows the occupied slots are first stable-compacted
+        (shadow_rays_c / shadow_states_c, their count in the live-shadow count slot) and
+        shadow_hits holds the answers in that order; otherwise in slot order."""
+        st = self.stream.handle
+        rays, states, cnt = self.shadow_rays, self.shadow_states, count_dev
+        if self.compact_shadows:
+            def compact_and_trace():
+                N.check(N.lib.imba_compact_rays_dev(self.shadow_rays.ptr, self.shadow_states.ptr, 16, capacity,
+                                                    count_dev, self.shadow_rays_c.ptr, self.shadow_states_c.ptr,
+                                                    self._live_shadow_cnt(), self.workspace.ptr, st), "compact shadows")
+                N.check(N.lib.imba_occluded_dev(self.gpu_scene.h, self.shadow_rays_c.ptr, self.shadow_hits.ptr,
+                                                capacity, self._live_shadow_cnt(), self._cfg(self.shadow_cfg), st),
+                        "occluded_dev")
+            self._timed("shadow", compact_and_trace)
+            rays, states, cnt = self.shadow_rays_c, self.shadow_states_c, self._live_shadow_cnt()
+        else:
+            self._timed("shadow", lambda: N.check(N.lib.imba_occluded_dev(
+                self.gpu_scene.h, rays.ptr, self.shadow_hits.ptr, capacity, cnt, self._cfg(self.shadow_cfg), st),
+                "occluded_dev"))
+        N.check(N.lib.imba_pt_shadow(self.shadow_hits.ptr, states.ptr, capacity, cnt, self.image.ptr, st),
+                "pt_shadow")
+
     def _cfg(self, cfg):
         return C.byref(cfg) if cfg else None
 
@@ -253,11 +299,7 @@ class VcmRenderer:
                                                 self.slot_a.at(32 * b * P), self.slot_b.at(32 * b * P),
                                                 self.rays[cur].ptr, self.shadow_rays.ptr, self.shadow_states.ptr, st),
                     "vcm_light_bounce")
-            self._timed("shadow", lambda: N.check(N.lib.imba_occluded_dev(
-                self.gpu_scene.h, self.shadow_rays.ptr, self.shadow_hits.ptr, P, cnt, self._cfg(self.shadow_cfg), st),
-                "occluded_dev"))
-            N.check(N.lib.imba_pt_shadow(self.shadow_hits.ptr, self.shadow_states.ptr, P, cnt, self.image.ptr, st),
-                    "pt_shadow")
+            self._trace_shadows(P, cnt)
             if keep is not None:
                 n = len(keep["rays"])
                 keep["out_rays"] = self.rays[cur].download(N.RAY_DTYPE, n, self.stream)
@@ -266,7 +308,7 @@ class VcmRenderer:
                 keep["vtx_b"] = self.slot_b.download(VTX_B_DTYPE, n, self.stream, offset=32 * b * P)
                 keep["shadow_rays"] = self.shadow_rays.download(N.RAY_DTYPE, n, self.stream)
                 keep["shadow_states"] = self.shadow_states.download(SHADOW_STATE_DTYPE, n, self.stream)
-                keep["shadow_hits"] = self.shadow_hits.download(N.HIT_DTYPE, n, self.stream)
+                keep["shadow_hits"] = self._slot_hits(keep["shadow_states"])
                 self.kept["light"].append(keep)
             N.check(N.lib.imba_compact_rays_dev(self.rays[cur].ptr, self.states[cur].ptr, 48, P, cnt,
                                                 self.rays[1 - cur].ptr, self.states[1 - cur].ptr,
@@ -326,11 +368,7 @@ class VcmRenderer:
             if keep is not None:
                 m = len(keep["rays"])
                 keep["image_emit"] = self.image.download(np.float32, self.w * self.h * 3, self.stream)
-            self._timed("shadow", lambda: N.check(N.lib.imba_occluded_dev(
-                self.gpu_scene.h, self.shadow_rays.ptr, self.shadow_hits.ptr, ns, self._shadow_cnt(),
-                self._cfg(self.shadow_cfg), st), "occluded_dev"))
-            N.check(N.lib.imba_pt_shadow(self.shadow_hits.ptr, self.shadow_states.ptr, ns, self._shadow_cnt(),
-                                         self.image.ptr, st), "pt_shadow")
+            self._trace_shadows(ns, self._shadow_cnt())
             if keep is not None:
                 keep["image_shadow"] = self.image.download(np.float32, self.w * self.h * 3, self.stream)
             if self.algo == "vcm":
@@ -343,7 +381,7 @@ class VcmRenderer:
                 keep["out_states"] = self.states[cur].download(VCM_STATE_DTYPE, m, self.stream)
                 keep["shadow_rays"] = self.shadow_rays.download(N.RAY_DTYPE, (1 + self.C) * m, self.stream)
                 keep["shadow_states"] = self.shadow_states.download(SHADOW_STATE_DTYPE, (1 + self.C) * m, self.stream)
-                keep["shadow_hits"] = self.shadow_hits.download(N.HIT_DTYPE, (1 + self.C) * m, self.stream)
+                keep["shadow_hits"] = self._slot_hits(keep["shadow_states"])
                 keep["queries"] = self.queries.download(VCM_QUERY_DTYPE, m, self.stream)
                 self.kept["camera"].append(keep)
             N.check(N.lib.imba_compact_rays_dev(self.rays[cur].ptr, self.states[cur].ptr, 48, n, cnt,
