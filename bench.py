@@ -22,8 +22,11 @@ launch, against the same queues launched one at a time.
 cross-session batching against the plain per-request path (p50/p99 latency and Mrays/s).
 
 Also reported: the roofline of the traversal kernel (algorithmic bytes per ray
-32 + 16 + 64 N_node + 48 N_tri + 64 N_inst with visit counts measured by the stats variant of
-the same kernel on the same rays, over the kernel time measured with HIP events on its stream)
+32 + 16 + 64 N_node + 48 N_tri + 64 N_inst with N_node the inner nodes of the reference's binary
+tree (BVH2) whose child boxes a ray tests, N_tri its triangle tests and N_inst its instance
+visits, counted on the same rays by the BVH2 stats variant, whose counts equal the oracle's,
+over the kernel time measured with HIP events on its stream; the same figure with the visit
+counts of the tree the kernel walks is reported beside it)
 and a CPU baseline: the reference's CPU path restated (oracle/cpu_bvh4.c over the reference's
 CPU layout, BVH4 nodes + 4-triangle packets built from the same scene) on host threads over a
 bounded sample of the same rays.
@@ -89,6 +92,31 @@ def cfg_dict(cfg):
         return "default"
     return {"block": cfg.block_size or 256, "stack_lds": cfg.stack_lds or 16, "persistent": cfg.persistent,
             "bvh_width": cfg.bvh_width or 2, "quantized": cfg.quantized, "treelet": cfg.treelet}
+
+
+BVH2_CFG_ARGS = (256, 16, 0, 0, 2, 0, 0)  # TraceConfig of the reference's binary tree
+
+
+def bvh2_visits(scene, rays, any_hit=False):
+    """Summed visit counts (inner nodes, triangles, instances) of rays on the reference's binary
+    tree (BVH2, nearer child first, ties to the left child: the oracle's traversal rule and
+    counts).  rays: a host ray array."""
+    from imbatracer_amd import native as N
+
+    n = len(rays)
+    if n == 0:
+        return np.zeros(3, np.int64)
+    rb = N.DeviceBuffer.from_array(rays)
+    hb, cb = N.DeviceBuffer(n * 16), N.DeviceBuffer(n * 12)
+    scene.trace_stats(rb, hb, cb, n, any_hit=any_hit, cfg=N.TraceConfig(*BVH2_CFG_ARGS))
+    return cb.download(np.uint32, 3 * n).reshape(-1, 3).astype(np.int64).sum(0)
+
+
+BYTES_RULE = ("per ray 32 (ray) + 16 (hit) + 64 x inner nodes of the reference's binary tree (BVH2) whose "
+              "child boxes the ray tests + 48 x triangles tested + 64 x instances visited, counted on the same "
+              "rays by the BVH2 stats variant (the oracle's counts); the kernel itself walks the quantized "
+              "4-wide tree derived from the BVH2 (own_tree: the same formula with that tree's counts, 64 B per "
+              "quantized node)")
 
 
 def node_bytes_of(cfg):
@@ -348,14 +376,19 @@ def run_pt(args):
     stat_bytes = [ray_bytes(v, n, node_bytes) for v, n in zip(visits, counts)]
     shadow_bytes = [ray_bytes(v, n, node_bytes_of(pt.shadow_cfg_of(b)))
                     for b, (v, n) in enumerate(zip(shadow_visits, counts))]
+    # the algorithmic bytes of the roofline: the reference's BVH2 visit counts on the same rays
+    bvh2_vis = [bvh2_visits(pt.gpu_scene, q) for q in queues]
+    bvh2_shadow_vis = [bvh2_visits(pt.gpu_scene, q, any_hit=True) for q in shadow_queues]
+    contract_bytes = [ray_bytes(v, len(q), 64) for v, q in zip(bvh2_vis, queues)]
+    contract_shadow_bytes = [ray_bytes(v, len(q), 64) for v, q in zip(bvh2_shadow_vis, shadow_queues)]
     log("rays per bounce %s, shadow rays %s" % (counts, shadow_counts))
 
     # eager frames (host reads each bounce's queue size): traversal kernel times for the roofline
     for _ in range(args.warmup):
         pt.frame()
     pt.stream.sync()
-    eager_ms, trace_ms, rays_eager, bytes_total = [], 0.0, 0, 0.0
-    shadow_ms, shadow_bytes_total, shadow_rays_eager = 0.0, 0.0, 0
+    eager_ms, trace_ms, rays_eager, bytes_total, contract_total = [], 0.0, 0, 0.0, 0.0
+    shadow_ms, shadow_bytes_total, shadow_rays_eager, contract_shadow_total = 0.0, 0.0, 0, 0.0
     per_bounce_ms = np.zeros(len(counts))
     per_bounce_shadow_ms = np.zeros(len(counts))
     t_eager = time.perf_counter()
@@ -369,10 +402,12 @@ def run_pt(args):
         for b, ((ms, n), sb) in enumerate(zip(pt.trace_times_ms(), stat_bytes)):
             trace_ms += ms
             bytes_total += sb
+            contract_total += contract_bytes[b]
             per_bounce_ms[b] += ms
         for b, ((ms, n), sb) in enumerate(zip(pt.shadow_times_ms(), shadow_bytes)):
             shadow_ms += ms
             shadow_bytes_total += sb
+            contract_shadow_total += contract_shadow_bytes[b] if b < len(contract_shadow_bytes) else 0.0
             per_bounce_shadow_ms[b] += ms
         rays_eager += sum(c)
         shadow_rays_eager += sum(shadow_counts)
@@ -422,8 +457,10 @@ def run_pt(args):
         wall = time.perf_counter() - t_start
     if pt.gpu_scene.error_count():
         raise RuntimeError("traversal reported a malformed scene")
-    achieved = bytes_total / (trace_ms * 1e-3) / 1e9
-    shadow_achieved = shadow_bytes_total / (shadow_ms * 1e-3) / 1e9
+    achieved = contract_total / (trace_ms * 1e-3) / 1e9
+    shadow_achieved = contract_shadow_total / (shadow_ms * 1e-3) / 1e9
+    own_achieved = bytes_total / (trace_ms * 1e-3) / 1e9
+    own_shadow_achieved = shadow_bytes_total / (shadow_ms * 1e-3) / 1e9
     workload = "pt-%dx%d-1spp-depth8-rr3-nee-sphere-grid-%dtri-sbvh" % (args.width, args.height, scene.tri_count)
     traffic, traffic_src = measured_traffic(workload, kernel_label(cfg))
     shadow_frame = sum(shadow_counts) * args.steps
@@ -484,11 +521,18 @@ def run_pt(args):
                                   "frac": shadow_achieved / HBM_PEAK_GBS,
                                   "kernel": "k_trace (any hit) over the shadow slots",
                                   "traffic": measured_traffic(workload, kernel_label(pt.shadow_cfg_of(0), any_hit=True))[0],
-                                  "algorithmic_bytes_per_launch": sum(shadow_bytes) / len(shadow_bytes),
-                                  "visits_per_slot": [float(x) / sum(counts) for x in np.sum(shadow_visits, 0)]}},
+                                  "algorithmic_bytes_per_launch": sum(contract_shadow_bytes) / len(shadow_bytes),
+                                  "bytes_rule": BYTES_RULE + "; the shadow rays only, not the empty slots",
+                                  "bvh2_visits_per_ray": [float(x) / max(sum(shadow_counts), 1)
+                                                          for x in np.sum(bvh2_shadow_vis, 0)],
+                                  "own_tree": {"achieved": own_shadow_achieved,
+                                               "frac": own_shadow_achieved / HBM_PEAK_GBS,
+                                               "visits_per_slot": [float(x) / sum(counts)
+                                                                   for x in np.sum(shadow_visits, 0)]}}},
         "frame_traversal_bandwidth": {
-            "achieved": (sum(stat_bytes) + sum(shadow_bytes)) / (wall / args.steps) / 1e9, "unit": "GB/s",
-            "frac": (sum(stat_bytes) + sum(shadow_bytes)) / (wall / args.steps) / 1e9 / HBM_PEAK_GBS,
+            "achieved": (sum(contract_bytes) + sum(contract_shadow_bytes)) / (wall / args.steps) / 1e9, "unit": "GB/s",
+            "frac": (sum(contract_bytes) + sum(contract_shadow_bytes)) / (wall / args.steps) / 1e9 / HBM_PEAK_GBS,
+            "own_tree_frac": (sum(stat_bytes) + sum(shadow_bytes)) / (wall / args.steps) / 1e9 / HBM_PEAK_GBS,
             "rule": "algorithmic bytes of a frame's closest-hit and shadow traversals over the measured frame "
                     "time (frames overlapping as configured): the traversal kernels' combined rate"},
         "per_bounce": table,
@@ -496,9 +540,14 @@ def run_pt(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_trace (closest hit)",
-                     "algorithmic_bytes_per_launch": sum(stat_bytes) / len(stat_bytes),
-                     "bytes_per_ray": bytes_total / max(rays_eager, 1),
-                     "visits_per_ray": [float(x) / sum(counts) for x in np.sum(visits, 0)]},
+                     "algorithmic_bytes_per_launch": sum(contract_bytes) / len(contract_bytes),
+                     "bytes_per_ray": contract_total / max(rays_eager, 1),
+                     "bytes_rule": BYTES_RULE,
+                     "bvh2_visits_per_ray": [float(x) / sum(counts) for x in np.sum(bvh2_vis, 0)],
+                     "own_tree": {"achieved": own_achieved, "frac": own_achieved / HBM_PEAK_GBS,
+                                  "algorithmic_bytes_per_launch": sum(stat_bytes) / len(stat_bytes),
+                                  "bytes_per_ray": bytes_total / max(rays_eager, 1),
+                                  "visits_per_ray": [float(x) / sum(counts) for x in np.sum(visits, 0)]}},
     }
     if not args.no_serving:
         # the metric's second half: served vs plain per-request throughput and p99 latency on the
@@ -560,7 +609,9 @@ def run_c1(args):
     st.sync()
     wall = time.perf_counter() - t0
     t_closest = sum(a.elapsed_ms(b) for a, b, c in ev)
-    achieved = ray_bytes(vis_p, n, node_bytes) * args.steps / (t_closest * 1e-3) / 1e9
+    vis_p2 = bvh2_visits(sc, prim)
+    achieved = ray_bytes(vis_p2, n, 64) * args.steps / (t_closest * 1e-3) / 1e9
+    own_achieved = ray_bytes(vis_p, n, node_bytes) * args.steps / (t_closest * 1e-3) / 1e9
     workload = "c1-65536-random-tris-binned-sah-1M-primary-1M-occlusion"
     traffic, traffic_src = measured_traffic(workload, kernel_label(cfg))
     res = {
@@ -572,8 +623,12 @@ def run_c1(args):
         "closest_mrays_s": n * args.steps / (t_closest * 1e-3) / 1e6,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": "k_trace (closest hit)", "algorithmic_bytes_per_launch": ray_bytes(vis_p, n, node_bytes),
-                     "bytes_per_ray": ray_bytes(vis_p, n, node_bytes) / n},
+                     "kernel": "k_trace (closest hit)", "algorithmic_bytes_per_launch": ray_bytes(vis_p2, n, 64),
+                     "bytes_per_ray": ray_bytes(vis_p2, n, 64) / n, "bytes_rule": BYTES_RULE,
+                     "bvh2_visits_per_ray": [float(x) / n for x in vis_p2],
+                     "own_tree": {"achieved": own_achieved, "frac": own_achieved / HBM_PEAK_GBS,
+                                  "bytes_per_ray": ray_bytes(vis_p, n, node_bytes) / n,
+                                  "visits_per_ray": [float(x) / n for x in vis_p]}},
     }
     # the scene (a few MB) is served from L2 / Infinity Cache: the same figure against their
     # measured read bandwidth
@@ -642,11 +697,15 @@ def run_c4(args):
         log("batch autotune: %s, %.2f ms (tried, ms: %s)" % (cfg_dict(cfg), best[0], ", ".join(tried)))
     # visit counts per queue (untimed)
     cb = N.DeviceBuffer(per_queue * 12)
-    bytes_step = 0.0
+    bytes_step, own_bytes_step = 0.0, 0.0
     node_bytes = node_bytes_of(cfg)
+    bvh2 = N.TraceConfig(*BVH2_CFG_ARGS)
     for sc, rb, hb, n, _, _ in [q[:6] for q in queues]:
         sc.trace_stats(rb, hb, cb, n, cfg=cfg)
-        bytes_step += ray_bytes(cb.download(np.uint32, 3 * n).reshape(-1, 3).astype(np.int64).sum(0), n, node_bytes)
+        own_bytes_step += ray_bytes(cb.download(np.uint32, 3 * n).reshape(-1, 3).astype(np.int64).sum(0), n,
+                                    node_bytes)
+        sc.trace_stats(rb, hb, cb, n, cfg=bvh2)
+        bytes_step += ray_bytes(cb.download(np.uint32, 3 * n).reshape(-1, 3).astype(np.int64).sum(0), n, 64)
 
     def per_queue_launches():
         for sc, rb, hb, n, _, _ in [q[:6] for q in queues]:
@@ -681,6 +740,7 @@ def run_c4(args):
     wall_q, gpu_q = timed(per_queue_launches)
     rays_step = int(q_sizes.sum())
     achieved = bytes_step * args.steps / (gpu_b * 1e-3) / 1e9
+    own_achieved = own_bytes_step * args.steps / (gpu_b * 1e-3) / 1e9
     c4_traffic, c4_traffic_src = measured_traffic("c4-256-queues-%d-rays-8-scenes" % rays_step, "k_trace_batch")
     res = {
         "metric": METRIC, "value": rays_step * args.steps / wall_b / 1e6, "unit": "Mrays/s", "n_gpus": 1,
@@ -696,7 +756,10 @@ def run_c4(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": c4_traffic, "traffic_source": c4_traffic_src,
                      "kernel": "k_trace_batch (closest hit)",
-                     "algorithmic_bytes_per_launch": bytes_step, "bytes_per_ray": bytes_step / rays_step},
+                     "algorithmic_bytes_per_launch": bytes_step, "bytes_per_ray": bytes_step / rays_step,
+                     "bytes_rule": BYTES_RULE,
+                     "own_tree": {"achieved": own_achieved, "frac": own_achieved / HBM_PEAK_GBS,
+                                  "bytes_per_ray": own_bytes_step / rays_step}},
     }
     if args.c4_bounces > 0:
         res.update(run_c4_loop(args, builds, scenes, queues, cfg, rays_step))
