@@ -269,8 +269,9 @@ class PathTracer:
 
     def __init__(self, scene: SphereGridScene, width: int, height: int, spp: int = 1, max_path_len: int = 8,
                  rr_start: int = 3, seed_base: int = 0x1234567, trace_cfg: N.TraceConfig | None = None,
-                 shadow_cfg: N.TraceConfig | None = None, gpu_scene: S.Scene | None = None):
-        """gpu_scene: an uploaded scene to share (another frame loop over the same scene)."""
+                 shadow_cfg: N.TraceConfig | None = None, gpu_scene: S.Scene | None = None, camera=None):
+        """gpu_scene: an uploaded scene to share (another frame loop over the same scene);
+        camera: (position, direction, up, fov in degrees), else the scene's."""
         self.s = scene
         self.gpu_scene = gpu_scene if gpu_scene is not None else S.Scene(scene.desc)
         self.w, self.h, self.spp = width, height, spp
@@ -280,7 +281,7 @@ class PathTracer:
         self.shadow_cfg = shadow_cfg
         self.bounce_cfg = None  # optional per-bounce launch configurations (see cfg_of)
         self.shadow_bounce_cfg = None
-        pos, d, up, fov = scene.camera
+        pos, d, up, fov = camera if camera is not None else scene.camera
         self.cam_pos = np.asarray(pos, np.float32)
         self.r2w = np.zeros(16, np.float32)
         dd, uu = np.asarray(d, np.float32), np.asarray(up, np.float32)
@@ -517,7 +518,8 @@ class PhotonMapper:
     merge="fused" replaces the last two steps by one launch that keeps each query's list in its
     wave and reads the photons' directions and states from records gathered into grid slot order
     when the grid is built (imba_ppm_merge_records / imba_ppm_merge_grid); query() still writes
-    the index lists on request.
+    the index lists on request.  camera: (position, direction, up, fov in degrees), else the
+    scene's.
 
     Every traversal, producer and compaction runs on the device with queue sizes kept on the
     device; the host reads one number per frame (the photon count the grid build needs)."""
@@ -525,7 +527,7 @@ class PhotonMapper:
     def __init__(self, scene: SphereGridScene, width: int, height: int, light_paths: int, max_depth: int = 6,
                  rr_start: int = 3, radius_factor: float = 0.005, k: int = 10, spp: int = 1,
                  seed_base: int = 0x1234567, light_seed: int = 0x7654321, trace_cfg: N.TraceConfig | None = None,
-                 merge: str = "lists"):
+                 merge: str = "lists", camera=None):
         from .scene import HashGrid
 
         if merge not in ("lists", "fused"):
@@ -544,7 +546,7 @@ class PhotonMapper:
         self.cfg = trace_cfg
         lo, hi = np.asarray(scene.bounds[0], np.float64), np.asarray(scene.bounds[1], np.float64)
         self.radius = float(np.float32(radius_factor * np.linalg.norm(hi - lo)))
-        pos, d, up, fov = scene.camera
+        pos, d, up, fov = camera if camera is not None else scene.camera
         self.cam_pos = np.asarray(pos, np.float32)
         self.r2w = np.zeros(16, np.float32)
         dd, uu = np.asarray(d, np.float32), np.asarray(up, np.float32)  # alive across the call

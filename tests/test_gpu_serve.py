@@ -147,3 +147,34 @@ def test_served_vcm_request_is_the_vcm_iteration(gpu, scenes):
     lit = np.abs(want) > 1e-6
     assert lit.mean() > 0.2
     assert np.mean(np.abs(got[lit] - want[lit]) / np.abs(want[lit])) <= 1e-4
+
+
+def test_served_pt_and_ppm_requests_are_the_renderers_frames(gpu, scenes):
+    """A served PT request renders the frame workloads.PathTracer renders, and a served
+    photon-mapping request the frame workloads.PhotonMapper renders, with the same camera,
+    seeds and settings (both renderers' steps are restated against the oracle in
+    tests/test_gpu_pt_image.py and tests/test_gpu_photon_bounces.py)."""
+    reqs = _mixed_requests()
+    pt_req = next(r for r in reqs if r.integrator == "pt")
+    ppm_req = next(r for r in reqs if r.integrator == "ppm")
+    srv = V.Server(scenes, 512 << 20, batched=False)
+    done, img = srv.run([pt_req, ppm_req], keep_images=True)
+    assert len(done) == 2
+    for req in (pt_req, ppm_req):
+        sc = scenes[req.scene]
+        cam = (req.cam_pos, req.cam_dir, (0.0, 1.0, 0.0), 60.0)
+        if req.integrator == "pt":
+            r = W.PathTracer(sc.s, req.width, req.height, spp=req.spp, max_path_len=srv.max_path_len,
+                             rr_start=srv.rr_start, seed_base=req.seed, gpu_scene=sc.gpu, camera=cam)
+        else:
+            r = W.PhotonMapper(sc.s, req.width, req.height, light_paths=req.width * req.height, max_depth=V.PPM_DEPTH,
+                               rr_start=srv.rr_start, radius_factor=0.005, k=V.PPM_K, spp=req.spp, seed_base=req.seed,
+                               light_seed=(req.seed ^ 0x5A5A5A5A) & 0xFFFFFFFF, camera=cam)
+            assert r.radius == sc.radius
+        r.frame()
+        r.stream.sync()
+        want = r.image.download(np.float32, req.width * req.height * 3)
+        got = img[req.rid]
+        lit = np.abs(want) > 1e-6
+        assert lit.sum() > 50, req.integrator
+        assert np.mean(np.abs(got[lit] - want[lit]) / np.abs(want[lit])) <= 1e-4, req.integrator
