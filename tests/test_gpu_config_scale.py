@@ -8,13 +8,16 @@
   C3  photon mapping over the same grid with 30% mirror / glass spheres and 921,600 light
       paths: the hash grid over the frame's ~4.2M photons (table size, bounds, cell ends, slot
       order) equal to the oracle grid's, a 20,000-query sample bit for bit, and a sample of the
-      first light queue's hits against the oracle.
+      first light queue's hits against the oracle; the VCM iteration on the same scene: the
+      light vertex cache, its grid and merge records, 20,000 grid queries and a light-queue
+      sample, each against its restatement or the oracle.
   C4  256 queues (1K..1M rays, log-uniform, ~37M rays; half pinhole, half cosine-hemisphere)
       over 8 random-triangle scenes of 10K..4M triangles in one batched launch: a 2,048-ray
       sample of every queue against the oracle.
   C5  requests drawn from the serving mix over the 8 sphere-grid scenes (20K..21M triangles):
       every request's counts and image alone and in the mix.
 """
+import ctypes as C
 import os
 
 import numpy as np
@@ -133,6 +136,77 @@ def test_c3_photon_frame_full_size(grid21m):
     assert bad == 0
     assert _same(pm.gpu_scene.trace_host(rays0[idx], cfg=QUANTIZED), want)
     assert pm.gpu_scene.error_count() == 0
+
+
+def test_c3_vcm_iteration_full_size(grid21m):
+    """One VCM iteration at the configuration's size: the light vertex cache is the stable
+    compaction of every light bounce's vertex slots, the grid over it (table, bounds, cell ends,
+    slot order) is the oracle grid's, the merge records are the cache in that order, 20,000 grid
+    queries around cached vertices return the oracle's lists bit for bit, and the first light
+    queue's hits are the oracle's on a sample."""
+    from imbatracer_amd import vcm as VC
+
+    scene, arrays = grid21m
+    c3 = scene.with_specular(0.3)
+    print("C3: VCM iteration, 921,600 light paths", flush=True)
+    vr = VC.VcmRenderer(c3, 1280, 720, light_paths=921_600, max_path_len=6, light_max_depth=6, rr_start=0,
+                        radius_factor=0.005, k=10, trace_cfg=QUANTIZED, shadow_cfg=QUANTIZED)
+    light, cam, nv = vr.frame()
+    assert light[0] == 921_600 and cam[0] == 921_600
+    assert 3_000_000 < nv < 6_000_000
+    a, b = vr.cache_host()
+    # the cache is the occupied slots in (bounce, path) order
+    sa = vr.slot_a.download(VC.VTX_A_DTYPE, vr.slots)
+    sb = vr.slot_b.download(VC.VTX_B_DTYPE, vr.slots)
+    keep = sb["key"] >= 0
+    assert keep.sum() == nv
+    assert np.array_equal(a.view(np.uint8), sa[keep].view(np.uint8))
+    assert np.array_equal(b.view(np.uint8), sb[keep].view(np.uint8))
+    del sa, sb
+    # grid and merge records
+    og = oracle.Grid()
+    og.build(np.ascontiguousarray(a["pos"]), vr.radius)
+    info = vr.grid.info()
+    assert info.photon_count == nv and info.table_size == og.g.table_size
+    assert list(info.bbox_min) == list(og.g.bbox_min) and list(info.bbox_max) == list(og.g.bbox_max)
+    cell_ends = np.empty(info.table_size, np.int32)
+    order = np.empty(nv, np.int32)
+    N.check(N.lib.imba_memcpy_d2h(N.ptr(cell_ends), info.cell_ends, cell_ends.nbytes, None), "d2h")
+    N.check(N.lib.imba_memcpy_d2h(N.ptr(order), info.order, order.nbytes, None), "d2h")
+    assert np.array_equal(cell_ends, og.cell_ends())
+    assert np.array_equal(order, og.order())
+    rec = vr.records_host()
+    assert np.array_equal(rec["out"], a["out"][order]) and np.array_equal(rec["tp"], b["tp"][order])
+    assert np.array_equal(rec["dVCM"].view(np.int32), b["dVCM"][order].view(np.int32))
+    assert np.array_equal(rec["dVM"].view(np.int32), b["dVM"][order].view(np.int32))
+    # grid queries around cached vertices (merge queries sit on surfaces among them)
+    rng = np.random.default_rng(5)
+    pick = rng.choice(nv, size=20_000, replace=False)
+    qpos = (a["pos"][pick] + rng.uniform(-0.5, 0.5, (20_000, 3)).astype(np.float32) * np.float32(vr.radius))
+    qpos = np.ascontiguousarray(qpos, np.float32)
+    qb = N.DeviceBuffer.from_array(qpos)
+    oi, od, oc = N.DeviceBuffer(20_000 * 40), N.DeviceBuffer(20_000 * 40), N.DeviceBuffer(20_000 * 4)
+    vr.grid.query(qb, 12, 0, 20_000, vr.k, oi, oc, od)
+    g_idx = oi.download(np.int32, 20_000 * vr.k).reshape(-1, vr.k)
+    g_dist = od.download(np.float32, 20_000 * vr.k).reshape(-1, vr.k)
+    g_cnt = oc.download(np.int32, 20_000)
+    o_idx, o_dist, o_cnt = og.query_many(qpos, vr.k)
+    assert np.array_equal(g_cnt, o_cnt) and (o_cnt == vr.k).mean() > 0.2
+    for r in range(20_000):
+        c = g_cnt[r]
+        assert np.array_equal(g_idx[r, :c], o_idx[r, :c]), r
+        assert np.array_equal(g_dist[r, :c].view(np.int32), o_dist[r, :c].view(np.int32)), r
+    # the image and the first light queue
+    img = vr.image_host()
+    assert np.all(np.isfinite(img)) and np.all(img >= 0) and (img.sum(-1) > 0).mean() > 0.5
+    lr, ls = N.DeviceBuffer(vr.P * 32), N.DeviceBuffer(vr.P * VC.VCM_STATE_DTYPE.itemsize)
+    N.check(N.lib.imba_vcm_light(C.byref(vr.frame_args), vr.light_seed, 0, vr.P, lr.ptr, ls.ptr, None), "vcm_light")
+    rays0 = lr.download(N.RAY_DTYPE, vr.P)
+    idx = _every_kth(vr.P, 65536)
+    want, bad = oracle.trace(arrays, rays0[idx], threads=THREADS)
+    assert bad == 0
+    assert _same(vr.gpu_scene.trace_host(rays0[idx], cfg=QUANTIZED), want)
+    assert vr.gpu_scene.error_count() == 0
 
 
 def test_c4_batched_queues_full_size(gpu):
