@@ -388,6 +388,12 @@ int imba_event_sync(void* event) {
     return IMBA_OK;
 }
 
+int imba_stream_wait_event(void* stream, void* event) {
+    IMBA_CHECK_ARG(stream && event, "NULL argument");
+    IMBA_HIP_TRY(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0));
+    return IMBA_OK;
+}
+
 int imba_event_elapsed_ms(float* ms, void* start, void* end) {
     IMBA_CHECK_ARG(ms && start && end, "NULL argument");
     IMBA_HIP_TRY(hipEventSynchronize((hipEvent_t)end));

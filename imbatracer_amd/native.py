@@ -147,6 +147,7 @@ _sig("imba_event_destroy", i32, vp)
 _sig("imba_event_record", i32, vp, vp)
 _sig("imba_event_elapsed_ms", i32, C.POINTER(f32), vp, vp)
 _sig("imba_event_sync", i32, vp)
+_sig("imba_stream_wait_event", i32, vp, vp)
 _sig("imba_build_create", i32, C.POINTER(vp))
 _sig("imba_build_release", i32, vp)
 _sig("imba_build_add_mesh", i32, vp, C.POINTER(MeshDesc), ip)
@@ -325,6 +326,10 @@ class Stream:
 
     def sync(self):
         check(lib.imba_stream_sync(self.handle), "imba_stream_sync")
+
+    def wait(self, event: "Event"):
+        """Work enqueued on this stream from now on waits for the work recorded before event."""
+        check(lib.imba_stream_wait_event(self.handle, event.handle), "imba_stream_wait_event")
 
     def __del__(self):
         try:

@@ -508,6 +508,10 @@ class Server:
     # A queue at least this long fills the GPU by itself: it gets its own k_trace launch in a
     # batched step, and the shorter queues share one batched launch.
     OWN_LAUNCH_RAYS = 131072
+    # The phased requests' own kernels (grid builds, VCM bounce, merge and compaction, photon
+    # mapping bounces) run on this many side streams in a batched step, so that the small
+    # launches of different requests overlap on the device.
+    SIDE_STREAMS = 4
 
     def __init__(self, scenes: list, arena_bytes: int, max_active: int = 32, batched: bool = True,
                  max_path_len: int = 8, rr_start: int = 3, cfg: N.TraceConfig | None = None,
@@ -546,6 +550,28 @@ class Server:
         # work queued; the rest it waited for the host
         self._ev = (N.Event(), N.Event())
         self.gpu_ms = 0.0
+        self.side = [N.Stream() for _ in range(self.SIDE_STREAMS)] if batched else []
+        self._fork_ev = N.Event()
+        self._join_ev = [N.Event() for _ in self.side]
+        self._forked = 0
+
+    def _fork(self, jobs) -> dict:
+        """Streams for the per-request work of `jobs` (job id -> stream handle): on the side
+        streams, round robin, behind the main stream's work so far (plain path: the main
+        stream).  _join() makes the main stream wait for that work."""
+        if not self.side or not jobs:
+            return {id(j): self.stream.handle for j in jobs}
+        self._fork_ev.record(self.stream)
+        self._forked = min(len(jobs), len(self.side))
+        for s in self.side[:self._forked]:
+            s.wait(self._fork_ev)
+        return {id(j): self.side[k % len(self.side)].handle for k, j in enumerate(jobs)}
+
+    def _join(self):
+        for k in range(self._forked):
+            self._join_ev[k].record(self.side[k])
+            self.stream.wait(self._join_ev[k])
+        self._forked = 0
 
     def _shade_item(self, j) -> N.ShadeItem:
         return N.ShadeItem(j.rays[j.cur].value, j.hits.value, j.states[j.cur].value, j.n,
@@ -616,8 +642,11 @@ class Server:
         st = self.stream.handle
         pt_jobs = [j for j in self.active if isinstance(j, _Job)]
         ppm_jobs = [j for j in self.active if isinstance(j, (_PPMJob, _VCMJob))]  # phased requests
+        grid_jobs = [j for j in ppm_jobs if j.phase in ("grid", "merge")]
+        js = self._fork(grid_jobs)
         for j in ppm_jobs:
-            j.before_trace(self, st)
+            j.before_trace(self, js.get(id(j), st))
+        self._join()
         items = [(j.sc.gpu, j.rays[j.cur], j.hits, j.n, self.counts_ptr(j.slot, j.bounce), False, j.queued)
                  for j in pt_jobs] + [j.trace_item(self) + (j.queued,) for j in ppm_jobs if j.tracing()]
         own = [it for it in items if not self.batched or it[6] >= self.OWN_LAUNCH_RAYS]
@@ -637,9 +666,11 @@ class Server:
             self.shade.launch(self.stream)
             sitems += [(j.sc.gpu, j.shadow_rays, j.shadow_hits, j.n, self.counts_ptr(j.slot, j.bounce), True, j.queued)
                        for j in pt_jobs]
+        js = self._fork(vcm_steps)
         for j in vcm_steps:
-            j.bounce_step(self, st)
+            j.bounce_step(self, js[id(j)])
             sitems.append(j.shadow_item(self))
+        self._join()
         # every shadow queue of the step (PT next event, VCM camera connections / next event /
         # vertex connections): long ones in their own launch, the rest in one batched launch
         for sc, rays, hits, cap, cnt, _, q in sitems:
@@ -666,11 +697,13 @@ class Server:
                                                     j.rays[1 - j.cur], j.states[1 - j.cur],
                                                     self.counts_ptr(j.slot, j.bounce + 1), j.workspace, st),
                         "compact_dev")
+        tail = vcm_steps + [j for j in ppm_jobs if isinstance(j, _PPMJob) and j.tracing()]
+        js = self._fork(tail)
         for j in vcm_steps:
-            j.finish_step(self, st)
-        for j in ppm_jobs:
-            if isinstance(j, _PPMJob) and j.tracing():
-                j.after_trace(self, st)
+            j.finish_step(self, js[id(j)])
+        for j in tail[len(vcm_steps):]:
+            j.after_trace(self, js[id(j)])
+        self._join()
         for j in pt_jobs:
             j.bounce += 1
             j.cur = 1 - j.cur

@@ -110,6 +110,9 @@ int imba_event_destroy(void* event);
 int imba_event_record(void* event, void* stream);
 int imba_event_elapsed_ms(float* ms, void* start, void* end);
 int imba_event_sync(void* event);  /* waits until the work before the event's record is done */
+/* Work enqueued on stream after this call waits for the work recorded before event (a device
+ * side wait: the host does not block). */
+int imba_stream_wait_event(void* stream, void* event);
 
 /* ------------------------------ host BVH construction ------------------------------------
  * Restatement of the reference builders and GPU layout writers:
