@@ -614,7 +614,7 @@ __device__ __forceinline__ void write_result(const TraceArgs& a, const TraceStat
     if (s.flags & kBad) atomicAdd(a.err, 1);
 }
 
-// One ray per thread, rays strided over a grid sized to the resident waves.
+// One ray per thread (rays strided over the grid when it is smaller than the ray count).
 // The 4-wide variants are held to 7 waves per SIMD (72 VGPRs, a few bytes of scratch per
 // lane): faster than their natural 6 (77 VGPRs; PT frames 4% faster with the any-hit variant
 // held too) and, for closest hit, than 8 (64 VGPRs, more spills: 9% slower).

@@ -282,7 +282,9 @@ typedef struct ImbaTraceConfig {
     int block_size;        /* threads per block: 64..1024, multiple of 64 (0 = default)    */
     int stack_lds;         /* per-lane stack entries kept in LDS (0 = default)             */
     int persistent;        /* 1: persistent waves fetching rays with a global counter      */
-    int blocks_per_cu;     /* persistent grid size per CU (0 = default)                    */
+    int blocks_per_cu;     /* grid of this many blocks per CU, threads striding over the
+                              rays (0 = default: persistent launches fill the resident
+                              blocks, others launch one thread per ray)                 */
     int bvh_width;         /* 2: the scene's BVH2 as given; 4: the 4-wide tree derived from
                               it at upload (same leaves); 0 = default                      */
     int quantized;         /* with bvh_width 4: 64-byte nodes whose child boxes are 8-bit
