@@ -219,7 +219,7 @@ def frame_tune(pt, queues, frames=3):
              if 0 < b < len(queues) and len(queues[b])]
     cands += [N.TraceConfig(256, 16, 0, 0, 4, 1, 0), N.TraceConfig(256, 16, 0, 0, 4, 1, 64),
               N.TraceConfig(128, 16, 0, 0, 4, 1, 0), N.TraceConfig(64, 32, 0, 0, 4, 1, 0),
-              N.TraceConfig(256, 16, 32, 0, 4, 1, 0)]
+              N.TraceConfig(256, 16, 32, 0, 4, 1, 0), N.TraceConfig(256, 16, 8, 0, 4, 1, 0)]
     seen, timed = set(), []
     for c in cands:
         key = tuple(getattr(c, f) for f, _ in c._fields_)
