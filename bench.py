@@ -630,6 +630,20 @@ def run_c1(args):
                                   "bytes_per_ray": ray_bytes(vis_p, n, node_bytes) / n,
                                   "visits_per_ray": [float(x) / n for x in vis_p]}},
     }
+    # the PCIe-inclusive path (RayQueue::traverse_gpu, ray_queue.h:269-293: host rays copied in,
+    # hits copied out, per call, from pageable host memory): reported beside `value`, never as it
+    host_hits = np.empty(n, N.HIT_DTYPE)
+    st.sync()
+    t_pcie = time.perf_counter()
+    for _ in range(args.steps):
+        N.check(N.lib.imba_memcpy_h2d(rb.ptr, N.ptr(prim), prim.nbytes, st.handle), "h2d")
+        sc.intersect(rb, hb, n, cfg, st.handle)
+        N.check(N.lib.imba_memcpy_d2h(N.ptr(host_hits), hb.ptr, host_hits.nbytes, st.handle), "d2h")
+        st.sync()
+    t_pcie = time.perf_counter() - t_pcie
+    res["pcie_inclusive"] = {"closest_mrays_s": n * args.steps / t_pcie / 1e6,
+                             "rule": "primary rays copied from pageable host memory, traversed, hits copied back, "
+                                     "one call at a time (the reference's RayQueue::traverse_gpu)"}
     # the scene (a few MB) is served from L2 / Infinity Cache: the same figure against their
     # measured read bandwidth
     bw = read_bandwidths(st)
