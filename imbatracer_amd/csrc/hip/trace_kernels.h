@@ -621,11 +621,14 @@ __device__ __forceinline__ void write_result(const TraceArgs& a, const TraceStat
 template <int BLOCK, int STACK_LDS, bool ANY, bool MASKED, bool STATS, int TREE>
 __global__ __launch_bounds__(BLOCK, (MASKED || STATS || TREE == 0) ? 1 : 7) void k_trace(TraceArgs a) {
     __shared__ int lds_stack[STACK_LDS * BLOCK];
+    const int count = launch_count(a);
+    // a block past the rays (a launch sized for the capacity of a queue whose size is read on
+    // the device) leaves before it copies the treelet: block-uniform, before any barrier
+    if ((int)(blockIdx.x * BLOCK) >= count) return;
     LaneStack<BLOCK, STACK_LDS> st;
     stack_init(st, lds_stack, a);
     treelet_init<BLOCK, STACK_LDS, TREE>(a, st);
     const int stride = gridDim.x * BLOCK;
-    const int count = launch_count(a);
     for (int id = blockIdx.x * BLOCK + threadIdx.x; id < count; id += stride) {
         TraceState s;
         state_init(s, a.rays[2 * (size_t)id], a.rays[2 * (size_t)id + 1], a.root);
