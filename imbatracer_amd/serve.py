@@ -142,8 +142,9 @@ def footprint(req: Request) -> int:
     shadow slots (ray, state, hit), image and the compaction workspace.  PPM: two ray and state
     buffers and hits for max(light paths, samples) entries, the photon slots and the compacted
     photons, the merge queries and their results, image and workspace.  VCM: two ray and state
-    buffers, hits and the shadow slots for max(light paths, samples) entries, the light vertex
-    slots, the cache and its merge records, the merge queries, image and workspace."""
+    buffers, hits and the shadow slots (and their compacted copies) for max(light paths,
+    samples) entries, the light vertex slots, the cache and its merge records, the merge queries,
+    image and workspace."""
     n = req.samples
     if req.integrator == "pt":
         ws = int(N.lib.imba_compact_workspace_size(n))
@@ -153,9 +154,10 @@ def footprint(req: Request) -> int:
     cap = max(p, n)
     if req.integrator == "vcm":
         slots = VCM_DEPTH * p
-        ws = int(N.lib.imba_compact_workspace_size(max(slots, cap)))
-        return (2 * 32 * cap + 2 * VC.VCM_STATE_DTYPE.itemsize * cap + 16 * cap + (1 + VCM_C) * cap * (32 + 16 + 16) +
-                (4 * 32 + 32) * slots + 64 * cap + 12 * req.width * req.height + ws + 24 * ALIGN)
+        ws = int(N.lib.imba_compact_workspace_size(max(slots, (1 + VCM_C) * cap)))
+        return (2 * 32 * cap + 2 * VC.VCM_STATE_DTYPE.itemsize * cap + 16 * cap +
+                (1 + VCM_C) * cap * (32 + 16 + 16 + 32 + 16) + (4 * 32 + 32) * slots + 64 * cap +
+                12 * req.width * req.height + ws + 24 * ALIGN)
     slots = PPM_DEPTH * p
     ws = int(N.lib.imba_compact_workspace_size(max(slots, cap)))
     return (2 * 32 * cap + 2 * PPM_STATE_BYTES * cap + 16 * cap + 2 * 48 * slots + 64 * n + 8 * PPM_K * n + 4 * n +
@@ -367,12 +369,14 @@ class _VCMJob:
         self.hits = take(16 * cap)
         ns = (1 + VCM_C) * cap
         self.shadow_rays, self.shadow_states, self.shadow_hits = take(32 * ns), take(16 * ns), take(16 * ns)
+        # the occupied shadow slots, stable-compacted before their traversal (as VcmRenderer does)
+        self.shadow_rays_c, self.shadow_states_c = take(32 * ns), take(16 * ns)
         self.slot_a, self.slot_b = take(32 * slots), take(32 * slots)
         self.vtx_a, self.vtx_b = take(32 * slots), take(32 * slots)
         self.records = take(32 * slots)
         self.queries = take(64 * cap)
         self.image = take(12 * req.width * req.height)
-        self.workspace = take(int(N.lib.imba_compact_workspace_size(max(slots, cap))))
+        self.workspace = take(int(N.lib.imba_compact_workspace_size(max(slots, ns))))
         self.grid = None  # taken from the server's pool when the cache size is known
         self.phase = "light"
         self.bounce = 0
@@ -397,6 +401,9 @@ class _VCMJob:
 
     def _shadow(self, srv):
         return srv.counts_ptr(self.slot, 2 * srv.L1 + 2)
+
+    def _live_shadow(self, srv):
+        return srv.counts_ptr(self.slot, 2 * srv.L1 + 3)
 
     def start(self, srv, st):
         c0 = np.zeros(srv.cs, np.int32)
@@ -444,15 +451,20 @@ class _VCMJob:
                                                  self._camera(srv, b), self.vtx_a, self.vtx_b, self._cache(srv),
                                                  self.rays[cur], self.shadow_rays, self.shadow_states,
                                                  self._shadow(srv), self.queries, self.image, st), "vcm_camera_bounce")
+        cap, cnt = (self.P, self._light(srv, b)) if self.phase == "light" else \
+            ((1 + VCM_C) * self.n, self._shadow(srv))
+        N.check(N.lib.imba_compact_rays_dev(self.shadow_rays, self.shadow_states, 16, cap, cnt, self.shadow_rays_c,
+                                            self.shadow_states_c, self._live_shadow(srv), self.workspace, st),
+                "compact shadows")
 
     def shadow_item(self, srv):
         """The bounce's shadow rays as a traversal item (any hit), with the estimate of their
         number the server uses to choose an own launch."""
         if self.phase == "light":
-            return (self.sc.gpu, self.shadow_rays, self.shadow_hits, self.P, self._light(srv, self.bounce), True,
+            return (self.sc.gpu, self.shadow_rays_c, self.shadow_hits, self.P, self._live_shadow(srv), True,
                     self.queued)
         ns = (1 + VCM_C) * self.n
-        return (self.sc.gpu, self.shadow_rays, self.shadow_hits, ns, self._shadow(srv), True,
+        return (self.sc.gpu, self.shadow_rays_c, self.shadow_hits, ns, self._live_shadow(srv), True,
                 (1 + VCM_C) * self.queued)
 
     def finish_step(self, srv, st):
@@ -462,8 +474,8 @@ class _VCMJob:
         sb = VC.VCM_STATE_DTYPE.itemsize
         if self.phase == "light":
             cnt = self._light(srv, b)
-            N.check(N.lib.imba_pt_shadow(self.shadow_hits, self.shadow_states, self.P, cnt, self.image, st),
-                    "pt_shadow")
+            N.check(N.lib.imba_pt_shadow(self.shadow_hits, self.shadow_states_c, self.P, self._live_shadow(srv),
+                                         self.image, st), "pt_shadow")
             N.check(N.lib.imba_compact_rays_dev(self.rays[cur], self.states[cur], sb, self.P, cnt, self.rays[1 - cur],
                                                 self.states[1 - cur], self._light(srv, b + 1), self.workspace, st),
                     "compact")
@@ -476,8 +488,8 @@ class _VCMJob:
         elif self.phase == "camera":
             cnt = self._camera(srv, b)
             ns = (1 + VCM_C) * self.n
-            N.check(N.lib.imba_pt_shadow(self.shadow_hits, self.shadow_states, ns, self._shadow(srv), self.image, st),
-                    "pt_shadow")
+            N.check(N.lib.imba_pt_shadow(self.shadow_hits, self.shadow_states_c, ns, self._live_shadow(srv), self.image,
+                                         st), "pt_shadow")
             if self.vertices > 0:
                 N.check(N.lib.imba_vcm_merge(fa, self.grid.h, self.records, self.queries, self.n, cnt, VCM_K,
                                              self.image, st), "vcm_merge")
@@ -531,9 +543,9 @@ class Server:
         self.stream = N.Stream()
         # per request slot: PT bounce counts [0, L]; PPM / VCM light counts [0, L], camera
         # counts [L1, L1 + L], photon (vertex) slots at 2 L1, photons (cache size) at 2 L1 + 1,
-        # VCM camera shadow slots at 2 L1 + 2
+        # VCM camera shadow slots at 2 L1 + 2, VCM occupied shadow slots at 2 L1 + 3
         self.L1 = max(max_path_len, PPM_DEPTH, VCM_DEPTH) + 1
-        self.cs = 2 * self.L1 + 3
+        self.cs = 2 * self.L1 + 4
         self.counts = N.DeviceBuffer(4 * self.cs * self.max_active)
         self.free_slots = list(range(self.max_active))
         self.pending: list[Request] = []
